@@ -1,0 +1,166 @@
+/*
+ * mdn.h -- C ABI of libmdn, the MI355X (gfx950) per-step force and
+ * integration path of the effective-domain nucleation MD method of
+ * tamadate/MD_nucleation.
+ *
+ * Conventions: every entry point returns 0 on success and a negative
+ * mdn_status on failure (mdn_last_error() gives the message of the calling
+ * thread's last failure).  Host buffers are owned by the caller and copied;
+ * device memory is owned by the library.  One mdn_ctx per host thread; a
+ * context is not thread-safe.  Units follow the reference: A, fs, g/mol,
+ * kcal/mol, e.
+ *
+ * Reference interfaces replaced (paths relative to the reference root):
+ *   mdn_sysdef_load        MD::readCondFile (src/initialization/setCondition.cpp:5-254),
+ *                          Variables::readIonFile (readIonFile.cpp:4-180),
+ *                          Variables::readVaporFile (readVaporFile.cpp:4-131),
+ *                          Variables::setCrossPotentials (setCrossPotential.cpp:4-23),
+ *                          MD::setPotential (setPotential.cpp:7-111)
+ *   mdn_sysdef_init_state  Variables::ionInitial (variables.cpp:12-30),
+ *                          MD::initialization_gas (initializationGas.cpp:24-90),
+ *                          MD::initialization_vapor (initializationVapor.cpp:14-77)
+ *   mdn_make_pair          MD::make_pair (src/pairList/pairlist.cpp:16-49) with
+ *                          updateInCenters, update_gas_in, update_vapor_in and
+ *                          every Potential::makePair (basePotential.hpp:13)
+ *   mdn_compute            Potential::compute (basePotential.hpp:12) of the
+ *                          registered potentials, as called at verlet.cpp:31-32
+ *   mdn_velocity_calculation  MD::velocity_calculation (verlet.cpp:47-65)
+ *   mdn_update_position    MD::update_position (verlet.cpp:73-136) and
+ *                          MD::updatePositionSHAKE + MD::shake (shake.cpp:6-92)
+ *   mdn_verlet             MD::verlet (verlet.cpp:20-39) x nsteps, with the
+ *                          energy cadence of MD::run (run.cpp:68-71)
+ *   mdn_run_batched        the OpenMP replica fan-out of main (main.cpp:21-30)
+ *                          around MD::run's stepping loop, as one persistent
+ *                          launch over all replicas
+ */
+#ifndef MDN_H
+#define MDN_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum mdn_status {
+  MDN_OK = 0,
+  MDN_E_ARG = -1,       /* invalid argument */
+  MDN_E_IO = -2,        /* input file missing or malformed */
+  MDN_E_HIP = -3,       /* HIP runtime error */
+  MDN_E_CAPACITY = -4,  /* a list or pool exceeded its capacity */
+  MDN_E_SHAKE = -5,     /* SHAKE did not converge within its iteration cap */
+  MDN_E_STATE = -6      /* call out of order (e.g. before mdn_commit) */
+} mdn_status;
+
+/* interaction bits: the FLAG fields set by the Interactions block
+ * (flags.hpp:3-66, setCondition.cpp:186-228) */
+enum {
+  MDN_GI = 1 << 0,  MDN_VI = 1 << 1,  MDN_GG = 1 << 2,  MDN_GV = 1 << 3,
+  MDN_VV = 1 << 4,  MDN_ION_AMBER = 1 << 5, MDN_ION_BMH = 1 << 6,
+  MDN_ION_TERSOFF = 1 << 7, MDN_ION_SW = 1 << 8, MDN_EFIELD = 1 << 9,
+  MDN_VAPOR_INTRA = 1 << 10, MDN_GAS_INTRA = 1 << 11
+};
+/* energy buckets, order of struct Potentials (struct.hpp:21-30) */
+enum { MDN_U_ION = 0, MDN_U_GAS, MDN_U_VAP, MDN_U_GI, MDN_U_GG, MDN_U_VG,
+       MDN_U_VI, MDN_U_VV, MDN_U_N };
+/* precision of the pair and bonded arithmetic */
+enum { MDN_FP64 = 0, MDN_MIXED = 1 };
+
+typedef struct mdn_sysdef mdn_sysdef;
+typedef struct mdn_ctx mdn_ctx;
+
+/* Read-only view of a parsed system definition (arrays owned by the sysdef).
+ * Species 0 = ion, 1 = carrier gas, 2 = vapor.  Molecule order: ion, the
+ * n_gas gas molecules, the n_vap vapor molecules (group[0..2] of the
+ * reference).  Atom arrays hold every molecule's atoms in that order. */
+typedef struct mdn_system {
+  int32_t n_types;
+  const double *pair_coeff;  /* [n_types*n_types*2] = 48 e s^12, 24 e s^6 */
+  const double *type_eps, *type_sigma, *type_mass; /* [n_types] */
+  int32_t n_btypes; const double *btype;   /* [n*2] k, r0 */
+  int32_t n_ctypes; const double *ctype;   /* [n*2] k, theta0 (rad) */
+  int32_t n_dtypes; const int32_t *dmulti; const double *dcoeff; /* [n*15] */
+  int32_t natoms[3];
+  const double *mass[3], *charge[3];
+  const int32_t *type[3];
+  const double *tmpl_x[3];
+  int32_t nbond[3];  const int32_t *bond[3];
+  int32_t nangle[3]; const int32_t *angle[3];
+  int32_t ndih[3];   const int32_t *dih[3];
+  int32_t nrigid[3]; const int32_t *rigid[3];
+  double molmass[3];
+  int32_t n_gas, n_vap;
+  int32_t ion_type_base;
+  double dt, T, pressure, d_size, cutoff, margin, ML2;
+  double m_gas_kg, m_vap_kg;
+  int32_t gastype, gas_spin;
+  uint32_t flags;
+  double efield[3];
+  int32_t shake, shake_max_iter;
+  int64_t total_steps, relax_steps;
+  int32_t observe;
+  double cluster_radius;     /* >0: vapor pre-packed around the ion */
+} mdn_system;
+
+/* Caller-owned host state of one replica (sizes from mdn_sysdef_sizes). */
+typedef struct mdn_state {
+  double *mol_x, *mol_v;      /* [n_mol*3] molecule COM (vars->position/velocity) */
+  uint8_t *region;            /* [n_mol] */
+  double *atom_x, *atom_v, *atom_f; /* [n_atoms*3] */
+  double *atom_w;             /* [n_atoms] blend weights */
+  int32_t loopPair, loop_update;
+  int64_t rebuild_count;
+  int32_t cap_gi, cap_gv, cap_gg;
+  int32_t n_gi, n_gv, n_gg;
+  int32_t *gi, *gv, *gg;      /* [cap*2] pair lists (molecule, ion atom) etc. */
+  double U[MDN_U_N];
+} mdn_state;
+
+const char *mdn_last_error(void);
+
+/* ---- host front end (initialization stays host C++) ---- */
+int mdn_sysdef_load(const char *cond_path, mdn_sysdef **out);
+int mdn_sysdef_view(const mdn_sysdef *d, mdn_system *out);
+int mdn_sysdef_sizes(const mdn_sysdef *d, int32_t *n_mol, int32_t *n_atoms);
+int mdn_sysdef_set_shake(mdn_sysdef *d, int32_t shake, int32_t max_iter);
+int mdn_sysdef_set_flags(mdn_sysdef *d, uint32_t flags);
+int mdn_sysdef_init_state(const mdn_sysdef *d, uint64_t seed, mdn_state *st);
+void mdn_sysdef_free(mdn_sysdef *d);
+
+/* ---- device engine ---- */
+int mdn_create(int device, mdn_ctx **out);
+void mdn_destroy(mdn_ctx *c);
+/* adds a replica (copies sysdef tables and state); seed/replica_id key the
+ * counter-based random stream of its rebuilds */
+int mdn_add_replica(mdn_ctx *c, const mdn_sysdef *d, const mdn_state *st,
+                    uint64_t seed, uint32_t replica_id, int32_t *handle);
+/* allocates device memory for all replicas and uploads them */
+int mdn_commit(mdn_ctx *c);
+int mdn_set_state(mdn_ctx *c, int32_t replica, const mdn_state *st);
+int mdn_get_state(mdn_ctx *c, int32_t replica, mdn_state *st);
+
+/* drop-in path: one replica, FP64, one launch per reference call */
+int mdn_make_pair(mdn_ctx *c, int32_t replica);
+int mdn_zero_forces(mdn_ctx *c, int32_t replica);
+int mdn_compute(mdn_ctx *c, int32_t replica, uint32_t bits, int32_t eflag,
+                int32_t precision);
+int mdn_velocity_calculation(mdn_ctx *c, int32_t replica);
+int mdn_update_position(mdn_ctx *c, int32_t replica);
+int mdn_verlet(mdn_ctx *c, int32_t replica, int32_t nsteps,
+               int32_t eflag_every, int64_t step0, int32_t precision);
+
+/* batched path: all replicas, nsteps in one persistent launch */
+int mdn_run_batched(mdn_ctx *c, int32_t nsteps, int32_t eflag_every,
+                    int64_t step0, int32_t precision);
+
+/* counters: in-domain atom-steps since commit, rebuilds, max SHAKE iters */
+int mdn_get_counters(mdn_ctx *c, int32_t replica, int64_t *atom_steps,
+                     int64_t *rebuilds, int32_t *shake_iters_max);
+/* device synchronisation and timing of the last mdn_run_batched /
+ * mdn_verlet call (HIP events on the library's stream), milliseconds */
+int mdn_sync(mdn_ctx *c);
+int mdn_last_kernel_ms(mdn_ctx *c, double *ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

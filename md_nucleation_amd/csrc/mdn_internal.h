@@ -1,0 +1,59 @@
+// mdn_internal.h -- host-side types shared by the front end and the engine.
+#pragma once
+#include <array>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "mdn.h"
+
+namespace mdn {
+
+struct AtomType {
+  std::string name;
+  bool is_h = false;
+  double mass = 0, eps = 0, sigma = 0;
+};
+
+// One molecule species (ion, gas or vapor) with its local topology.
+struct Species {
+  int natoms = 0;
+  std::vector<double> mass, charge, tmpl_x;
+  std::vector<int> type;
+  std::vector<int> bond, angle, dih, rigid;  // 3, 4, 5, 3 ints per term
+  double molmass = 0;
+};
+
+// Parsed system definition: what readCondFile, readIonFile, readVaporFile,
+// setPhysicalProp and setCrossPotentials leave in Variables/MD/Physical.
+struct SysDef {
+  std::vector<AtomType> atypes;
+  std::vector<double> pair_coeff, t_eps, t_sig, t_mass;
+  std::vector<double> btype, ctype, dcoeff;
+  std::vector<int> dmulti;
+  Species sp[3];
+  int n_gas = 1, n_vap = 1, n_mol = 0, n_atoms = 0;
+  int ion_type_base = 0;
+  double dt = 0.5, T = 300, pressure = 1e5, d_size = 0, cutoff = 20, margin = 10, ML2 = 900;
+  double Mgas = 28.0, m_gas_kg = 0, m_vap_kg = 0;
+  int gastype = 2, gas_spin = 1;
+  bool has_gas_line = false, n_gas_default = false;
+  uint32_t flags = 0;
+  double efield[3] = {0, 0, 0};
+  int shake = 0, shake_max_iter = 1000;
+  int64_t total_steps = 0, relax_steps = 0;
+  int observe = 10000000;
+  std::string output_name, nvt_ion, nvt_gas;
+  double cluster_radius = 0, cluster_density = 0;
+
+  void finalize_views();
+  int mol_offset(int i) const;
+};
+
+void load_sysdef(SysDef &d, const std::string &cond);
+void init_state(const SysDef &d, uint64_t seed, mdn_state *st);
+
+int set_error(int code, const std::string &msg);
+
+}  // namespace mdn
