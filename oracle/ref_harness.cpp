@@ -22,6 +22,9 @@
 //                       atom_x and current positions = array x_new
 //   tersoff | sw        neighbour list by the potential's own make_pair, then
 //                       its compute() alone (forces are x23.06 in the reference)
+//   born BASE           PotentialBorn::compute alone on the ion, with the ion's
+//                       atom types shifted down by BASE first (its coefficient
+//                       table is indexed by ion-local type 0 = Na, 1 = Cl)
 //   rotation            Variables::ROTATION on rows of array rot_in
 //   time N              N verlet steps as the reference runs them (with its own
 //                       random rebuilds); prints wall time and in-domain
@@ -50,6 +53,7 @@
 // the reference's SW neighbour builder is private; the driver calls it
 #define private public
 #include "md.hpp"
+#include "potential/BMHFT/potentialBMHFT.hpp"
 #undef private
 
 typedef std::map<std::string, std::vector<double>> Arrays;
@@ -319,6 +323,18 @@ int main(int argc, char **argv) {
     std::vector<double> res;
     for (auto &p : v->positionAtom[mol]) res.insert(res.end(), p.begin(), p.end());
     put(out, "x_shake", res);
+  } else if (o == "born") {
+    Variables *v = md->vars;
+    for (auto &f : v->forceAtom[0]) f[0] = f[1] = f[2] = 0;
+    for (auto &t : v->typeAtom[0]) t -= (int)arg;
+    v->Uzero();
+    md->flags->eflag = 1;
+    PotentialBorn b;
+    b.compute(v, md->flags);
+    std::vector<double> fo;
+    for (auto &f : v->forceAtom[0]) fo.insert(fo.end(), f.begin(), f.end());
+    put(out, "ion_f", fo);
+    put(out, "U_ion", {v->Utotal.Uion});
   } else if (o == "tersoff" || o == "sw") {
     Variables *v = md->vars;
     for (auto &f : v->forceAtom[0]) f[0] = f[1] = f[2] = 0;
