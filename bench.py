@@ -1,0 +1,220 @@
+"""Benchmark: in-domain atom-steps/s of the batched mixed-precision path.
+
+Workload: 512 independent nucleation replicas in one persistent launch
+(ion drawn from {Na+, Cl-, 150-atom AMBER chain}, 10-400 methanol or water
+vapor molecules, 5,000 N2, 300 K, dt 1 fs), FP32 pair arithmetic with FP64
+accumulation and integration.  Beside it, on the same workload:
+  * the plain FP64 per-call GPU path (one replica after another, one launch
+    per reference call, no graph), timed on a sample of replicas;
+  * the CPU baseline: the FP64 oracle (a C restatement of the reference's
+    loops, one replica per OpenMP thread as the reference's main does),
+    timed on a bounded sample;
+  * the relative RMS force error of the mixed path against the FP64 path on
+    identical states.
+
+    python bench.py [--gpus 1] [--steps 10000] [--warmup 200]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "atom-steps/s: batched mixed-prec vs plain FP64 GPU at ≤1e-5 rel force err"
+PEAK_FP32_TFLOPS = 157.3
+PEAK_FP64_TFLOPS = 78.6
+# algorithmic FLOPs per unique pair / bonded term, as written in the reference
+FLOP = {"ljcoul_w": 31, "ljcoul": 30, "lj": 24, "bond": 22, "angle": 70, "dihedral": 150}
+
+
+def amber_long_count(sp):
+    n = sp["mass"].size
+    ex = set()
+    for arr, w, a, b in ((sp["bond"], 3, 0, 1), (sp["rigid"], 3, 0, 1), (sp["angle"], 4, 0, 2), (sp["dih"], 5, 0, 3)):
+        t = np.asarray(arr).reshape(-1, w)
+        for row in t:
+            i, j = int(row[a]), int(row[b])
+            ex.add((min(i, j), max(i, j)))
+    return n * (n - 1) // 2 - len(ex)
+
+
+def step_flops(sd, st):
+    """Algorithmic FLOPs of one step of one replica in state st."""
+    from md_nucleation_amd import _lib as M
+    v = sd.view
+    fl = v["flags"]
+    sp = v["species"]
+    n_ion, na_g, na_v = sp[0]["mass"].size, sp[1]["mass"].size, sp[2]["mass"].size
+    ng = int(st.region[1:1 + v["n_gas"]].sum())
+    nv = int(st.region[1 + v["n_gas"]:].sum())
+    f = 0
+    if fl & M.MDN_VI:
+        f += nv * na_v * n_ion * FLOP["ljcoul_w"]
+    if fl & M.MDN_VV:
+        f += nv * (nv - 1) // 2 * na_v * na_v * FLOP["ljcoul_w"]
+    if fl & M.MDN_GI:
+        f += st.c.n_gi * na_g * FLOP["lj"]
+    if fl & M.MDN_GV:
+        f += st.c.n_gv * na_g * min(na_g, na_v) * FLOP["lj"]
+    if fl & M.MDN_ION_AMBER:
+        f += amber_long_count(sp[0]) * FLOP["ljcoul"]
+        f += (sp[0]["bond"].size // 3) * FLOP["bond"] + (sp[0]["angle"].size // 4) * FLOP["angle"]
+        f += (sp[0]["dih"].size // 5) * FLOP["dihedral"]
+    if fl & M.MDN_VAPOR_INTRA:
+        f += nv * ((sp[2]["bond"].size // 3) * FLOP["bond"] + (sp[2]["angle"].size // 4) * FLOP["angle"]
+                   + (sp[2]["dih"].size // 5) * FLOP["dihedral"])
+    if fl & M.MDN_GAS_INTRA:
+        f += ng * (sp[1]["bond"].size // 3) * FLOP["bond"]
+    return f
+
+
+def total_atom_steps(eng, n):
+    return sum(eng.counters(r)[0] for r in range(n))
+
+
+def cpu_baseline(reps, seconds=12.0):
+    """FP64 oracle on the host cores: one replica per OpenMP thread."""
+    from oracle import oracle as O
+    threads = max(1, min(16, os.cpu_count() or 1))
+    sample = reps[:threads]
+    orcs, states = [], []
+    for sd, st, seed, _ in sample:
+        v = sd.view
+        sysd = {k: v[k] for k in ("pair_coeff", "btype", "ctype", "dmulti", "dcoeff", "species", "n_gas", "n_vap",
+                                  "ion_type_base", "dt", "T", "d_size", "margin", "ML2", "m_gas_kg", "m_vap_kg",
+                                  "gas_spin", "flags", "efield", "shake", "shake_max_iter")}
+        sysd.update(seed=seed, replica=0)
+        orc = O.Oracle(sysd)
+        S = orc.new_state(st.as_dict())
+        orc.make_pair(S)
+        orcs.append(orc)
+        states.append(S)
+    n_cal = 50
+    t0 = time.perf_counter()
+    O.batch_verlet(orcs, states, n_cal, 0, threads)
+    dt_cal = time.perf_counter() - t0
+    nsteps = int(max(50, min(20000, n_cal * seconds / max(dt_cal, 1e-6))))
+    t0 = time.perf_counter()
+    rc, acc = O.batch_verlet(orcs, states, nsteps, 0, threads)
+    el = time.perf_counter() - t0
+    return {"value": sum(acc) / el, "unit": "atom-steps/s", "cores": threads, "kind": "port",
+            "sample": "%d replicas (first %d of the workload) x %d steps, FP64 oracle, %.1f s wall, OpenMP %d threads"
+                      % (len(sample), len(sample), nsteps, el, threads)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10000)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--replicas", type=int, default=512)
+    ap.add_argument("--plain-replicas", type=int, default=4)
+    ap.add_argument("--plain-steps", type=int, default=300)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU; replicas would shard across GPUs with no exchange "
+              "(see DESIGN.md), but no multi-GPU run is part of it", file=sys.stderr)
+        sys.exit(2)
+
+    from md_nucleation_amd import _lib as M
+    from md_nucleation_amd import workloads as W
+
+    with tempfile.TemporaryDirectory() as d:
+        t0 = time.perf_counter()
+        reps = W.batched_replicas(d, n_rep=args.replicas)
+        eng = W.build_engine(reps)
+        print("workload ready: %d replicas in %.1f s" % (len(reps), time.perf_counter() - t0), file=sys.stderr,
+              flush=True)
+        n = len(reps)
+        # mixed vs FP64 force error on identical states (one step from the
+        # same start: both hold F(x1) for the same x1)
+        k_err = min(n, 24)
+        e64 = W.build_engine(reps[:k_err])
+        e32 = W.build_engine(reps[:k_err])
+        e64.run_batched(1, precision=M.FP64)
+        e32.run_batched(1, precision=M.MIXED)
+        num = den = 0.0
+        fmax = 0.0
+        for r in range(k_err):
+            a, b = e32.get_state(r), e64.get_state(r)
+            m = np.repeat(b.region.astype(bool), [reps[r][0].view["species"][0]["mass"].size]
+                          + [reps[r][0].view["species"][1]["mass"].size] * reps[r][0].view["n_gas"]
+                          + [reps[r][0].view["species"][2]["mass"].size] * reps[r][0].view["n_vap"])
+            fa, fb = a.atom_f.reshape(-1, 3)[m], b.atom_f.reshape(-1, 3)[m]
+            num += float(((fa - fb) ** 2).sum())
+            den += float((fb ** 2).sum())
+            fmax = max(fmax, float(np.abs(fa - fb).max()))
+        force_err = (num / den) ** 0.5
+        e64.close()
+        e32.close()
+
+        # batched mixed precision: warmup, then the timed region
+        if args.warmup:
+            eng.run_batched(args.warmup, precision=M.MIXED)
+        eng.sync()
+        a0 = total_atom_steps(eng, n)
+        t0 = time.perf_counter()
+        eng.run_batched(args.steps, eflag_every=0, step0=args.warmup, precision=M.MIXED)
+        eng.sync()
+        el = time.perf_counter() - t0
+        kern_ms = eng.last_kernel_ms()
+        a1 = total_atom_steps(eng, n)
+        value = (a1 - a0) / el
+        flops = 0
+        for r in range(n):
+            flops += step_flops(reps[r][0], eng.get_state(r))
+        achieved = flops * args.steps / (kern_ms * 1e-3) / 1e12
+        print("batched: %.3f s for %d steps, kernel %.3f ms" % (el, args.steps, kern_ms), file=sys.stderr, flush=True)
+
+        # plain FP64 per-call path on a sample of the same replicas
+        pr = min(args.plain_replicas, n)
+        ep = W.build_engine(reps[:pr])
+        ep.verlet(0, 20, precision=M.FP64)
+        ep.sync()
+        p0 = total_atom_steps(ep, pr)
+        t0 = time.perf_counter()
+        for r in range(pr):
+            ep.verlet(r, args.plain_steps, precision=M.FP64)
+        ep.sync()
+        pel = time.perf_counter() - t0
+        plain = (total_atom_steps(ep, pr) - p0) / pel
+        ep.close()
+
+        cpu = None if args.no_cpu else cpu_baseline(reps, args.cpu_seconds)
+
+    n_in = (a1 - a0) / args.steps / n
+    out = {
+        "metric": METRIC, "value": value, "unit": "atom-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": el * 1e3 / args.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp32 pair math, fp64 accumulation/integration", "data": "synthetic",
+        "config": {"workload": "batched replicas: %d x (Na+ | Cl- | 150-atom AMBER chain) + 10-400 methanol|water "
+                               "+ 5000 N2, 300 K, dt 1 fs, SHAKE H, gi/gv LJ, vi/vv LJcoul" % n,
+                   "replicas": n, "mean_in_domain_atoms": n_in},
+        "plain_fp64_gpu": {"value": plain, "unit": "atom-steps/s",
+                           "sample": "%d replicas x %d steps, per-call kernels, one replica after another" %
+                                     (pr, args.plain_steps)},
+        "batched_vs_plain": value / plain,
+        "force_err": {"rel_rms": force_err, "max_abs": fmax, "vs": "FP64 path, identical states, %d replicas" % k_err},
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                     "kernel": "k_batched<float>", "kernel_ms": kern_ms,
+                     "flops_per_step": flops},
+        "cpu_baseline": cpu,
+    }
+    if cpu:
+        out["vs_cpu"] = value / cpu["value"]
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

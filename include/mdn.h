@@ -152,6 +152,12 @@ int mdn_verlet(mdn_ctx *c, int32_t replica, int32_t nsteps,
 int mdn_run_batched(mdn_ctx *c, int32_t nsteps, int32_t eflag_every,
                     int64_t step0, int32_t precision);
 
+/* launch shape of mdn_run_batched: threads per workgroup (64/128/256), LDS
+ * capacity in in-domain atoms per replica (0 = largest that keeps two
+ * workgroups per CU), grid size (0 = one workgroup per replica).  Chosen by
+ * the autotuner (md_nucleation_amd/autotune.py); no reference counterpart. */
+int mdn_set_batch_shape(mdn_ctx *c, int32_t threads, int32_t lds_cap_atoms, int32_t grid);
+
 /* counters: in-domain atom-steps since commit, rebuilds, max SHAKE iters */
 int mdn_get_counters(mdn_ctx *c, int32_t replica, int64_t *atom_steps,
                      int64_t *rebuilds, int32_t *shake_iters_max);

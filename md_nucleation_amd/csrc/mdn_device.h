@@ -1,0 +1,290 @@
+// mdn_device.h -- device-side data model and arithmetic of libmdn (gfx950).
+//
+// One replica = one effective-domain system (ion + carrier gas + vapor) of
+// the reference's Variables (src/variables.hpp:6-101), flattened to
+// structure-of-arrays device buffers.  Every per-step operation is written
+// once here, over a "store" policy that says where the in-domain atoms live:
+//   GStore  global memory, one replica per launch (the per-call drop-in path)
+//   SStore  LDS, one workgroup per replica (the persistent batched path)
+// Both index in-domain atoms in the same local order:
+//   [ion atoms | atoms of in-domain vapor molecules | atoms of in-domain gas
+//    molecules], molecules in ascending id order.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mdn {
+
+// constants.hpp:44-48 (effective domain), verlet.cpp:49 (kick factor)
+constexpr double kRinter = 100.0, kOverlap = 10.0;
+constexpr double kRcenter = kRinter - kOverlap * 0.5;
+constexpr double kRi2 = kRinter * kRinter;
+constexpr double kRI2 = (kRinter + kOverlap) * (kRinter + kOverlap);
+constexpr double kPI = 3.14159265358979323846;
+constexpr double kKickUnit = 4.184e-4;
+
+// interaction bits (same values as mdn.h)
+enum : uint32_t {
+  B_GI = 1u << 0, B_VI = 1u << 1, B_GG = 1u << 2, B_GV = 1u << 3, B_VV = 1u << 4,
+  B_AMBER = 1u << 5, B_BMH = 1u << 6, B_TERSOFF = 1u << 7, B_SW = 1u << 8,
+  B_EFIELD = 1u << 9, B_VINTRA = 1u << 10, B_GINTRA = 1u << 11
+};
+enum { U_ION = 0, U_GAS, U_VAP, U_GI, U_GG, U_VG, U_VI, U_VV, U_N };
+
+// status bits raised by kernels (host turns them into mdn_status codes)
+enum : int { ST_CAP_GI = 1, ST_CAP_GV = 2, ST_CAP_GG = 4, ST_SHAKE = 8, ST_LDS_CAP = 16 };
+
+struct DevSpecies {
+  int natoms, nbond, nangle, ndih, nrigid;
+  double molmass;
+  const double *mass, *invmass, *charge, *tmpl;
+  const int *type, *bond, *angle, *dih, *rigid;
+};
+
+// mutable per-replica scalars
+struct Dyn {
+  double U[U_N];
+  long long rebuild_count, atom_steps;
+  int loopPair, loop_update;
+  int n_gi, n_gv, n_gg;
+  int n_in_gas, n_in_vap;
+  int status, shake_iters_max;
+  int pad[3];
+};
+
+struct DevReplica {
+  DevSpecies sp[3];
+  int n_gas, n_vap, n_mol, n_atoms, n_types;
+  int off_gas, off_vap;
+  uint32_t flags;
+  int shake, shake_max_iter, gas_spin, ion_type_base;
+  double dt, d_size, margin, ML2, qqrd2e, sig_gas, sig_vap;
+  double efield[3];
+  uint32_t key0, key1;
+  const double *pair_coeff, *btype, *ctype, *dcoeff;
+  const int *dmulti;
+  int n_long;
+  const int *long_pairs;
+  double *mol_x, *mol_v;
+  uint8_t *region;
+  double *atom_x, *atom_v, *atom_f, *atom_w, *atom_xo;
+  int *in_gas, *in_vap;
+  int *gi, *gv, *gg;  // local-index lists (in-list positions), 2 ints per entry
+  int cap_gi, cap_gv, cap_gg;
+  Dyn *dyn;
+};
+
+// ------------------------------------------------------------------------
+// Random stream: Philox4x32-10 (Salmon et al., SC'11), keyed by (seed,
+// replica) and countered by (molecule, rebuild, purpose).  The same stream is
+// specified for the host, so any host C++ caller can reproduce every draw.
+__host__ __device__ inline void philox4x32_10(const uint32_t ci[4], const uint32_t ki[2], uint32_t out[4]) {
+  uint32_t c0 = ci[0], c1 = ci[1], c2 = ci[2], c3 = ci[3], k0 = ki[0], k1 = ki[1];
+  for (int r = 0; r < 10; r++) {
+    if (r) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
+    uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+    uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+    uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+    uint32_t n1 = (uint32_t)p1;
+    uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+    uint32_t n3 = (uint32_t)p0;
+    c0 = n0; c1 = n1; c2 = n2; c3 = n3;
+  }
+  out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+__host__ __device__ inline void rng_words(uint32_t k0, uint32_t k1, uint32_t mol, long long rebuild, uint32_t purpose,
+                                          uint32_t w[4]) {
+  uint32_t ctr[4] = {mol, (uint32_t)(unsigned long long)rebuild, (uint32_t)((unsigned long long)rebuild >> 32),
+                     purpose};
+  uint32_t key[2] = {k0, k1};
+  philox4x32_10(ctr, key, w);
+}
+__host__ __device__ inline double u01_53(uint32_t hi, uint32_t lo) {
+  uint64_t v = ((uint64_t)hi << 32) | lo;
+  return (double)(v >> 11) * (1.0 / 9007199254740992.0);
+}
+__host__ __device__ inline void rng_uniform2(uint32_t k0, uint32_t k1, uint32_t mol, long long rb, uint32_t purpose,
+                                             double u[2]) {
+  uint32_t w[4];
+  rng_words(k0, k1, mol, rb, purpose, w);
+  u[0] = u01_53(w[0], w[1]);
+  u[1] = u01_53(w[2], w[3]);
+}
+// Box-Muller on (0,1] x [0,1)
+__host__ __device__ inline void rng_normal2(uint32_t k0, uint32_t k1, uint32_t mol, long long rb, uint32_t purpose,
+                                            double z[2]) {
+  uint32_t w[4];
+  rng_words(k0, k1, mol, rb, purpose, w);
+  uint64_t v = ((uint64_t)w[0] << 32) | w[1];
+  double u1 = (double)((v >> 11) + 1) * (1.0 / 9007199254740992.0);
+  double u2 = u01_53(w[2], w[3]);
+  double r = sqrt(-2.0 * log(u1));
+  double th = 2.0 * kPI * u2;
+  z[0] = r * cos(th);
+  z[1] = r * sin(th);
+}
+
+// Variables::ROTATION, src/variables.hpp:94-98
+__device__ inline void rotation(double &X, double &Y, double &Z, double A, double B, double C, double x, double y,
+                                double z) {
+  double cA = cos(A), sA = sin(A), cB = cos(B), sB = sin(B), cC = cos(C), sC = sin(C);
+  double t = x * cB + y * sA * sB - z * cA * sB;
+  double u = y * cA + z * sA;
+  X = cC * t + sC * u;
+  Y = -sC * t + cC * u;
+  Z = x * sB - y * sA * cB + z * cA * cB;
+}
+
+// blend weight of an atom at squared distance r2 from the ion COM,
+// src/verlet.cpp:117-130 (Rcenter = 95: 1 -> 0.5 at 100 A, 0 at 105 A,
+// 0.5 at 110 A^-, 0 beyond)
+__device__ inline double blend_weight(double r2) {
+  if (r2 < kRi2) return 1.0;
+  if (r2 > kRI2) return 0.0;
+  double dr = sqrt(r2) - kRcenter;
+  double C = cos(kPI / kOverlap * 0.5 * dr);
+  return C * C;
+}
+
+// ------------------------------------------------------------------------
+// Pair arithmetic.  R = double (plain) or float (mixed: the displacement is
+// formed in FP64 and rounded, the rest is FP32; the caller accumulates the
+// force in FP64).
+// LJ + Coulomb, VaporIon.cpp:19-37 / VaporVapor.cpp:21-39 / computeLong.
+template <typename R>
+__device__ inline R ljcoul_fpair(R rsq, R c0, R c1, R qq, R w, bool eflag, double &e) {
+  R r2inv = R(1) / rsq;
+  R r6inv = r2inv * r2inv * r2inv;
+  R flj = r6inv * (c0 * r6inv - c1);
+  R fc = qq * sqrt(r2inv);
+  if (eflag) e += (double)(r6inv * (c0 / R(12) * r6inv - c1 / R(6))) + (double)fc;
+  return (flj + fc) * r2inv * w;
+}
+// LJ only, GasIon.cpp:19-33 / VaporGas.cpp:19-35
+template <typename R>
+__device__ inline R lj_fpair(R rsq, R c0, R c1, R w, bool eflag, double &e) {
+  R r2inv = R(1) / rsq;
+  R r6inv = r2inv * r2inv * r2inv;
+  R flj = r6inv * (c0 * r6inv - c1);
+  if (eflag) e += (double)(r6inv * (c0 / R(12) * r6inv - c1 / R(6)));
+  return flj * r2inv * w;
+}
+
+// harmonic bond, compute_AMBER.cpp:52-77: returns fb with F_i = fb*d, F_j = -fb*d
+__device__ inline double bond_fb(double rsq, double k, double r0, bool eflag, double &e) {
+  double r = sqrt(rsq);
+  double dr = r - r0;
+  double rk = k * dr;
+  if (eflag) e += rk * dr;
+  return -2.0 * rk / r;
+}
+
+// harmonic angle, compute_AMBER.cpp:80-127
+__device__ inline void angle_f(const double d1[3], const double d2[3], double k, double th0, bool eflag, double &e,
+                               double f1[3], double f3[3]) {
+  double rsq1 = d1[0] * d1[0] + d1[1] * d1[1] + d1[2] * d1[2];
+  double rsq2 = d2[0] * d2[0] + d2[1] * d2[1] + d2[2] * d2[2];
+  double r1 = sqrt(rsq1), r2 = sqrt(rsq2);
+  double C = d1[0] * d2[0] + d1[1] * d2[1] + d1[2] * d2[2];
+  C /= r1 * r2;
+  double Cs = 1.0 / sqrt(1.0 - C * C);
+  double dtheta = acos(C) - th0;
+  double tk = k * dtheta;
+  if (eflag) e += tk * dtheta;
+  double a = -2.0 * tk * Cs;
+  double a11 = a * C / rsq1, a12 = -a / (r1 * r2), a22 = a * C / rsq2;
+  for (int q = 0; q < 3; q++) {
+    f1[q] = a11 * d1[q] + a12 * d2[q];
+    f3[q] = a22 * d2[q] + a12 * d1[q];
+  }
+}
+
+// multi-term dihedral, compute_AMBER.cpp:130-253 (LAMMPS charmm form, n-fold
+// recurrence for cos/sin(n phi); the n==0 special case checks the first
+// term's n for every term, compute_AMBER.cpp:199)
+__device__ inline void dihedral_f(const double vb1[3], const double vb2[3], const double vb3[3], const double *co,
+                                  int multi, bool eflag, double &e, double f1[3], double f2[3], double f3[3],
+                                  double f4[3]) {
+  double vb2m[3] = {-vb2[0], -vb2[1], -vb2[2]};
+  double ax = vb1[1] * vb2m[2] - vb1[2] * vb2m[1];
+  double ay = vb1[2] * vb2m[0] - vb1[0] * vb2m[2];
+  double az = vb1[0] * vb2m[1] - vb1[1] * vb2m[0];
+  double bx = vb3[1] * vb2m[2] - vb3[2] * vb2m[1];
+  double by = vb3[2] * vb2m[0] - vb3[0] * vb2m[2];
+  double bz = vb3[0] * vb2m[1] - vb3[1] * vb2m[0];
+  double rasq = ax * ax + ay * ay + az * az;
+  double rbsq = bx * bx + by * by + bz * bz;
+  double rgsq = vb2m[0] * vb2m[0] + vb2m[1] * vb2m[1] + vb2m[2] * vb2m[2];
+  double rg = sqrt(rgsq);
+  double rginv = rg > 0 ? 1.0 / rg : 0.0;
+  double ra2inv = rasq > 0 ? 1.0 / rasq : 0.0;
+  double rb2inv = rbsq > 0 ? 1.0 / rbsq : 0.0;
+  double rabinv = sqrt(ra2inv * rb2inv);
+  double c = (ax * bx + ay * by + az * bz) * rabinv;
+  double s = rg * rabinv * (ax * vb3[0] + ay * vb3[1] + az * vb3[2]);
+  double df = 0.0;
+  for (int JJ = 0; JJ < multi; JJ++) {
+    const double *t = co + 5 * JJ;
+    double p = 1.0, df1 = 0.0, ddf1 = 0.0;
+    for (int l = 0; l < t[1]; l++) {
+      ddf1 = p * c - df1 * s;
+      df1 = p * s + df1 * c;
+      p = ddf1;
+    }
+    p = p * t[3] + df1 * t[4];
+    df1 = df1 * t[3] - ddf1 * t[4];
+    df1 *= -t[1];
+    p += 1.0;
+    if (co[1] == 0) {
+      p = 1.0 + t[3];
+      df1 = 0.0;
+    }
+    df += -t[0] * df1;
+    if (eflag) e += t[0] * p;
+  }
+  double fg = vb1[0] * vb2m[0] + vb1[1] * vb2m[1] + vb1[2] * vb2m[2];
+  double hg = vb3[0] * vb2m[0] + vb3[1] * vb2m[1] + vb3[2] * vb2m[2];
+  double fga = fg * ra2inv * rginv, hgb = hg * rb2inv * rginv;
+  double gaa = -ra2inv * rg, gbb = rb2inv * rg;
+  double dtf[3] = {gaa * ax, gaa * ay, gaa * az};
+  double dtg[3] = {fga * ax - hgb * bx, fga * ay - hgb * by, fga * az - hgb * bz};
+  double dth[3] = {gbb * bx, gbb * by, gbb * bz};
+  for (int q = 0; q < 3; q++) {
+    double sq = df * dtg[q];
+    f1[q] = df * dtf[q];
+    f2[q] = sq - f1[q];
+    f4[q] = df * dth[q];
+    f3[q] = -sq - f4[q];
+  }
+}
+
+// ------------------------------------------------------------------------
+// Workgroup helpers (nthr = blockDim.x, multiple of 64, at most 1024)
+__device__ inline double wave_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+// exclusive scan of one int per thread over the workgroup; s_tmp holds
+// >= nthr/64 ints.  Returns the prefix, *total = the workgroup sum.
+__device__ inline int block_scan(int v, int *s_tmp, int *total) {
+  int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+  int x = v;
+  for (int o = 1; o < 64; o <<= 1) {
+    int y = __shfl_up(x, o);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) s_tmp[wave] = x;
+  __syncthreads();
+  int base = 0, tot = 0;
+  for (int w = 0; w < nw; w++) {
+    int c = s_tmp[w];
+    if (w < wave) base += c;
+    tot += c;
+  }
+  __syncthreads();
+  *total = tot;
+  return base + x - v;
+}
+
+}  // namespace mdn

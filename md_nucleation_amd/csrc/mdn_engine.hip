@@ -1,0 +1,1039 @@
+// mdn_engine.hip -- device engine and C ABI of libmdn (gfx950).
+//
+// Two execution paths over the same replica data and the same per-term code
+// (mdn_terms.h):
+//  * per-call path: one replica, one launch per reference call
+//    (velocity_calculation, update_position, check_pairlist/make_pair,
+//    Potential::compute), atoms in global memory, FP64 atomics.  This is the
+//    plain drop-in for MD::verlet; it also carries single replicas too large
+//    for one workgroup's LDS (tiled vv kernel below).
+//  * batched path: every replica in one persistent launch, one workgroup per
+//    replica, its in-domain atoms resident in LDS across steps; the pair-list
+//    rebuild runs inside the same kernel when the replica's own loopPair
+//    exceeds loop_update.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "mdn.h"
+#include "mdn_device.h"
+#include "mdn_internal.h"
+#include "mdn_terms.h"
+
+namespace mdn {
+
+static thread_local std::string g_err;
+int set_error(int code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+
+// ======================= per-call kernels ================================
+constexpr int kBlk = 256;
+
+// local in-domain atom a -> global atom index (GStore order)
+__device__ inline int gatom(const DevReplica &R, const LView &L, int a) {
+  if (a < L.n_ion) return a;
+  if (a < L.gbase) {
+    int k = (a - L.vbase) / L.na_v, i = a - L.vbase - k * L.na_v;
+    return R.off_vap + (R.in_vap[k] - 1 - R.n_gas) * L.na_v + i;
+  }
+  int k = (a - L.gbase) / L.na_g, i = a - L.gbase - k * L.na_g;
+  return R.off_gas + (R.in_gas[k] - 1) * L.na_g + i;
+}
+__device__ inline double gmass(const DevReplica &R, const LView &L, int a) {
+  if (a < L.n_ion) return R.sp[0].mass[a];
+  if (a < L.gbase) return R.sp[2].mass[(a - L.vbase) % L.na_v];
+  return R.sp[1].mass[(a - L.gbase) % L.na_g];
+}
+
+// MD::velocity_calculation, verlet.cpp:47-65 (region atoms only).  At the
+// start of an energy step it also clears the energy buckets (Uzero,
+// run.cpp:68-71); at the end of a step it counts in-domain atom-steps.
+__global__ void k_kick(const DevReplica *__restrict__ Rp, int zeroU, int count) {
+  const DevReplica &R = *Rp;
+  LView L = view_from_dyn(R);
+  int t = blockIdx.x * blockDim.x + threadIdx.x, nt = gridDim.x * blockDim.x;
+  if (t == 0 && zeroU)
+    for (int k = 0; k < U_N; k++) R.dyn->U[k] = 0;
+  if (t == 0 && count) R.dyn->atom_steps += L.n_in;
+  const double C = 0.5 * R.dt * kKickUnit;
+  for (int a = t; a < L.n_in; a += nt) {
+    int g = gatom(R, L, a);
+    double c2 = C / gmass(R, L, a);
+    double *v = R.atom_v + 3 * (size_t)g, *f = R.atom_f + 3 * (size_t)g;
+    v[0] += f[0] * c2;
+    v[1] += f[1] * c2;
+    v[2] += f[2] * c2;
+  }
+}
+
+// drift + force clear, verlet.cpp:76-88 / shake.cpp:6-24
+__global__ void k_drift(const DevReplica *__restrict__ Rp, int save_old) {
+  const DevReplica &R = *Rp;
+  LView L = view_from_dyn(R);
+  int t = blockIdx.x * blockDim.x + threadIdx.x, nt = gridDim.x * blockDim.x;
+  for (int a = t; a < L.n_in; a += nt) {
+    size_t g = 3 * (size_t)gatom(R, L, a);
+    double *x = R.atom_x + g, *v = R.atom_v + g, *f = R.atom_f + g;
+    if (save_old) { R.atom_xo[g] = x[0]; R.atom_xo[g + 1] = x[1]; R.atom_xo[g + 2] = x[2]; }
+    x[0] += v[0] * R.dt;
+    x[1] += v[1] * R.dt;
+    x[2] += v[2] * R.dt;
+    f[0] = f[1] = f[2] = 0.0;
+  }
+}
+
+// SHAKE per in-domain molecule with rigid pairs, shake.cpp:25-37
+__global__ void k_shake(const DevReplica *__restrict__ Rp) {
+  const DevReplica &R = *Rp;
+  LView L = view_from_dyn(R);
+  int t = blockIdx.x * blockDim.x + threadIdx.x, nt = gridDim.x * blockDim.x;
+  int nm = 1 + L.nv + L.ng;
+  for (int m = t; m < nm; m += nt) {
+    int sp, o;
+    if (m == 0) { sp = 0; o = 0; }
+    else if (m <= L.nv) { sp = 2; o = R.off_vap + (R.in_vap[m - 1] - 1 - R.n_gas) * L.na_v; }
+    else { sp = 1; o = R.off_gas + (R.in_gas[m - 1 - L.nv] - 1) * L.na_g; }
+    const DevSpecies &S = R.sp[sp];
+    if (!S.nrigid) continue;
+    double *x = R.atom_x + 3 * (size_t)o;
+    const double *xo = R.atom_xo + 3 * (size_t)o;
+    int it = shake_molecule(R, S, [&](int i, int q) -> double & { return x[3 * i + q]; },
+                            [&](int i, int q) -> double { return xo[3 * i + q]; }, R.shake_max_iter);
+    if (it < 0) { atomicOr(&R.dyn->status, ST_SHAKE); it = -it; }
+    atomicMax(&R.dyn->shake_iters_max, it);
+  }
+}
+
+// ion COM + blend weights, verlet.cpp:90-134 (one workgroup)
+__global__ void __launch_bounds__(1024) k_com_weights(const DevReplica *__restrict__ Rp) {
+  const DevReplica &R = *Rp;
+  __shared__ double s_red[6][16];
+  __shared__ double s_com[6];
+  LView L = view_from_dyn(R);
+  int tid = threadIdx.x, nt = blockDim.x;
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  const DevSpecies &I = R.sp[0];
+  for (int i = tid; i < I.natoms; i += nt) {
+    double m = I.mass[i];
+    const double *x = R.atom_x + 3 * (size_t)i, *v = R.atom_v + 3 * (size_t)i;
+    acc[0] += x[0] * m; acc[1] += x[1] * m; acc[2] += x[2] * m;
+    acc[3] += v[0] * m; acc[4] += v[1] * m; acc[5] += v[2] * m;
+  }
+  for (int k = 0; k < 6; k++) {
+    double s = wave_sum(acc[k]);
+    if ((tid & 63) == 0) s_red[k][tid >> 6] = s;
+  }
+  __syncthreads();
+  if (tid < 6) {
+    double s = 0;
+    for (int w = 0; w < (nt + 63) / 64; w++) s += s_red[tid][w];
+    double c = s * (1 / I.molmass);
+    s_com[tid] = c;
+    if (tid < 3) R.mol_x[tid] = c; else R.mol_v[tid - 3] = c;
+  }
+  __syncthreads();
+  for (int a = L.n_ion + tid; a < L.n_in; a += nt) {
+    int g = gatom(R, L, a);
+    const double *x = R.atom_x + 3 * (size_t)g;
+    double dx = x[0] - s_com[0], dy = x[1] - s_com[1], dz = x[2] - s_com[2];
+    R.atom_w[g] = blend_weight(dx * dx + dy * dy + dz * dz);
+  }
+}
+
+// MD::check_pairlist + MD::make_pair (pairlist.cpp:16-63), one workgroup
+__global__ void __launch_bounds__(1024) k_rebuild(const DevReplica *__restrict__ Rp, int force) {
+  const DevReplica &R = *Rp;
+  __shared__ int s_int[64];
+  __shared__ double s_dbl[16];
+  __shared__ int s_do;
+  if (threadIdx.x == 0) {
+    int go = force;
+    if (!force) {
+      int lp = ++R.dyn->loopPair;
+      go = lp > R.dyn->loop_update;
+    }
+    s_do = go;
+  }
+  __syncthreads();
+  if (s_do) rebuild_wg(R, s_int, s_dbl);
+}
+
+// wave-reduced energy flush
+__device__ inline void flush_energy(const DevReplica &R, const double *e) {
+  for (int k = 0; k < U_N; k++) {
+    double s = wave_sum(e[k]);
+    if ((threadIdx.x & 63) == 0 && s != 0.0) atomicAdd(&R.dyn->U[k], s);
+  }
+}
+
+template <typename Rl>
+__global__ void k_terms(const DevReplica *__restrict__ Rp, uint32_t bits, int eflag) {
+  const DevReplica &R = *Rp;
+  GStore s;
+  s.R = Rp;
+  s.L = view_from_dyn(R);
+  int t = blockIdx.x * blockDim.x + threadIdx.x, nt = gridDim.x * blockDim.x;
+  double e[U_N] = {0, 0, 0, 0, 0, 0, 0, 0};
+  all_terms<Rl>(R, s, bits, t, nt, eflag != 0, e);
+  if (eflag) flush_energy(R, e);
+}
+
+// vv for large in-domain vapor sets (e.g. a 2,000-methanol cluster):
+// i-centric, j tiles staged through LDS, no atomics on j; grid.y splits the j
+// range so that a launch fills the chip.  Each unordered pair is evaluated
+// from both sides, so its energy is halved.
+constexpr int kVVTile = 256;
+template <typename Rl>
+__global__ void __launch_bounds__(kVVTile) k_vv_tiled(const DevReplica *__restrict__ Rp, int eflag, int tv0, int ntv) {
+  const DevReplica &R = *Rp;
+  __shared__ double sx[kVVTile * 3];
+  __shared__ double sq[kVVTile], sw[kVVTile];
+  __shared__ int st[kVVTile], sm[kVVTile];
+  __shared__ double sc[2 * 64];
+  LView L = view_from_dyn(R);
+  const int nva = L.nv * L.na_v;
+  // vapor-vapor coefficient table (vapor types are contiguous)
+  for (int k = threadIdx.x; k < ntv * ntv; k += blockDim.x) {
+    int a = k / ntv, b = k - a * ntv;
+    const double *p = R.pair_coeff + 2 * ((size_t)(tv0 + a) * R.n_types + tv0 + b);
+    sc[2 * k] = p[0];
+    sc[2 * k + 1] = p[1];
+  }
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool act = i < nva;
+  double xi[3] = {0, 0, 0}, qi = 0, wi = 0;
+  int ti = 0, mi = -1, gi = 0;
+  const DevSpecies &V = R.sp[2];
+  if (act) {
+    int k = i / L.na_v, ii = i - k * L.na_v;
+    gi = R.off_vap + (R.in_vap[k] - 1 - R.n_gas) * L.na_v + ii;
+    xi[0] = R.atom_x[3 * (size_t)gi]; xi[1] = R.atom_x[3 * (size_t)gi + 1]; xi[2] = R.atom_x[3 * (size_t)gi + 2];
+    qi = R.qqrd2e * V.charge[ii];
+    wi = R.atom_w[gi];
+    ti = V.type[ii] - tv0;
+    mi = k;
+  }
+  int per = (nva + gridDim.y - 1) / gridDim.y;
+  int j0 = blockIdx.y * per, j1 = min(nva, j0 + per);
+  double fx = 0, fy = 0, fz = 0, e = 0;
+  for (int jt = j0; jt < j1; jt += kVVTile) {
+    __syncthreads();
+    int j = jt + threadIdx.x;
+    if (j < j1) {
+      int k = j / L.na_v, jj = j - k * L.na_v;
+      int g = R.off_vap + (R.in_vap[k] - 1 - R.n_gas) * L.na_v + jj;
+      sx[3 * threadIdx.x] = R.atom_x[3 * (size_t)g];
+      sx[3 * threadIdx.x + 1] = R.atom_x[3 * (size_t)g + 1];
+      sx[3 * threadIdx.x + 2] = R.atom_x[3 * (size_t)g + 2];
+      sq[threadIdx.x] = V.charge[jj];
+      sw[threadIdx.x] = R.atom_w[g];
+      st[threadIdx.x] = V.type[jj] - tv0;
+      sm[threadIdx.x] = k;
+    }
+    __syncthreads();
+    int nj = min(kVVTile, j1 - jt);
+    if (act) {
+      Rl ax = 0, ay = 0, az = 0;
+      for (int b = 0; b < nj; b++) {
+        if (sm[b] == mi) continue;
+        double dx = xi[0] - sx[3 * b], dy = xi[1] - sx[3 * b + 1], dz = xi[2] - sx[3 * b + 2];
+        Rl rx = (Rl)dx, ry = (Rl)dy, rz = (Rl)dz;
+        Rl rsq = rx * rx + ry * ry + rz * rz;
+        int c = 2 * (ti * ntv + st[b]);
+        Rl fp = ljcoul_fpair<Rl>(rsq, (Rl)sc[c], (Rl)sc[c + 1], (Rl)(qi * sq[b]), (Rl)(wi * sw[b]), eflag != 0, e);
+        ax += fp * rx; ay += fp * ry; az += fp * rz;
+      }
+      fx += (double)ax; fy += (double)ay; fz += (double)az;
+    }
+  }
+  if (act) {
+    atomicAdd(&R.atom_f[3 * (size_t)gi], fx);
+    atomicAdd(&R.atom_f[3 * (size_t)gi + 1], fy);
+    atomicAdd(&R.atom_f[3 * (size_t)gi + 2], fz);
+  }
+  if (eflag) {
+    double s = wave_sum(0.5 * e);
+    if ((threadIdx.x & 63) == 0 && s != 0.0) atomicAdd(&R.dyn->U[U_VV], s);
+  }
+}
+
+// ======================= batched persistent kernel ========================
+struct BatchLds {
+  double *x, *v, *f, *xo, *w, *q, *kc;
+  int *type;
+  double *red;   // [U_N + 6][16]
+  int *sint;     // [64]
+  double *sdbl;  // [16]
+  int *flag;
+};
+
+__device__ inline BatchLds carve(char *base, int cap, int has_shake) {
+  BatchLds B;
+  double *d = reinterpret_cast<double *>(base);
+  B.x = d; d += 3 * cap;
+  B.v = d; d += 3 * cap;
+  B.f = d; d += 3 * cap;
+  B.xo = d; if (has_shake) d += 3 * cap;
+  B.w = d; d += cap;
+  B.q = d; d += cap;
+  B.kc = d; d += cap;
+  B.red = d; d += (U_N + 6) * 16;
+  B.sdbl = d; d += 16;
+  int *ip = reinterpret_cast<int *>(d);
+  B.type = ip; ip += cap;
+  B.sint = ip; ip += 64;
+  B.flag = ip; ip += 4;
+  return B;
+}
+
+static size_t batch_lds_bytes(int cap, int has_shake) {
+  return sizeof(double) * ((size_t)(9 + (has_shake ? 3 : 0) + 3) * cap + (U_N + 6) * 16 + 16) +
+         sizeof(int) * ((size_t)cap + 64 + 4);
+}
+
+__device__ inline void batch_load(const DevReplica &R, const LView &L, const BatchLds &B) {
+  const double C = 0.5 * R.dt * kKickUnit;
+  for (int a = threadIdx.x; a < L.n_in; a += blockDim.x) {
+    int sp, idx;
+    if (a < L.n_ion) { sp = 0; idx = a; }
+    else if (a < L.gbase) { sp = 2; idx = (a - L.vbase) % L.na_v; }
+    else { sp = 1; idx = (a - L.gbase) % L.na_g; }
+    size_t g = (size_t)gatom(R, L, a);
+    for (int q = 0; q < 3; q++) {
+      B.x[3 * a + q] = R.atom_x[3 * g + q];
+      B.v[3 * a + q] = R.atom_v[3 * g + q];
+      B.f[3 * a + q] = R.atom_f[3 * g + q];
+    }
+    B.w[a] = R.atom_w[g];
+    B.q[a] = R.sp[sp].charge[idx];
+    B.type[a] = R.sp[sp].type[idx];
+    B.kc[a] = C / R.sp[sp].mass[idx];
+  }
+}
+
+__device__ inline void batch_store(const DevReplica &R, const LView &L, const BatchLds &B) {
+  for (int a = threadIdx.x; a < L.n_in; a += blockDim.x) {
+    size_t g = (size_t)gatom(R, L, a);
+    for (int q = 0; q < 3; q++) {
+      R.atom_x[3 * g + q] = B.x[3 * a + q];
+      R.atom_v[3 * g + q] = B.v[3 * a + q];
+      R.atom_f[3 * g + q] = B.f[3 * a + q];
+    }
+    R.atom_w[g] = B.w[a];
+  }
+}
+
+template <typename Rl>
+__global__ void __launch_bounds__(256) k_batched(const DevReplica *__restrict__ reps, int n_rep, int nsteps,
+                                                 int eflag_every, long long step0, int cap, int has_shake) {
+  extern __shared__ __align__(16) char smem[];
+  BatchLds B = carve(smem, cap, has_shake);
+  const int tid = threadIdx.x, nt = blockDim.x;
+  for (int r = blockIdx.x; r < n_rep; r += gridDim.x) {
+    const DevReplica &R = reps[r];
+    Dyn *D = R.dyn;
+    if (D->status & ST_LDS_CAP) continue;
+    LView L = view_from_dyn(R);
+    if (L.n_in > cap) {
+      if (tid == 0) atomicOr(&D->status, ST_LDS_CAP);
+      continue;
+    }
+    batch_load(R, L, B);
+    int lp = D->loopPair, lu = D->loop_update;
+    long long acc_steps = 0;
+    const DevSpecies &I = R.sp[0];
+    const bool shake = R.shake && has_shake;
+    __syncthreads();
+    bool dead = false;
+    for (int s = 0; s < nsteps && !dead; s++) {
+      const bool eflag = eflag_every > 0 && ((step0 + s + 1) % eflag_every) == 0;
+      // first half-kick + drift + force clear (verlet.cpp:21-25)
+      for (int a = tid; a < L.n_in; a += nt) {
+        double kc = B.kc[a];
+        for (int q = 0; q < 3; q++) {
+          double v = B.v[3 * a + q] + B.f[3 * a + q] * kc;
+          B.v[3 * a + q] = v;
+          if (shake) B.xo[3 * a + q] = B.x[3 * a + q];
+          B.x[3 * a + q] += v * R.dt;
+          B.f[3 * a + q] = 0.0;
+        }
+      }
+      __syncthreads();
+      if (shake) {
+        int nm = 1 + L.nv + L.ng;
+        for (int m = tid; m < nm; m += nt) {
+          int sp, base;
+          if (m == 0) { sp = 0; base = 0; }
+          else if (m <= L.nv) { sp = 2; base = L.vbase + (m - 1) * L.na_v; }
+          else { sp = 1; base = L.gbase + (m - 1 - L.nv) * L.na_g; }
+          const DevSpecies &S = R.sp[sp];
+          if (!S.nrigid) continue;
+          double *x = B.x + 3 * base;
+          const double *xo = B.xo + 3 * base;
+          int it = shake_molecule(R, S, [&](int i, int q) -> double & { return x[3 * i + q]; },
+                                  [&](int i, int q) -> double { return xo[3 * i + q]; }, R.shake_max_iter);
+          if (it < 0) { atomicOr(&D->status, ST_SHAKE); it = -it; }
+          atomicMax(&D->shake_iters_max, it);
+        }
+        __syncthreads();
+      }
+      // ion COM (verlet.cpp:90-107)
+      {
+        double c[6] = {0, 0, 0, 0, 0, 0};
+        for (int i = tid; i < I.natoms; i += nt) {
+          double m = I.mass[i];
+          for (int q = 0; q < 3; q++) { c[q] += B.x[3 * i + q] * m; c[3 + q] += B.v[3 * i + q] * m; }
+        }
+        for (int k = 0; k < 6; k++) {
+          double sv = wave_sum(c[k]);
+          if ((tid & 63) == 0) B.red[k * 16 + (tid >> 6)] = sv;
+        }
+        __syncthreads();
+        if (tid < 6) {
+          double sv = 0;
+          for (int w = 0; w < nt / 64; w++) sv += B.red[tid * 16 + w];
+          B.sdbl[tid] = sv * (1 / I.molmass);
+        }
+        __syncthreads();
+      }
+      const double cx = B.sdbl[0], cy = B.sdbl[1], cz = B.sdbl[2];
+      // blend weights (verlet.cpp:109-134)
+      for (int a = L.n_ion + tid; a < L.n_in; a += nt) {
+        double dx = B.x[3 * a] - cx, dy = B.x[3 * a + 1] - cy, dz = B.x[3 * a + 2] - cz;
+        B.w[a] = blend_weight(dx * dx + dy * dy + dz * dz);
+      }
+      // check_pairlist (pairlist.cpp:58-63)
+      lp++;
+      if (lp > lu) {
+        __syncthreads();
+        batch_store(R, L, B);
+        if (tid < 3) { R.mol_x[tid] = B.sdbl[tid]; R.mol_v[tid] = B.sdbl[3 + tid]; }
+        if (tid == 0) D->loopPair = lp;
+        __syncthreads();
+        rebuild_wg(R, B.sint, B.red);
+        L = view_from_dyn(R);
+        lp = D->loopPair;
+        lu = D->loop_update;
+        if (L.n_in > cap) {
+          if (tid == 0) atomicOr(&D->status, ST_LDS_CAP);
+          dead = true;
+          break;
+        }
+        batch_load(R, L, B);
+      }
+      __syncthreads();
+      // forces of every registered potential (verlet.cpp:31-32)
+      SStore ss;
+      ss.L = L; ss.px = B.x; ss.pf = B.f; ss.pw = B.w; ss.pq = B.q; ss.ptype = B.type;
+      double e[U_N] = {0, 0, 0, 0, 0, 0, 0, 0};
+      all_terms<Rl>(R, ss, R.flags, tid, nt, eflag, e);
+      __syncthreads();
+      // second half-kick (verlet.cpp:34)
+      for (int a = tid; a < L.n_in; a += nt) {
+        double kc = B.kc[a];
+        for (int q = 0; q < 3; q++) B.v[3 * a + q] += B.f[3 * a + q] * kc;
+      }
+      acc_steps += L.n_in;
+      if (eflag) {
+        for (int k = 0; k < U_N; k++) {
+          double sv = wave_sum(e[k]);
+          if ((tid & 63) == 0) B.red[(6 + k) * 16 + (tid >> 6)] = sv;
+        }
+        __syncthreads();
+        if (tid < U_N) {
+          double sv = 0;
+          for (int w = 0; w < nt / 64; w++) sv += B.red[(6 + tid) * 16 + w];
+          D->U[tid] = sv;
+        }
+      }
+      __syncthreads();
+    }
+    if (!dead) {
+      batch_store(R, L, B);
+      if (tid < 3) { R.mol_x[tid] = B.sdbl[tid]; R.mol_v[tid] = B.sdbl[3 + tid]; }
+      if (tid == 0) D->loopPair = lp;
+    }
+    if (tid == 0) D->atom_steps += acc_steps;
+    __syncthreads();
+  }
+}
+
+// ======================= host side =======================================
+#define HIPCHK(x)                                                                              \
+  do {                                                                                         \
+    hipError_t e_ = (x);                                                                       \
+    if (e_ != hipSuccess) return set_error(MDN_E_HIP, std::string(#x ": ") + hipGetErrorString(e_)); \
+  } while (0)
+
+struct HostReplica {
+  SysDef def;
+  std::vector<double> mol_x, mol_v, atom_x, atom_v, atom_f, atom_w;
+  std::vector<uint8_t> region;
+  std::vector<int> gi, gv, gg;  // reference format (global ids)
+  int loopPair = 0, loop_update = 0;
+  long long rebuild_count = 0;
+  double U[U_N] = {0};
+  uint64_t seed = 1;
+  uint32_t replica_id = 0;
+  int cap_gi = 0, cap_gv = 0, cap_gg = 0;
+  std::vector<int> long_pairs;
+  DevReplica dev;
+  size_t off_dyn = 0;
+};
+
+}  // namespace mdn
+
+struct mdn_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::vector<mdn::HostReplica> reps;
+  char *dbuf = nullptr;
+  size_t dbytes = 0;
+  mdn::DevReplica *d_reps = nullptr;
+  bool committed = false;
+  int batch_cap = 0, batch_threads = 256, batch_grid = 0;
+  int any_shake = 0;
+};
+
+namespace mdn {
+
+// AMBER exclusion list, initialAMBER (compute_AMBER.cpp:256-298): ion pairs
+// i<j that are not bonded, rigid, angle ends or dihedral ends
+static std::vector<int> amber_long_pairs(const Species &I) {
+  int n = I.natoms;
+  std::vector<uint8_t> ex((size_t)n * n, 0);
+  auto mark = [&](int a, int b) { ex[(size_t)a * n + b] = ex[(size_t)b * n + a] = 1; };
+  for (size_t k = 0; 3 * k < I.bond.size(); k++) mark(I.bond[3 * k], I.bond[3 * k + 1]);
+  for (size_t k = 0; 3 * k < I.rigid.size(); k++) mark(I.rigid[3 * k], I.rigid[3 * k + 1]);
+  for (size_t k = 0; 4 * k < I.angle.size(); k++) mark(I.angle[4 * k], I.angle[4 * k + 2]);
+  for (size_t k = 0; 5 * k < I.dih.size(); k++) mark(I.dih[5 * k], I.dih[5 * k + 3]);
+  std::vector<int> out;
+  for (int i = 0; i < n - 1; i++)
+    for (int j = i + 1; j < n; j++)
+      if (!ex[(size_t)i * n + j]) { out.push_back(i); out.push_back(j); }
+  return out;
+}
+
+// bump layout over one device allocation; identical read-only tables are
+// stored once (replicas of one batch usually share species)
+struct Layout {
+  std::vector<char> host;
+  std::map<std::string, size_t> dedup;
+  size_t put(const void *p, size_t n, bool share) {
+    size_t off = (host.size() + 255) & ~(size_t)255;
+    if (share && n) {
+      std::string key(reinterpret_cast<const char *>(p), n);
+      auto it = dedup.find(key);
+      if (it != dedup.end()) return it->second;
+      host.resize(off + n);
+      memcpy(host.data() + off, p, n);
+      dedup.emplace(std::move(key), off);
+      return off;
+    }
+    host.resize(off + (n ? n : 8), 0);
+    if (n && p) memcpy(host.data() + off, p, n);
+    return off;
+  }
+  template <class T> size_t vec(const std::vector<T> &v, bool share) { return put(v.data(), v.size() * sizeof(T), share); }
+  size_t zeros(size_t n) { return put(nullptr, n, false); }
+};
+
+template <class T> static T *at(char *base, size_t off) { return reinterpret_cast<T *>(base + off); }
+
+static void in_lists(const HostReplica &h, std::vector<int> &ig, std::vector<int> &iv) {
+  ig.clear(); iv.clear();
+  for (int i = 1; i <= h.def.n_gas; i++) if (h.region[i]) ig.push_back(i);
+  for (int i = 1 + h.def.n_gas; i < h.def.n_mol; i++) if (h.region[i]) iv.push_back(i);
+}
+
+// reference-format lists (global ids) -> in-list positions
+static int lists_to_local(const HostReplica &h, std::vector<int> &gi, std::vector<int> &gv, std::vector<int> &gg) {
+  std::vector<int> ig, iv;
+  in_lists(h, ig, iv);
+  std::vector<int> pos(h.def.n_mol, -1);
+  for (size_t k = 0; k < ig.size(); k++) pos[ig[k]] = (int)k;
+  for (size_t k = 0; k < iv.size(); k++) pos[iv[k]] = (int)k;
+  auto conv = [&](const std::vector<int> &src, std::vector<int> &dst, bool second_is_mol) -> int {
+    dst.resize(src.size());
+    for (size_t k = 0; k < src.size(); k += 2) {
+      int a = src[k], b = src[k + 1];
+      if (a < 0 || a >= h.def.n_mol || pos[a] < 0) return -1;
+      dst[k] = pos[a];
+      if (second_is_mol) {
+        if (b < 0 || b >= h.def.n_mol || pos[b] < 0) return -1;
+        dst[k + 1] = pos[b];
+      } else {
+        dst[k + 1] = b;
+      }
+    }
+    return 0;
+  };
+  if (conv(h.gi, gi, false) || conv(h.gv, gv, true) || conv(h.gg, gg, true)) return -1;
+  return 0;
+}
+
+static int upload_all(mdn_ctx *c) {
+  Layout lay;
+  struct Offs {
+    size_t sp[3][10];
+    size_t pc, bt, ct, dc, dm, lp;
+    size_t mol_x, mol_v, region, ax, av, af, aw, axo, ing, inv, gi, gv, gg, dyn;
+  };
+  std::vector<Offs> offs(c->reps.size());
+  c->any_shake = 0;
+  for (size_t r = 0; r < c->reps.size(); r++) {
+    HostReplica &h = c->reps[r];
+    const SysDef &d = h.def;
+    Offs &o = offs[r];
+    for (int s = 0; s < 3; s++) {
+      const Species &S = d.sp[s];
+      std::vector<double> inv(S.mass.size());
+      for (size_t k = 0; k < S.mass.size(); k++) inv[k] = 1.0 / S.mass[k];
+      o.sp[s][0] = lay.vec(S.mass, true);
+      o.sp[s][1] = lay.vec(inv, true);
+      o.sp[s][2] = lay.vec(S.charge, true);
+      o.sp[s][3] = lay.vec(S.tmpl_x, true);
+      o.sp[s][4] = lay.vec(S.type, true);
+      o.sp[s][5] = lay.vec(S.bond, true);
+      o.sp[s][6] = lay.vec(S.angle, true);
+      o.sp[s][7] = lay.vec(S.dih, true);
+      o.sp[s][8] = lay.vec(S.rigid, true);
+      if (d.shake && S.rigid.size()) c->any_shake = 1;
+    }
+    o.pc = lay.vec(d.pair_coeff, true);
+    o.bt = lay.vec(d.btype, true);
+    o.ct = lay.vec(d.ctype, true);
+    o.dc = lay.vec(d.dcoeff, true);
+    o.dm = lay.vec(d.dmulti, true);
+    h.long_pairs = (d.flags & MDN_ION_AMBER) ? amber_long_pairs(d.sp[0]) : std::vector<int>();
+    o.lp = lay.vec(h.long_pairs, true);
+    o.mol_x = lay.vec(h.mol_x, false);
+    o.mol_v = lay.vec(h.mol_v, false);
+    o.region = lay.vec(h.region, false);
+    o.ax = lay.vec(h.atom_x, false);
+    o.av = lay.vec(h.atom_v, false);
+    o.af = lay.vec(h.atom_f, false);
+    o.aw = lay.vec(h.atom_w, false);
+    o.axo = lay.zeros(d.shake ? sizeof(double) * 3 * (size_t)d.n_atoms : 8);
+    std::vector<int> ig, iv, lgi, lgv, lgg;
+    in_lists(h, ig, iv);
+    if (lists_to_local(h, lgi, lgv, lgg)) return set_error(MDN_E_ARG, "state pair list names a molecule that is not in the domain");
+    ig.resize(std::max(d.n_gas, 1), 0);
+    iv.resize(std::max(d.n_vap, 1), 0);
+    o.ing = lay.vec(ig, false);
+    o.inv = lay.vec(iv, false);
+    lgi.resize(2 * (size_t)std::max(h.cap_gi, 1), 0);
+    lgv.resize(2 * (size_t)std::max(h.cap_gv, 1), 0);
+    lgg.resize(2 * (size_t)std::max(h.cap_gg, 1), 0);
+    o.gi = lay.vec(lgi, false);
+    o.gv = lay.vec(lgv, false);
+    o.gg = lay.vec(lgg, false);
+    Dyn dyn;
+    memset(&dyn, 0, sizeof(dyn));
+    for (int k = 0; k < U_N; k++) dyn.U[k] = h.U[k];
+    dyn.rebuild_count = h.rebuild_count;
+    dyn.loopPair = h.loopPair;
+    dyn.loop_update = h.loop_update;
+    dyn.n_gi = (int)h.gi.size() / 2;
+    dyn.n_gv = (int)h.gv.size() / 2;
+    dyn.n_gg = (int)h.gg.size() / 2;
+    std::vector<int> ig2, iv2;
+    in_lists(h, ig2, iv2);
+    dyn.n_in_gas = (int)ig2.size();
+    dyn.n_in_vap = (int)iv2.size();
+    o.dyn = lay.put(&dyn, sizeof(dyn), false);
+    h.off_dyn = o.dyn;
+  }
+  size_t reps_off = lay.zeros(sizeof(DevReplica) * c->reps.size());
+  if (c->dbuf) { hipFree(c->dbuf); c->dbuf = nullptr; }
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipMalloc(&c->dbuf, lay.host.size()));
+  c->dbytes = lay.host.size();
+  char *B = c->dbuf;
+  std::vector<DevReplica> drs(c->reps.size());
+  for (size_t r = 0; r < c->reps.size(); r++) {
+    HostReplica &h = c->reps[r];
+    const SysDef &d = h.def;
+    Offs &o = offs[r];
+    DevReplica &R = drs[r];
+    memset(&R, 0, sizeof(R));
+    for (int s = 0; s < 3; s++) {
+      const Species &S = d.sp[s];
+      DevSpecies &D = R.sp[s];
+      D.natoms = S.natoms;
+      D.nbond = (int)S.bond.size() / 3; D.nangle = (int)S.angle.size() / 4;
+      D.ndih = (int)S.dih.size() / 5; D.nrigid = d.shake ? (int)S.rigid.size() / 3 : 0;
+      D.molmass = S.molmass;
+      D.mass = at<double>(B, o.sp[s][0]); D.invmass = at<double>(B, o.sp[s][1]);
+      D.charge = at<double>(B, o.sp[s][2]); D.tmpl = at<double>(B, o.sp[s][3]);
+      D.type = at<int>(B, o.sp[s][4]); D.bond = at<int>(B, o.sp[s][5]); D.angle = at<int>(B, o.sp[s][6]);
+      D.dih = at<int>(B, o.sp[s][7]); D.rigid = at<int>(B, o.sp[s][8]);
+    }
+    R.n_gas = d.n_gas; R.n_vap = d.n_vap; R.n_mol = d.n_mol; R.n_atoms = d.n_atoms;
+    R.n_types = (int)d.atypes.size();
+    R.off_gas = d.sp[0].natoms;
+    R.off_vap = d.sp[0].natoms + d.n_gas * d.sp[1].natoms;
+    R.flags = d.flags;
+    R.shake = d.shake; R.shake_max_iter = d.shake_max_iter; R.gas_spin = d.gas_spin;
+    R.ion_type_base = d.ion_type_base;
+    R.dt = d.dt; R.d_size = d.d_size; R.margin = d.margin; R.ML2 = d.ML2;
+    R.qqrd2e = 1.602e-19 * 1.602e-19 / 4.0 / M_PI / 8.85e-12 * 6.02e23 * 1e10 / 4184.0;  // constants.hpp:50
+    R.sig_gas = sqrt(1.38e-23 * d.T / d.m_gas_kg);
+    R.sig_vap = d.m_vap_kg > 0 ? sqrt(1.38e-23 * d.T / d.m_vap_kg) : 0.0;
+    for (int k = 0; k < 3; k++) R.efield[k] = d.efield[k];
+    R.key0 = (uint32_t)h.seed;
+    R.key1 = (uint32_t)(h.seed >> 32) ^ h.replica_id;
+    R.pair_coeff = at<double>(B, o.pc); R.btype = at<double>(B, o.bt); R.ctype = at<double>(B, o.ct);
+    R.dcoeff = at<double>(B, o.dc); R.dmulti = at<int>(B, o.dm);
+    R.n_long = (int)h.long_pairs.size() / 2; R.long_pairs = at<int>(B, o.lp);
+    R.mol_x = at<double>(B, o.mol_x); R.mol_v = at<double>(B, o.mol_v); R.region = at<uint8_t>(B, o.region);
+    R.atom_x = at<double>(B, o.ax); R.atom_v = at<double>(B, o.av); R.atom_f = at<double>(B, o.af);
+    R.atom_w = at<double>(B, o.aw); R.atom_xo = at<double>(B, o.axo);
+    R.in_gas = at<int>(B, o.ing); R.in_vap = at<int>(B, o.inv);
+    R.gi = at<int>(B, o.gi); R.gv = at<int>(B, o.gv); R.gg = at<int>(B, o.gg);
+    R.cap_gi = h.cap_gi; R.cap_gv = h.cap_gv; R.cap_gg = h.cap_gg;
+    R.dyn = at<Dyn>(B, o.dyn);
+    h.dev = R;
+  }
+  memcpy(lay.host.data() + reps_off, drs.data(), sizeof(DevReplica) * drs.size());
+  c->d_reps = at<DevReplica>(B, reps_off);
+  HIPCHK(hipMemcpy(c->dbuf, lay.host.data(), lay.host.size(), hipMemcpyHostToDevice));
+  return MDN_OK;
+}
+
+static int status_to_code(int st) {
+  if (st & (ST_CAP_GI | ST_CAP_GV | ST_CAP_GG | ST_LDS_CAP)) return MDN_E_CAPACITY;
+  if (st & ST_SHAKE) return MDN_E_SHAKE;
+  return MDN_OK;
+}
+
+static int check_replica(mdn_ctx *c, int32_t r) {
+  if (!c) return set_error(MDN_E_ARG, "null context");
+  if (!c->committed) return set_error(MDN_E_STATE, "call mdn_commit first");
+  if (r < 0 || r >= (int)c->reps.size()) return set_error(MDN_E_ARG, "replica index out of range");
+  return MDN_OK;
+}
+
+static int grid_for(int n, int blk = kBlk, int cap = 2048) {
+  int g = (n + blk - 1) / blk;
+  return std::max(1, std::min(g, cap));
+}
+
+// one MD::verlet step of one replica through the per-call kernels
+template <typename Rl>
+static int launch_step(mdn_ctx *c, const HostReplica &h, const DevReplica *R, bool eflag) {
+  const SysDef &d = h.def;
+  int g = grid_for(d.n_atoms);
+  hipLaunchKernelGGL(k_kick, dim3(g), dim3(kBlk), 0, c->stream, R, eflag ? 1 : 0, 0);
+  bool sh = d.shake && (d.sp[0].rigid.size() || d.sp[1].rigid.size() || d.sp[2].rigid.size());
+  hipLaunchKernelGGL(k_drift, dim3(g), dim3(kBlk), 0, c->stream, R, sh ? 1 : 0);
+  if (sh) hipLaunchKernelGGL(k_shake, dim3(grid_for(d.n_mol, 64)), dim3(64), 0, c->stream, R);
+  hipLaunchKernelGGL(k_com_weights, dim3(1), dim3(1024), 0, c->stream, R);
+  hipLaunchKernelGGL(k_rebuild, dim3(1), dim3(1024), 0, c->stream, R, 0);
+  uint32_t bits = d.flags;
+  int tv0 = d.sp[2].natoms ? *std::min_element(d.sp[2].type.begin(), d.sp[2].type.end()) : 0;
+  int ntv = d.sp[2].natoms ? *std::max_element(d.sp[2].type.begin(), d.sp[2].type.end()) - tv0 + 1 : 1;
+  bool tiled = ntv <= 8;
+  hipLaunchKernelGGL(k_terms<Rl>, dim3(512), dim3(kBlk), 0, c->stream, R, tiled ? bits & ~(uint32_t)MDN_VV : bits,
+                     eflag ? 1 : 0);
+  if ((bits & MDN_VV) && tiled) {
+    int nva = d.n_vap * d.sp[2].natoms;
+    int gx = std::max(1, (nva + kVVTile - 1) / kVVTile);
+    int gy = std::max(1, std::min(64, 2048 / gx));
+    hipLaunchKernelGGL(k_vv_tiled<Rl>, dim3(gx, gy), dim3(kVVTile), 0, c->stream, R, eflag ? 1 : 0, tv0, ntv);
+  }
+  hipLaunchKernelGGL(k_kick, dim3(g), dim3(kBlk), 0, c->stream, R, 0, 1);
+  HIPCHK(hipGetLastError());
+  return MDN_OK;
+}
+
+}  // namespace mdn
+
+using namespace mdn;
+
+extern "C" {
+
+const char *mdn_last_error(void) { return g_err.c_str(); }
+
+int mdn_create(int device, mdn_ctx **out) {
+  if (!out) return set_error(MDN_E_ARG, "mdn_create: null out");
+  int n = 0;
+  HIPCHK(hipGetDeviceCount(&n));
+  if (device < 0 || device >= n) return set_error(MDN_E_ARG, "mdn_create: no such device");
+  mdn_ctx *c = new mdn_ctx();
+  c->device = device;
+  HIPCHK(hipSetDevice(device));
+  HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  HIPCHK(hipEventCreate(&c->ev0));
+  HIPCHK(hipEventCreate(&c->ev1));
+  *out = c;
+  return MDN_OK;
+}
+
+void mdn_destroy(mdn_ctx *c) {
+  if (!c) return;
+  hipSetDevice(c->device);
+  if (c->stream) hipStreamSynchronize(c->stream);
+  if (c->dbuf) hipFree(c->dbuf);
+  if (c->ev0) hipEventDestroy(c->ev0);
+  if (c->ev1) hipEventDestroy(c->ev1);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+}
+
+static int copy_state_in(HostReplica &h, const mdn_state *st) {
+  const SysDef &d = h.def;
+  int nm = d.n_mol, na = d.n_atoms;
+  if (!st->mol_x || !st->mol_v || !st->region || !st->atom_x || !st->atom_v || !st->atom_f || !st->atom_w)
+    return set_error(MDN_E_ARG, "state: null array");
+  h.mol_x.assign(st->mol_x, st->mol_x + 3 * (size_t)nm);
+  h.mol_v.assign(st->mol_v, st->mol_v + 3 * (size_t)nm);
+  h.region.assign(st->region, st->region + nm);
+  h.atom_x.assign(st->atom_x, st->atom_x + 3 * (size_t)na);
+  h.atom_v.assign(st->atom_v, st->atom_v + 3 * (size_t)na);
+  h.atom_f.assign(st->atom_f, st->atom_f + 3 * (size_t)na);
+  h.atom_w.assign(st->atom_w, st->atom_w + na);
+  h.region[0] = 1;
+  h.loopPair = st->loopPair;
+  h.loop_update = st->loop_update;
+  h.rebuild_count = st->rebuild_count;
+  for (int k = 0; k < U_N; k++) h.U[k] = st->U[k];
+  h.gi.assign(st->gi ? st->gi : nullptr, st->gi ? st->gi + 2 * (size_t)st->n_gi : nullptr);
+  h.gv.assign(st->gv ? st->gv : nullptr, st->gv ? st->gv + 2 * (size_t)st->n_gv : nullptr);
+  h.gg.assign(st->gg ? st->gg : nullptr, st->gg ? st->gg + 2 * (size_t)st->n_gg : nullptr);
+  // list capacities: the caller's, else an estimate from the gas density in
+  // the r < 110 A sphere (x3 + 64 molecules of headroom)
+  double vol_dom = 4.0 / 3.0 * M_PI * 110.0 * 110.0 * 110.0;
+  double frac = d.d_size > 0 ? vol_dom / (d.d_size * d.d_size * d.d_size) : 1.0;
+  int g_est = (int)std::min<double>(d.n_gas, 3.0 * d.n_gas * std::min(1.0, frac) + 64);
+  int v_est = (int)std::min<double>(d.n_vap, 3.0 * d.n_vap * std::min(1.0, frac) + 64);
+  h.cap_gi = st->cap_gi > 0 ? st->cap_gi : std::max(64, g_est * d.sp[0].natoms);
+  h.cap_gv = st->cap_gv > 0 ? st->cap_gv : std::max(64, g_est * std::max(1, v_est));
+  h.cap_gg = st->cap_gg > 0 ? st->cap_gg : std::max(64, g_est * 32);
+  h.cap_gi = std::max<int>(h.cap_gi, (int)h.gi.size() / 2);
+  h.cap_gv = std::max<int>(h.cap_gv, (int)h.gv.size() / 2);
+  h.cap_gg = std::max<int>(h.cap_gg, (int)h.gg.size() / 2);
+  return MDN_OK;
+}
+
+int mdn_add_replica(mdn_ctx *c, const mdn_sysdef *sd, const mdn_state *st, uint64_t seed, uint32_t replica_id,
+                    int32_t *handle) {
+  if (!c || !sd || !st) return set_error(MDN_E_ARG, "mdn_add_replica: null argument");
+  c->reps.emplace_back();
+  HostReplica &h = c->reps.back();
+  h.def = *reinterpret_cast<const SysDef *>(sd);
+  h.seed = seed;
+  h.replica_id = replica_id;
+  int rc = copy_state_in(h, st);
+  if (rc) { c->reps.pop_back(); return rc; }
+  c->committed = false;
+  if (handle) *handle = (int32_t)c->reps.size() - 1;
+  return MDN_OK;
+}
+
+int mdn_commit(mdn_ctx *c) {
+  if (!c) return set_error(MDN_E_ARG, "mdn_commit: null context");
+  if (c->reps.empty()) return set_error(MDN_E_STATE, "mdn_commit: no replicas");
+  int rc = upload_all(c);
+  if (rc) return rc;
+  c->committed = true;
+  return MDN_OK;
+}
+
+int mdn_set_state(mdn_ctx *c, int32_t r, const mdn_state *st) {
+  int rc = check_replica(c, r);
+  if (rc) return rc;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  rc = copy_state_in(c->reps[r], st);
+  if (rc) return rc;
+  return mdn_commit(c);
+}
+
+int mdn_get_state(mdn_ctx *c, int32_t r, mdn_state *st) {
+  int rc = check_replica(c, r);
+  if (rc) return rc;
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  HostReplica &h = c->reps[r];
+  const DevReplica &R = h.dev;
+  const SysDef &d = h.def;
+  int nm = d.n_mol, na = d.n_atoms;
+  HIPCHK(hipMemcpy(st->mol_x, R.mol_x, sizeof(double) * 3 * nm, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(st->mol_v, R.mol_v, sizeof(double) * 3 * nm, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(st->region, R.region, nm, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(st->atom_x, R.atom_x, sizeof(double) * 3 * na, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(st->atom_v, R.atom_v, sizeof(double) * 3 * na, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(st->atom_f, R.atom_f, sizeof(double) * 3 * na, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(st->atom_w, R.atom_w, sizeof(double) * na, hipMemcpyDeviceToHost));
+  Dyn dyn;
+  HIPCHK(hipMemcpy(&dyn, R.dyn, sizeof(Dyn), hipMemcpyDeviceToHost));
+  st->loopPair = dyn.loopPair;
+  st->loop_update = dyn.loop_update;
+  st->rebuild_count = dyn.rebuild_count;
+  for (int k = 0; k < U_N; k++) st->U[k] = dyn.U[k];
+  std::vector<int> ig(std::max(dyn.n_in_gas, 1)), iv(std::max(dyn.n_in_vap, 1));
+  if (dyn.n_in_gas) HIPCHK(hipMemcpy(ig.data(), R.in_gas, sizeof(int) * dyn.n_in_gas, hipMemcpyDeviceToHost));
+  if (dyn.n_in_vap) HIPCHK(hipMemcpy(iv.data(), R.in_vap, sizeof(int) * dyn.n_in_vap, hipMemcpyDeviceToHost));
+  auto fetch = [&](const int *src, int n, int cap, int *dst, int *ndst, int kind) -> int {
+    *ndst = n;
+    if (!dst || n == 0) return MDN_OK;
+    if (n > cap) return set_error(MDN_E_CAPACITY, "mdn_get_state: caller list capacity too small");
+    std::vector<int> tmp(2 * (size_t)n);
+    hipError_t e = hipMemcpy(tmp.data(), src, sizeof(int) * 2 * n, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return set_error(MDN_E_HIP, hipGetErrorString(e));
+    for (int k = 0; k < n; k++) {
+      dst[2 * k] = ig[tmp[2 * k]];
+      dst[2 * k + 1] = kind == 0 ? tmp[2 * k + 1] : (kind == 1 ? iv[tmp[2 * k + 1]] : ig[tmp[2 * k + 1]]);
+    }
+    return MDN_OK;
+  };
+  if ((rc = fetch(R.gi, dyn.n_gi, st->cap_gi, st->gi, &st->n_gi, 0))) return rc;
+  if ((rc = fetch(R.gv, dyn.n_gv, st->cap_gv, st->gv, &st->n_gv, 1))) return rc;
+  if ((rc = fetch(R.gg, dyn.n_gg, st->cap_gg, st->gg, &st->n_gg, 2))) return rc;
+  return status_to_code(dyn.status) ? set_error(status_to_code(dyn.status), "replica status flags set") : MDN_OK;
+}
+
+int mdn_make_pair(mdn_ctx *c, int32_t r) {
+  int rc = check_replica(c, r);
+  if (rc) return rc;
+  HIPCHK(hipSetDevice(c->device));
+  hipLaunchKernelGGL(k_rebuild, dim3(1), dim3(1024), 0, c->stream, c->d_reps + r, 1);
+  HIPCHK(hipGetLastError());
+  return MDN_OK;
+}
+
+int mdn_zero_forces(mdn_ctx *c, int32_t r) {
+  int rc = check_replica(c, r);
+  if (rc) return rc;
+  const DevReplica &R = c->reps[r].dev;
+  HIPCHK(hipMemsetAsync(R.atom_f, 0, sizeof(double) * 3 * (size_t)c->reps[r].def.n_atoms, c->stream));
+  HIPCHK(hipMemsetAsync(R.dyn->U, 0, sizeof(double) * U_N, c->stream));
+  return MDN_OK;
+}
+
+int mdn_compute(mdn_ctx *c, int32_t r, uint32_t bits, int32_t eflag, int32_t precision) {
+  int rc = check_replica(c, r);
+  if (rc) return rc;
+  HIPCHK(hipSetDevice(c->device));
+  const HostReplica &h = c->reps[r];
+  if (precision == MDN_MIXED)
+    hipLaunchKernelGGL(k_terms<float>, dim3(512), dim3(kBlk), 0, c->stream, c->d_reps + r, bits, eflag);
+  else
+    hipLaunchKernelGGL(k_terms<double>, dim3(512), dim3(kBlk), 0, c->stream, c->d_reps + r, bits, eflag);
+  HIPCHK(hipGetLastError());
+  return MDN_OK;
+}
+
+int mdn_velocity_calculation(mdn_ctx *c, int32_t r) {
+  int rc = check_replica(c, r);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_kick, dim3(grid_for(c->reps[r].def.n_atoms)), dim3(kBlk), 0, c->stream, c->d_reps + r, 0, 0);
+  HIPCHK(hipGetLastError());
+  return MDN_OK;
+}
+
+int mdn_update_position(mdn_ctx *c, int32_t r) {
+  int rc = check_replica(c, r);
+  if (rc) return rc;
+  const HostReplica &h = c->reps[r];
+  bool sh = h.def.shake && (h.def.sp[0].rigid.size() || h.def.sp[1].rigid.size() || h.def.sp[2].rigid.size());
+  const DevReplica *R = c->d_reps + r;
+  hipLaunchKernelGGL(k_drift, dim3(grid_for(h.def.n_atoms)), dim3(kBlk), 0, c->stream, R, sh ? 1 : 0);
+  if (sh) hipLaunchKernelGGL(k_shake, dim3(grid_for(h.def.n_mol, 64)), dim3(64), 0, c->stream, R);
+  hipLaunchKernelGGL(k_com_weights, dim3(1), dim3(1024), 0, c->stream, R);
+  HIPCHK(hipGetLastError());
+  return MDN_OK;
+}
+
+int mdn_verlet(mdn_ctx *c, int32_t r, int32_t nsteps, int32_t eflag_every, int64_t step0, int32_t precision) {
+  int rc = check_replica(c, r);
+  if (rc) return rc;
+  HIPCHK(hipSetDevice(c->device));
+  const HostReplica &h = c->reps[r];
+  HIPCHK(hipEventRecord(c->ev0, c->stream));
+  for (int s = 0; s < nsteps; s++) {
+    bool ef = eflag_every > 0 && ((step0 + s + 1) % eflag_every) == 0;
+    rc = precision == MDN_MIXED ? launch_step<float>(c, h, c->d_reps + r, ef) : launch_step<double>(c, h, c->d_reps + r, ef);
+    if (rc) return rc;
+  }
+  HIPCHK(hipEventRecord(c->ev1, c->stream));
+  return MDN_OK;
+}
+
+int mdn_run_batched(mdn_ctx *c, int32_t nsteps, int32_t eflag_every, int64_t step0, int32_t precision) {
+  if (!c) return set_error(MDN_E_ARG, "null context");
+  if (!c->committed) return set_error(MDN_E_STATE, "call mdn_commit first");
+  HIPCHK(hipSetDevice(c->device));
+  int cap = c->batch_cap;
+  if (cap <= 0) {
+    // LDS budget for two workgroups per CU (160 KB of LDS per CU)
+    int need = 16;
+    for (auto &h : c->reps) need = std::max(need, h.def.n_atoms);
+    cap = 32;
+    while (cap + 32 <= need + 31 && batch_lds_bytes(cap + 32, c->any_shake) <= 80 * 1024) cap += 32;
+  }
+  size_t lds = batch_lds_bytes(cap, c->any_shake);
+  if (lds > 160 * 1024) return set_error(MDN_E_ARG, "mdn_run_batched: LDS capacity too large");
+  int n = (int)c->reps.size();
+  int grid = c->batch_grid > 0 ? std::min(c->batch_grid, n) : n;
+  HIPCHK(hipEventRecord(c->ev0, c->stream));
+  if (precision == MDN_MIXED) {
+    HIPCHK(hipFuncSetAttribute((const void *)k_batched<float>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(k_batched<float>, dim3(grid), dim3(c->batch_threads), lds, c->stream, c->d_reps, n, nsteps,
+                       eflag_every, (long long)step0, cap, c->any_shake);
+  } else {
+    HIPCHK(hipFuncSetAttribute((const void *)k_batched<double>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(k_batched<double>, dim3(grid), dim3(c->batch_threads), lds, c->stream, c->d_reps, n, nsteps,
+                       eflag_every, (long long)step0, cap, c->any_shake);
+  }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipEventRecord(c->ev1, c->stream));
+  return MDN_OK;
+}
+
+int mdn_set_batch_shape(mdn_ctx *c, int32_t threads, int32_t lds_cap_atoms, int32_t grid) {
+  if (!c || (threads != 64 && threads != 128 && threads != 256)) return set_error(MDN_E_ARG, "mdn_set_batch_shape: threads must be 64, 128 or 256");
+  c->batch_threads = threads;
+  c->batch_cap = lds_cap_atoms;
+  c->batch_grid = grid;
+  return MDN_OK;
+}
+
+int mdn_get_counters(mdn_ctx *c, int32_t r, int64_t *atom_steps, int64_t *rebuilds, int32_t *shake_iters_max) {
+  int rc = check_replica(c, r);
+  if (rc) return rc;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  Dyn dyn;
+  HIPCHK(hipMemcpy(&dyn, c->reps[r].dev.dyn, sizeof(Dyn), hipMemcpyDeviceToHost));
+  if (atom_steps) *atom_steps = dyn.atom_steps;
+  if (rebuilds) *rebuilds = dyn.rebuild_count;
+  if (shake_iters_max) *shake_iters_max = dyn.shake_iters_max;
+  int code = status_to_code(dyn.status);
+  return code ? set_error(code, "replica status flags set: " + std::to_string(dyn.status)) : MDN_OK;
+}
+
+int mdn_sync(mdn_ctx *c) {
+  if (!c) return set_error(MDN_E_ARG, "null context");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return MDN_OK;
+}
+
+int mdn_last_kernel_ms(mdn_ctx *c, double *ms) {
+  if (!c || !ms) return set_error(MDN_E_ARG, "null argument");
+  float f = 0;
+  HIPCHK(hipEventSynchronize(c->ev1));
+  HIPCHK(hipEventElapsedTime(&f, c->ev0, c->ev1));
+  *ms = f;
+  return MDN_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,759 @@
+// mdn_terms.h -- the per-step operations of one replica, written once over a
+// store policy (GStore: global memory; SStore: LDS), plus the workgroup-level
+// pair-list rebuild.  Every loop is "for (p = t; p < n; p += nt)" over a flat
+// index space, so the same code runs grid-wide (per-call path: t = global
+// thread id) or inside one workgroup (batched path: t = threadIdx.x).
+// Forces are accumulated with FP64 atomics (global_atomic_add_f64 /
+// ds_add_f64 on gfx950); energies per thread in e[U_N].
+#pragma once
+#include "mdn_device.h"
+
+namespace mdn {
+
+// Local view of the in-domain set: atom order [ion | vapor | gas].
+struct LView {
+  int n_ion, nv, ng, na_v, na_g;
+  int vbase, gbase, n_in;
+  int n_gi, n_gv, n_gg;
+  __host__ __device__ void set(int nion, int nvin, int ngin, int nav, int nag) {
+    n_ion = nion; nv = nvin; ng = ngin; na_v = nav; na_g = nag;
+    vbase = nion; gbase = nion + nvin * nav; n_in = gbase + ngin * nag;
+  }
+};
+
+__device__ inline LView view_from_dyn(const DevReplica &R) {
+  LView L;
+  L.set(R.sp[0].natoms, R.dyn->n_in_vap, R.dyn->n_in_gas, R.sp[2].natoms, R.sp[1].natoms);
+  L.n_gi = R.dyn->n_gi; L.n_gv = R.dyn->n_gv; L.n_gg = R.dyn->n_gg;
+  return L;
+}
+
+// global-memory store: local index -> (global atom, species, index in molecule)
+struct GStore {
+  const DevReplica *R;
+  LView L;
+  struct H { int g, sp, idx; };
+  __device__ H h(int a) const {
+    H r;
+    if (a < L.n_ion) { r.g = a; r.sp = 0; r.idx = a; return r; }
+    if (a < L.gbase) {
+      int k = (a - L.vbase) / L.na_v, i = a - L.vbase - k * L.na_v;
+      r.g = R->off_vap + (R->in_vap[k] - 1 - R->n_gas) * L.na_v + i; r.sp = 2; r.idx = i; return r;
+    }
+    int k = (a - L.gbase) / L.na_g, i = a - L.gbase - k * L.na_g;
+    r.g = R->off_gas + (R->in_gas[k] - 1) * L.na_g + i; r.sp = 1; r.idx = i; return r;
+  }
+  __device__ void pos(const H &q, double p[3]) const {
+    const double *x = R->atom_x + 3 * (size_t)q.g; p[0] = x[0]; p[1] = x[1]; p[2] = x[2];
+  }
+  __device__ void fadd(const H &q, double fx, double fy, double fz) const {
+    double *f = R->atom_f + 3 * (size_t)q.g;
+    atomicAdd(f, fx); atomicAdd(f + 1, fy); atomicAdd(f + 2, fz);
+  }
+  __device__ double w(const H &q) const { return R->atom_w[q.g]; }
+  __device__ double q(const H &h_) const { return R->sp[h_.sp].charge[h_.idx]; }
+  __device__ int type(const H &h_) const { return R->sp[h_.sp].type[h_.idx]; }
+};
+
+// LDS store (one workgroup per replica)
+struct SStore {
+  LView L;
+  double *px, *pf, *pw, *pq;
+  const int *ptype;
+  typedef int H;
+  __device__ H h(int a) const { return a; }
+  __device__ void pos(H a, double p[3]) const { p[0] = px[3 * a]; p[1] = px[3 * a + 1]; p[2] = px[3 * a + 2]; }
+  __device__ void fadd(H a, double fx, double fy, double fz) const {
+    atomicAdd(&pf[3 * a], fx); atomicAdd(&pf[3 * a + 1], fy); atomicAdd(&pf[3 * a + 2], fz);
+  }
+  __device__ double w(H a) const { return pw[a]; }
+  __device__ double q(H a) const { return pq[a]; }
+  __device__ int type(H a) const { return ptype[a]; }
+};
+
+__device__ inline void pc(const DevReplica &R, int t1, int t2, double &c0, double &c1) {
+  const double *p = R.pair_coeff + 2 * ((size_t)t1 * R.n_types + t2);
+  c0 = p[0]; c1 = p[1];
+}
+
+// one LJ / LJcoul atom pair: F_a += fp*d, F_b -= fp*d (d = x_a - x_b)
+template <typename Rl, bool COUL, class S>
+__device__ inline void pair_ab(const DevReplica &R, const S &s, const typename S::H &ha, const typename S::H &hb,
+                               bool weighted, bool eflag, double &e) {
+  double xa[3], xb[3];
+  s.pos(ha, xa);
+  s.pos(hb, xb);
+  double dx = xa[0] - xb[0], dy = xa[1] - xb[1], dz = xa[2] - xb[2];
+  double c0, c1;
+  pc(R, s.type(ha), s.type(hb), c0, c1);
+  Rl rx = (Rl)dx, ry = (Rl)dy, rz = (Rl)dz;
+  Rl rsq = rx * rx + ry * ry + rz * rz;
+  Rl w = weighted ? (Rl)(s.w(ha) * s.w(hb)) : Rl(1);
+  Rl fp;
+  if (COUL)
+    fp = ljcoul_fpair<Rl>(rsq, (Rl)c0, (Rl)c1, (Rl)(R.qqrd2e * s.q(ha) * s.q(hb)), w, eflag, e);
+  else
+    fp = lj_fpair<Rl>(rsq, (Rl)c0, (Rl)c1, w, eflag, e);
+  double fx = (double)(fp * rx), fy = (double)(fp * ry), fz = (double)(fp * rz);
+  s.fadd(ha, fx, fy, fz);
+  s.fadd(hb, -fx, -fy, -fz);
+}
+
+// ---- gi: PotentialGasIon::compute, GasIon.cpp:4-38 (list entries (gas
+// molecule, ion atom); every gas atom of the molecule; unweighted) ----
+template <typename Rl, class S>
+__device__ void term_gi(const DevReplica &R, const S &s, int t, int nt, bool eflag, double *e) {
+  const LView &L = s.L;
+  int n = L.n_gi * L.na_g;
+  for (int p = t; p < n; p += nt) {
+    int ent = p / L.na_g, ii = p - ent * L.na_g;
+    int k = R.gi[2 * ent], j = R.gi[2 * ent + 1];
+    pair_ab<Rl, false>(R, s, s.h(L.gbase + k * L.na_g + ii), s.h(j), false, eflag, e[U_GI]);
+  }
+}
+
+// ---- vi: PotentialVaporIon::compute, VaporIon.cpp:5-43 (all in-domain
+// vapor atoms x all ion atoms, weighted) ----
+template <typename Rl, class S>
+__device__ void term_vi(const DevReplica &R, const S &s, int t, int nt, bool eflag, double *e) {
+  const LView &L = s.L;
+  int n = L.nv * L.na_v * L.n_ion;
+  for (int p = t; p < n; p += nt) {
+    int a = p / L.n_ion, j = p - a * L.n_ion;
+    pair_ab<Rl, true>(R, s, s.h(L.vbase + a), s.h(j), true, eflag, e[U_VI]);
+  }
+}
+
+// ---- gg: gas-gas LJ over the COM list, weighted w_a w_b (the evident
+// intent of GasGas.cpp:4-63, whose list indexes loop counters) ----
+template <typename Rl, class S>
+__device__ void term_gg(const DevReplica &R, const S &s, int t, int nt, bool eflag, double *e) {
+  const LView &L = s.L;
+  int nn = L.na_g * L.na_g, n = L.n_gg * nn;
+  for (int p = t; p < n; p += nt) {
+    int ent = p / nn, r = p - ent * nn, ii = r / L.na_g, jj = r - ii * L.na_g;
+    int k = R.gg[2 * ent], l = R.gg[2 * ent + 1];
+    pair_ab<Rl, false>(R, s, s.h(L.gbase + k * L.na_g + ii), s.h(L.gbase + l * L.na_g + jj), true, eflag,
+                       e[U_GG]);
+  }
+}
+
+// ---- gv: PotentialVaporGas::compute, VaporGas.cpp:5-44 (unweighted; the
+// inner bound is the gas atom count, VaporGas.cpp:15) ----
+template <typename Rl, class S>
+__device__ void term_gv(const DevReplica &R, const S &s, int t, int nt, bool eflag, double *e) {
+  const LView &L = s.L;
+  int nj = L.na_g < L.na_v ? L.na_g : L.na_v;
+  int nn = L.na_g * nj, n = L.n_gv * nn;
+  for (int p = t; p < n; p += nt) {
+    int ent = p / nn, r = p - ent * nn, ii = r / nj, jj = r - ii * nj;
+    int k = R.gv[2 * ent], l = R.gv[2 * ent + 1];
+    pair_ab<Rl, false>(R, s, s.h(L.gbase + k * L.na_g + ii), s.h(L.vbase + l * L.na_v + jj), false, eflag,
+                       e[U_VG]);
+  }
+}
+
+// ---- vv: PotentialVaporVapor::compute, VaporVapor.cpp:5-45 (all atom pairs
+// of distinct in-domain vapor molecules, weighted) ----
+template <typename Rl, class S>
+__device__ void term_vv(const DevReplica &R, const S &s, int t, int nt, bool eflag, double *e) {
+  const LView &L = s.L;
+  int nva = L.nv * L.na_v;
+  // pairs (a, b) with molecule(a) < molecule(b): iterate the strictly upper
+  // block triangle row by row
+  long long n = (long long)nva * nva;
+  for (long long p = t; p < n; p += nt) {
+    int a = (int)(p / nva), b = (int)(p - (long long)a * nva);
+    if (a / L.na_v >= b / L.na_v) continue;
+    pair_ab<Rl, true>(R, s, s.h(L.vbase + a), s.h(L.vbase + b), true, eflag, e[U_VV]);
+  }
+}
+
+// ---- Efield: PotentialEfield::compute, electricField.cpp:54-64 ----
+template <class S>
+__device__ void term_efield(const DevReplica &R, const S &s, int t, int nt) {
+  for (int i = t; i < s.L.n_ion; i += nt) {
+    typename S::H h = s.h(i);
+    double c = 5.55e-5 * s.q(h);
+    s.fadd(h, c * R.efield[0], c * R.efield[1], c * R.efield[2]);
+  }
+}
+
+// bonded terms of one species' molecule whose first local atom is `base`
+template <class S>
+__device__ inline void bond_term(const DevReplica &R, const S &s, int base, const int *b, bool eflag, double &e) {
+  typename S::H hi = s.h(base + b[0]), hj = s.h(base + b[1]);
+  double xi[3], xj[3];
+  s.pos(hi, xi); s.pos(hj, xj);
+  double d[3] = {xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2]};
+  double fb = bond_fb(d[0] * d[0] + d[1] * d[1] + d[2] * d[2], R.btype[2 * b[2]], R.btype[2 * b[2] + 1], eflag, e);
+  s.fadd(hi, fb * d[0], fb * d[1], fb * d[2]);
+  s.fadd(hj, -fb * d[0], -fb * d[1], -fb * d[2]);
+}
+template <class S>
+__device__ inline void angle_term(const DevReplica &R, const S &s, int base, const int *a, bool eflag, double &e) {
+  typename S::H hi = s.h(base + a[0]), hj = s.h(base + a[1]), hk = s.h(base + a[2]);
+  double xi[3], xj[3], xk[3], f1[3], f3[3];
+  s.pos(hi, xi); s.pos(hj, xj); s.pos(hk, xk);
+  double d1[3] = {xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2]};
+  double d2[3] = {xk[0] - xj[0], xk[1] - xj[1], xk[2] - xj[2]};
+  angle_f(d1, d2, R.ctype[2 * a[3]], R.ctype[2 * a[3] + 1], eflag, e, f1, f3);
+  s.fadd(hi, f1[0], f1[1], f1[2]);
+  s.fadd(hj, -(f1[0] + f3[0]), -(f1[1] + f3[1]), -(f1[2] + f3[2]));
+  s.fadd(hk, f3[0], f3[1], f3[2]);
+}
+template <class S>
+__device__ inline void dihedral_term(const DevReplica &R, const S &s, int base, const int *d, bool eflag,
+                                     double &e) {
+  typename S::H h1 = s.h(base + d[0]), h2 = s.h(base + d[1]), h3 = s.h(base + d[2]), h4 = s.h(base + d[3]);
+  double x1[3], x2[3], x3[3], x4[3], f1[3], f2[3], f3[3], f4[3];
+  s.pos(h1, x1); s.pos(h2, x2); s.pos(h3, x3); s.pos(h4, x4);
+  double vb1[3] = {x1[0] - x2[0], x1[1] - x2[1], x1[2] - x2[2]};
+  double vb2[3] = {x3[0] - x2[0], x3[1] - x2[1], x3[2] - x2[2]};
+  double vb3[3] = {x4[0] - x3[0], x4[1] - x3[1], x4[2] - x3[2]};
+  dihedral_f(vb1, vb2, vb3, R.dcoeff + 15 * d[4], R.dmulti[d[4]], eflag, e, f1, f2, f3, f4);
+  s.fadd(h1, f1[0], f1[1], f1[2]);
+  s.fadd(h2, f2[0], f2[1], f2[2]);
+  s.fadd(h3, f3[0], f3[1], f3[2]);
+  s.fadd(h4, f4[0], f4[1], f4[2]);
+}
+
+// ---- AMBER ion: PotentialAMBER::compute, compute_AMBER.cpp:10-253
+// (non-excluded pairs from initialAMBER, unweighted LJcoul; bonds, angles,
+// dihedrals; all into Uion) ----
+template <typename Rl, class S>
+__device__ void term_amber(const DevReplica &R, const S &s, int t, int nt, bool eflag, double *e) {
+  for (int p = t; p < R.n_long; p += nt)
+    pair_ab<Rl, true>(R, s, s.h(R.long_pairs[2 * p]), s.h(R.long_pairs[2 * p + 1]), false, eflag, e[U_ION]);
+  const DevSpecies &I = R.sp[0];
+  for (int p = t; p < I.nbond; p += nt) bond_term(R, s, 0, I.bond + 3 * p, eflag, e[U_ION]);
+  for (int p = t; p < I.nangle; p += nt) angle_term(R, s, 0, I.angle + 4 * p, eflag, e[U_ION]);
+  for (int p = t; p < I.ndih; p += nt) dihedral_term(R, s, 0, I.dih + 5 * p, eflag, e[U_ION]);
+}
+
+// ---- vapor intra: PotentialVaporIntra::compute, compute_intra.cpp:45-245
+// (bonds + angles -> Ugas, dihedrals -> Uion, as labelled there) ----
+template <class S>
+__device__ void term_vintra(const DevReplica &R, const S &s, int t, int nt, bool eflag, double *e) {
+  const LView &L = s.L;
+  const DevSpecies &V = R.sp[2];
+  int per = V.nbond + V.nangle + V.ndih;
+  int n = L.nv * per;
+  for (int p = t; p < n; p += nt) {
+    int k = p / per, r = p - k * per;
+    int base = L.vbase + k * L.na_v;
+    if (r < V.nbond) bond_term(R, s, base, V.bond + 3 * r, eflag, e[U_GAS]);
+    else if (r < V.nbond + V.nangle) angle_term(R, s, base, V.angle + 4 * (r - V.nbond), eflag, e[U_GAS]);
+    else dihedral_term(R, s, base, V.dih + 5 * (r - V.nbond - V.nangle), eflag, e[U_ION]);
+  }
+}
+
+// ---- gas intra: PotentialGasIntra::compute, compute_intra.cpp:14-41 ----
+template <class S>
+__device__ void term_gintra(const DevReplica &R, const S &s, int t, int nt, bool eflag, double *e) {
+  const LView &L = s.L;
+  const DevSpecies &G = R.sp[1];
+  int n = L.ng * G.nbond;
+  for (int p = t; p < n; p += nt) {
+    int k = p / G.nbond, r = p - k * G.nbond;
+    bond_term(R, s, L.gbase + k * L.na_g, G.bond + 3 * r, eflag, e[U_GAS]);
+  }
+}
+
+// ---- Born-Mayer-Huggins NaCl: PotentialBorn::compute, compute_born.cpp:10-56
+// with the coefficient table of potentialBMHFT.hpp:13-41 indexed by ion-local
+// type (0 = Na, 1 = Cl) ----
+__device__ inline void born_coeff(int t1, int t2, double B[5]) {
+  int k = t1 + t2;  // 0 NaNa, 1 NaCl, 2 ClCl
+  const double A[3] = {25.4435, 20.3548, 15.2661}, C[3] = {101.1719, 674.4793, 6985.6786},
+               D[3] = {48.1771, 837.077, 14031.5785}, sg[3] = {2.340, 2.755, 3.170};
+  B[0] = A[k] * 3.1546 / 4.184;
+  B[1] = 6 * C[k] / 4.184;
+  B[2] = 8 * D[k] / 4.184;
+  B[3] = sg[k];
+  B[4] = 3.1546;
+}
+template <class S>
+__device__ void term_born(const DevReplica &R, const S &s, int t, int nt, bool eflag, double *e) {
+  int is = s.L.n_ion, n = is * is;
+  for (int p = t; p < n; p += nt) {
+    int i = p / is, j = p - i * is;
+    if (i >= j) continue;
+    typename S::H hi = s.h(i), hj = s.h(j);
+    double xi[3], xj[3];
+    s.pos(hi, xi); s.pos(hj, xj);
+    double dx = xi[0] - xj[0], dy = xi[1] - xj[1], dz = xi[2] - xj[2];
+    double rsq = dx * dx + dy * dy + dz * dz;
+    double r2inv = 1 / rsq, r6inv = r2inv * r2inv * r2inv, r = sqrt(rsq);
+    int t1 = s.type(hi) - R.ion_type_base, t2 = s.type(hj) - R.ion_type_base;
+    t1 = t1 < 0 ? 0 : (t1 > 1 ? 1 : t1);
+    t2 = t2 < 0 ? 0 : (t2 > 1 ? 1 : t2);
+    double B[5];
+    born_coeff(t1, t2, B);
+    double rexp = exp(B[4] * (B[3] - r));
+    double f1 = B[4] * B[0] * r * rexp, f2 = -B[1] * r6inv, f3 = -B[2] * r6inv * r2inv;
+    double fc = R.qqrd2e * s.q(hi) * s.q(hj) * sqrt(r2inv);
+    double fp = (f1 + f2 + f3 + fc) * r2inv;
+    s.fadd(hi, fp * dx, fp * dy, fp * dz);
+    s.fadd(hj, -fp * dx, -fp * dy, -fp * dz);
+    if (eflag) e[U_ION] += fc + rexp * B[0] - B[1] / 6.0 * r6inv - B[2] / 8.0 * r6inv * r2inv;
+  }
+}
+
+// ---- Tersoff (Tersoff/Tersoff.cpp:13-297, Si constants of
+// potentialTersoff.hpp:15-39) with its neighbour list built; forces and
+// energy converted eV -> kcal/mol by 23.06 on its own contribution ----
+struct TersP {
+  double gamma = 1.0, lamda3 = 0, c2 = 100390.0 * 100390.0, d2 = 16.218 * 16.218, h0 = -0.59826, n = 0.78734,
+         beta = 1.0999e-6, lamda2 = 1.7322, B = 471.18, R = 2.85, D = 0.15, lamda1 = 2.4799, A = 1830.8;
+};
+__device__ inline double t_fC(const TersP &p, double r) {
+  if (r < p.R - p.D) return 1;
+  if (r > p.R + p.D) return 0;
+  return 0.5 * (1 - sin(kPI * 0.5 * (r - p.R) / p.D));
+}
+__device__ inline double t_fC_d(const TersP &p, double r) {
+  if (r < p.R - p.D || r > p.R + p.D) return 0;
+  return -0.25 * kPI / p.D * cos(kPI * 0.5 * (r - p.R) / p.D);
+}
+__device__ inline double t_g(const TersP &p, double ct) {
+  double h = p.h0 - ct;
+  return p.gamma * (1.0 + p.c2 / p.d2 - p.c2 / (p.d2 + h * h));
+}
+__device__ inline double t_g_d(const TersP &p, double ct) {
+  double h = p.h0 - ct, den = 1 / (p.d2 + h * h);
+  return p.gamma * (-2.0 * p.c2 * h) * den * den;
+}
+__device__ inline double t_bij(const TersP &p, double z) {
+  if (z == 0) return 1.0;
+  return pow(1.0 + pow(p.beta * z, p.n), -1.0 / (2.0 * p.n));
+}
+__device__ inline double t_bij_d(const TersP &p, double z) {
+  double tn = pow(p.beta * z, p.n);
+  return -0.5 * pow(1.0 + tn, -1.0 - 1.0 / (2.0 * p.n)) * tn / z;
+}
+template <class S>
+__device__ void term_tersoff(const DevReplica &R, const S &s, int t, int nt, bool eflag, double *e) {
+  TersP P;
+  const double cut = P.R + P.D, cut2 = cut * cut, sc = 23.06;
+  int is = s.L.n_ion;
+  for (int i = t; i < is; i += nt) {
+    typename S::H hi = s.h(i);
+    double xi[3];
+    s.pos(hi, xi);
+    double U = 0;
+    for (int j = 0; j < is; j++) {
+      if (j == i) continue;
+      typename S::H hj = s.h(j);
+      double xj[3];
+      s.pos(hj, xj);
+      double d[3] = {xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2]};
+      double rsq = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+      if (rsq > cut2) continue;
+      double r = sqrt(rsq);
+      double fR = P.A * exp(-P.lamda1 * r), fR_d = -P.A * P.lamda1 * exp(-P.lamda1 * r);
+      double fC = t_fC(P, r), fC_d = t_fC_d(P, r);
+      double fA = -P.B * exp(-P.lamda2 * r), fA_d = P.B * P.lamda2 * exp(-P.lamda2 * r);
+      // repulsive pair
+      double fp = -(fC * fR_d + fR * fC_d) / r * 0.5;
+      U += fC * fR * 0.5;
+      // bond order
+      double zeta = 0;
+      for (int k = 0; k < is; k++) {
+        if (k == i || k == j) continue;
+        double xk[3];
+        s.pos(s.h(k), xk);
+        double d2[3] = {xi[0] - xk[0], xi[1] - xk[1], xi[2] - xk[2]};
+        double rsq2 = d2[0] * d2[0] + d2[1] * d2[1] + d2[2] * d2[2];
+        if (rsq2 > cut2) continue;
+        double r2 = sqrt(rsq2);
+        double ct = (d[0] * d2[0] + d[1] * d2[1] + d[2] * d2[2]) / (r * r2);
+        zeta += exp(P.lamda3 * P.lamda3 * P.lamda3 * (r - r2) * (r - r2) * (r - r2)) * t_g(P, ct) * t_fC(P, r2);
+      }
+      double bij = t_bij(P, zeta);
+      double fz = bij * (fA * fC_d + fA_d * fC) / r;
+      fp += -0.5 * fz;
+      U += bij * fC * fA * 0.5;
+      s.fadd(hi, sc * fp * d[0], sc * fp * d[1], sc * fp * d[2]);
+      s.fadd(hj, -sc * fp * d[0], -sc * fp * d[1], -sc * fp * d[2]);
+      if (zeta == 0) continue;
+      double pre = fA * fC * t_bij_d(P, zeta);
+      double rij_hat[3] = {d[0] / r, d[1] / r, d[2] / r};
+      double fi[3] = {0, 0, 0}, fj[3] = {0, 0, 0};
+      for (int k = 0; k < is; k++) {
+        if (k == i || k == j) continue;
+        typename S::H hk = s.h(k);
+        double xk[3];
+        s.pos(hk, xk);
+        double d2[3] = {xi[0] - xk[0], xi[1] - xk[1], xi[2] - xk[2]};
+        double rsq2 = d2[0] * d2[0] + d2[1] * d2[1] + d2[2] * d2[2];
+        if (rsq2 > cut2) continue;
+        double r2 = sqrt(rsq2);
+        double rik_hat[3] = {d2[0] / r2, d2[1] / r2, d2[2] / r2};
+        double fc = t_fC(P, r2), dfc = t_fC_d(P, r2);
+        double ct = rij_hat[0] * rik_hat[0] + rij_hat[1] * rik_hat[1] + rij_hat[2] * rik_hat[2];
+        double g = t_g(P, ct), g_d = t_g_d(P, ct);
+        double fk[3];
+        for (int q = 0; q < 3; q++) {
+          double dcdrj = (rik_hat[q] - ct * rij_hat[q]) / r;
+          double dcdrk = (rij_hat[q] - ct * rik_hat[q]) / r2;
+          double dcdri = -dcdrj - dcdrk;
+          fi[q] += 0.5 * pre * (-dfc * g * rik_hat[q] + fc * g_d * dcdri);
+          fj[q] += 0.5 * pre * (fc * g_d * dcdrj);
+          fk[q] = 0.5 * pre * (dfc * g * rik_hat[q] + fc * g_d * dcdrk);
+        }
+        s.fadd(hk, sc * fk[0], sc * fk[1], sc * fk[2]);
+      }
+      s.fadd(hi, sc * fi[0], sc * fi[1], sc * fi[2]);
+      s.fadd(hj, sc * fj[0], sc * fj[1], sc * fj[2]);
+    }
+    if (eflag) e[U_ION] += U * sc;
+  }
+}
+
+// ---- Stillinger-Weber (StillingerWeber/StillingerWeber.cpp:11-198, Si
+// constants of potentialSW.hpp:13-37); same list / unit treatment ----
+struct SWP {
+  double c1, c2, c3, c4, c5, c6, sigma, sigma_gamma, lambda_epsilon, lambda_epsilon2, costheta0, p, q, cut;
+  __device__ SWP() {
+    double epsilon = 2.1683, sg = 2.0951, a = 1.80, lambda = 21.0, gamma = 1.20;
+    double A = 7.049556277, B = 0.6022245584;
+    p = 4.0; q = 0.0; costheta0 = -0.333333333333; sigma = sg;
+    c1 = A * epsilon * p * B * pow(sg, p);
+    c2 = A * epsilon * q * pow(sg, q);
+    c3 = A * epsilon * B * pow(sg, p + 1);
+    c4 = A * epsilon * pow(sg, q + 1);
+    c5 = A * epsilon * B * pow(sg, p);
+    c6 = A * epsilon * pow(sg, q);
+    sigma_gamma = sg * gamma; lambda_epsilon = lambda * epsilon; lambda_epsilon2 = 2 * lambda * epsilon;
+    cut = a * sg;
+  }
+};
+template <class S>
+__device__ void term_sw(const DevReplica &R, const S &s, int t, int nt, bool eflag, double *e) {
+  SWP P;
+  const double cut2 = P.cut * P.cut, sc = 23.06;
+  int is = s.L.n_ion;
+  for (int i = t; i < is; i += nt) {
+    typename S::H hi = s.h(i);
+    double xi[3];
+    s.pos(hi, xi);
+    double U = 0, fi[3] = {0, 0, 0};
+    for (int j = 0; j < is; j++) {
+      if (j == i) continue;
+      typename S::H hj = s.h(j);
+      double xj[3];
+      s.pos(hj, xj);
+      double d[3] = {xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2]};
+      double rsq = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+      if (rsq > cut2) continue;
+      // two-body, StillingerWeber.cpp:104-116
+      double r = sqrt(rsq), rinvsq = 1.0 / rsq;
+      double rp = pow(r, -P.p), rq = pow(r, -P.q);
+      double rainv = 1.0 / (r - P.cut), rainvsq = rainv * rainv * r, expsrainv = exp(P.sigma * rainv);
+      U += (P.c5 * rp - P.c6 * rq) * expsrainv;
+      double fp = (P.c1 * rp - P.c2 * rq + (P.c3 * rp - P.c4 * rq) * rainvsq) * expsrainv * rinvsq * 0.5;
+      for (int q = 0; q < 3; q++) fi[q] += fp * d[q];
+      s.fadd(hj, -sc * fp * d[0], -sc * fp * d[1], -sc * fp * d[2]);
+      // three-body over k != j within the cutoff, StillingerWeber.cpp:120-168
+      double fj[3] = {0, 0, 0};
+      for (int k = 0; k < is; k++) {
+        if (k == i || k == j) continue;
+        typename S::H hk = s.h(k);
+        double xk[3];
+        s.pos(hk, xk);
+        double d2[3] = {xi[0] - xk[0], xi[1] - xk[1], xi[2] - xk[2]};
+        double rsq2 = d2[0] * d2[0] + d2[1] * d2[1] + d2[2] * d2[2];
+        if (rsq2 > cut2) continue;
+        double r1 = r, rinvsq1 = rinvsq, rainv1 = 1.0 / (r1 - P.cut);
+        double gs1 = P.sigma_gamma * rainv1, gsq1 = gs1 * rainv1 / r1, eg1 = exp(gs1);
+        double r2 = sqrt(rsq2), rinvsq2 = 1.0 / rsq2, rainv2 = 1.0 / (r2 - P.cut);
+        double gs2 = P.sigma_gamma * rainv2, gsq2 = gs2 * rainv2 / r2, eg2 = exp(gs2);
+        double rinv12 = 1.0 / (r1 * r2);
+        double cs = (d[0] * d2[0] + d[1] * d2[1] + d[2] * d2[2]) * rinv12;
+        double delcs = cs - P.costheta0, delcssq = delcs * delcs;
+        double facexp = eg1 * eg2, facrad = P.lambda_epsilon * facexp * delcssq;
+        double frad1 = facrad * gsq1, frad2 = facrad * gsq2;
+        double facang = P.lambda_epsilon2 * facexp * delcs, facang12 = rinv12 * facang;
+        double csfacang = cs * facang, csfac1 = rinvsq1 * csfacang, csfac2 = rinvsq2 * csfacang;
+        double tj[3], tk[3];
+        for (int q = 0; q < 3; q++) {
+          tj[q] = d[q] * (frad1 + csfac1) - d2[q] * facang12;
+          tk[q] = d2[q] * (frad2 + csfac2) - d[q] * facang12;
+          fi[q] += (tj[q] + tk[q]) * 0.5;
+          fj[q] -= tj[q] * 0.5;
+        }
+        U += facrad;
+        s.fadd(hk, -sc * tk[0] * 0.5, -sc * tk[1] * 0.5, -sc * tk[2] * 0.5);
+      }
+      s.fadd(hj, sc * fj[0], sc * fj[1], sc * fj[2]);
+    }
+    s.fadd(hi, sc * fi[0], sc * fi[1], sc * fi[2]);
+    if (eflag) e[U_ION] += U * 0.5 * sc;
+  }
+}
+
+// all registered potentials in the registration order of MD::setPotential
+// (setPotential.cpp:13-108): InterInter gi, vi, gg, gv, vv, Efield; then
+// IntraInter Tersoff, SW, AMBER, BMH, vapor intra, gas intra.
+template <typename Rl, class S>
+__device__ void all_terms(const DevReplica &R, const S &s, uint32_t bits, int t, int nt, bool eflag, double *e) {
+  if (bits & B_GI) term_gi<Rl>(R, s, t, nt, eflag, e);
+  if (bits & B_VI) term_vi<Rl>(R, s, t, nt, eflag, e);
+  if (bits & B_GG) term_gg<Rl>(R, s, t, nt, eflag, e);
+  if (bits & B_GV) term_gv<Rl>(R, s, t, nt, eflag, e);
+  if (bits & B_VV) term_vv<Rl>(R, s, t, nt, eflag, e);
+  if (bits & B_EFIELD) term_efield(R, s, t, nt);
+  if (bits & B_TERSOFF) term_tersoff(R, s, t, nt, eflag, e);
+  if (bits & B_SW) term_sw(R, s, t, nt, eflag, e);
+  if (bits & B_AMBER) term_amber<Rl>(R, s, t, nt, eflag, e);
+  if (bits & B_BMH) term_born(R, s, t, nt, eflag, e);
+  if (bits & B_VINTRA) term_vintra(R, s, t, nt, eflag, e);
+  if (bits & B_GINTRA) term_gintra(R, s, t, nt, eflag, e);
+}
+
+// ------------------------------------------------------------------------
+// SHAKE of one molecule (MD::shake, shake.cpp:44-92: positions only,
+// Gauss-Seidel over the rigid pairs until (r^2 - d^2)^2 <= 1e-20 for all),
+// with an iteration cap.  x: current positions (updated), r: positions before
+// the drift.  Returns iterations used, negative when the cap was hit.
+template <class XA, class RA>
+__device__ int shake_molecule(const DevReplica &R, const DevSpecies &Sp, XA x, RA r, int max_iter) {
+  const double tolerance = 1e-20, rho_zero = 0.01;
+  int it = 0;
+  bool converge = false;
+  while (!converge) {
+    if (it >= max_iter) return -it;
+    it++;
+    converge = true;
+    for (int k = 0; k < Sp.nrigid; k++) {
+      int i = Sp.rigid[3 * k], j = Sp.rigid[3 * k + 1], ty = Sp.rigid[3 * k + 2];
+      double dk = R.btype[2 * ty + 1];
+      double rk[3], rn[3];
+      for (int q = 0; q < 3; q++) { rk[q] = r(i, q) - r(j, q); rn[q] = x(i, q) - x(j, q); }
+      double rk2 = rk[0] * rk[0] + rk[1] * rk[1] + rk[2] * rk[2];
+      double rn2 = rn[0] * rn[0] + rn[1] * rn[1] + rn[2] * rn[2];
+      double dot = rk[0] * rn[0] + rk[1] * rn[1] + rk[2] * rn[2];
+      double mred_inv = Sp.invmass[i] + Sp.invmass[j];
+      double ramk = (dk * dk - rn2) / 2 * mred_inv;
+      if (dot > 1e-10) ramk /= dot;
+      else ramk = 0.5 * rho_zero + ramk / (rk2 * rho_zero) * mred_inv;
+      double ci = ramk * Sp.invmass[i], cj = ramk * Sp.invmass[j];
+      for (int q = 0; q < 3; q++) { x(i, q) += rk[q] * ci; x(j, q) -= rk[q] * cj; }
+      for (int q = 0; q < 3; q++) rn[q] = x(i, q) - x(j, q);
+      rn2 = rn[0] * rn[0] + rn[1] * rn[1] + rn[2] * rn[2];
+      double dr = rn2 - dk * dk;
+      if (dr * dr > tolerance) converge = false;
+    }
+  }
+  return it;
+}
+
+// ------------------------------------------------------------------------
+// MD::make_pair (pairlist.cpp:16-49) for one replica, by one workgroup, on
+// global memory.  FP decisions (advance, COM, classification, list radii,
+// v_max) use the reference's operation order with contraction disabled, so
+// that given the same state the lists are bit-identical.  s_int >= 64 ints.
+__device__ inline void enter_molecule(const DevReplica &R, int i, int sp, long long rb) {
+#pragma clang fp contract(off)
+  const DevSpecies &S = R.sp[sp];
+  int na = S.natoms;
+  int o = sp == 1 ? R.off_gas + (i - 1) * na : R.off_vap + (i - 1 - R.n_gas) * na;
+  double *x = R.atom_x + 3 * (size_t)o, *v = R.atom_v + 3 * (size_t)o;
+  for (int q = 0; q < 3 * na; q++) { x[q] = S.tmpl[q]; v[q] = 0; }
+  if (sp == 2)
+    for (int q = 0; q < 3 * na; q++) R.atom_f[3 * (size_t)o + q] = 0;
+  if (sp == 1 && R.gas_spin && na >= 2) {
+    // N2 internal rotation, updateGasIn.cpp:66-76
+    double z[2];
+    rng_normal2(R.key0, R.key1, (uint32_t)i, rb, 4, z);
+    double vy = R.sig_gas * z[0] * 1e-5;
+    v[0] = 0; v[1] = vy; v[2] = vy;
+    v[3] = 0; v[4] = -vy; v[5] = -vy;
+  }
+  double u[2], u3[2];
+  rng_uniform2(R.key0, R.key1, (uint32_t)i, rb, 2, u);
+  rng_uniform2(R.key0, R.key1, (uint32_t)i, rb, 3, u3);
+  double a = u[0] * 2 * kPI, b = u[1] * 2 * kPI, c = u3[0] * 2 * kPI;
+  const double *X = R.mol_x + 3 * (size_t)i, *V = R.mol_v + 3 * (size_t)i;
+  for (int ii = 0; ii < na; ii++) {
+    double *p = x + 3 * ii;
+    // the gas variant passes (x, y, y), updateGasIn.cpp:84
+    double za = sp == 1 ? p[1] : p[2];
+    rotation(p[0], p[1], p[2], a, b, c, p[0], p[1], za);
+    p[0] += X[0]; p[1] += X[1]; p[2] += X[2];
+  }
+  for (int ii = 0; ii < na; ii++) {
+    double *p = v + 3 * ii;
+    double za = sp == 1 ? p[1] : p[2];
+    rotation(p[0], p[1], p[2], a, b, c, p[0], p[1], za);
+    p[0] += V[0]; p[1] += V[1]; p[2] += V[2];
+  }
+}
+
+__device__ inline void rebuild_wg(const DevReplica &R, int *s_int, double *s_dbl) {
+#pragma clang fp contract(off)
+  const int tid = threadIdx.x, nt = blockDim.x;
+  Dyn *D = R.dyn;
+  const int lp = D->loopPair;
+  const long long rb = D->rebuild_count;
+  double *mol = R.mol_x, *molv = R.mol_v;
+  const double HL = R.d_size * 0.5;
+  const double cx = mol[0], cy = mol[1], cz = mol[2];
+  // advance (out) / COM (in) / classify, one molecule per thread
+  for (int i = 1 + tid; i < R.n_mol; i += nt) {
+    int sp = i <= R.n_gas ? 1 : 2;
+    const DevSpecies &S = R.sp[sp];
+    double *m = mol + 3 * (size_t)i, *mv = molv + 3 * (size_t)i;
+    bool was_in = R.region[i] != 0;
+    if (!was_in) {
+      // pairlist.cpp:20-29
+      m[0] += mv[0] * R.dt * lp;
+      m[1] += mv[1] * R.dt * lp;
+      m[2] += mv[2] * R.dt * lp;
+    } else {
+      // updateCenters.cpp:6-57 (divides by the declared molecule mass)
+      int o = sp == 1 ? R.off_gas + (i - 1) * S.natoms : R.off_vap + (i - 1 - R.n_gas) * S.natoms;
+      double s0 = 0, s1 = 0, s2 = 0, v0 = 0, v1 = 0, v2 = 0;
+      for (int ii = 0; ii < S.natoms; ii++) {
+        const double *xa = R.atom_x + 3 * (size_t)(o + ii), *va = R.atom_v + 3 * (size_t)(o + ii);
+        double ma = S.mass[ii];
+        s0 += xa[0] * ma; s1 += xa[1] * ma; s2 += xa[2] * ma;
+        v0 += va[0] * ma; v1 += va[1] * ma; v2 += va[2] * ma;
+      }
+      double mi = 1 / S.molmass;
+      m[0] = s0 * mi; m[1] = s1 * mi; m[2] = s2 * mi;
+      mv[0] = v0 * mi; mv[1] = v1 * mi; mv[2] = v2 * mi;
+    }
+    // updateGasIn.cpp:21-47 / updateVaporIn.cpp:23-51
+    double dx = m[0] - cx, dy = m[1] - cy, dz = m[2] - cz;
+    double r2 = (dx * dx + dy * dy + dz * dz);
+    if (r2 < kRI2) {
+      if (!was_in) enter_molecule(R, i, sp, rb);
+      R.region[i] = 1;
+    } else {
+      R.region[i] = 0;
+      if (dx < -HL) m[0] += R.d_size;
+      if (dy < -HL) m[1] += R.d_size;
+      if (dz < -HL) m[2] += R.d_size;
+      if (dx > HL) m[0] -= R.d_size;
+      if (dy > HL) m[1] -= R.d_size;
+      if (dz > HL) m[2] -= R.d_size;
+      double sg = sp == 1 ? R.sig_gas : R.sig_vap;
+      double z[2], z2[2];
+      rng_normal2(R.key0, R.key1, (uint32_t)i, rb, 0, z);
+      rng_normal2(R.key0, R.key1, (uint32_t)i, rb, 1, z2);
+      mv[0] = sg * z[0] * 1e-5;
+      mv[1] = sg * z[1] * 1e-5;
+      mv[2] = sg * z2[0] * 1e-5;
+    }
+  }
+  __syncthreads();
+  // ordered compaction of the in-domain gas and vapor ids
+  int ng = 0, nv = 0;
+  for (int c0 = 0; c0 < R.n_gas; c0 += nt) {
+    int k = c0 + tid, flag = k < R.n_gas ? R.region[1 + k] : 0, tot;
+    int pre = block_scan(flag, s_int, &tot);
+    if (flag) R.in_gas[ng + pre] = 1 + k;
+    ng += tot;
+  }
+  for (int c0 = 0; c0 < R.n_vap; c0 += nt) {
+    int k = c0 + tid, flag = k < R.n_vap ? R.region[1 + R.n_gas + k] : 0, tot;
+    int pre = block_scan(flag, s_int, &tot);
+    if (flag) R.in_vap[nv + pre] = 1 + R.n_gas + k;
+    nv += tot;
+  }
+  __syncthreads();
+  // Potential::makePair lists (GasIon.cpp:41-60, VaporGas.cpp:47-64, gas-gas),
+  // ascending gas molecule then partner, stored as in-list positions
+  const int n_ion = R.sp[0].natoms;
+  int n_gi = 0, n_gv = 0, n_gg = 0;
+  for (int c0 = 0; c0 < ng; c0 += nt) {
+    int k = c0 + tid;
+    bool act = k < ng;
+    const double *g = act ? mol + 3 * (size_t)R.in_gas[k] : mol;
+    int cgi = 0, cgv = 0, cgg = 0;
+    if (act && (R.flags & B_GI))
+      for (int j = 0; j < n_ion; j++) {
+        const double *xj = R.atom_x + 3 * (size_t)j;
+        double dx = g[0] - xj[0], dy = g[1] - xj[1], dz = g[2] - xj[2];
+        if ((dx * dx + dy * dy + dz * dz) < R.ML2) cgi++;
+      }
+    if (act && (R.flags & B_GV))
+      for (int l = 0; l < nv; l++) {
+        const double *h = mol + 3 * (size_t)R.in_vap[l];
+        double dx = g[0] - h[0], dy = g[1] - h[1], dz = g[2] - h[2];
+        if ((dx * dx + dy * dy + dz * dz) < R.ML2) cgv++;
+      }
+    if (act && (R.flags & B_GG))
+      for (int l = k + 1; l < ng; l++) {
+        const double *h = mol + 3 * (size_t)R.in_gas[l];
+        double dx = g[0] - h[0], dy = g[1] - h[1], dz = g[2] - h[2];
+        if ((dx * dx + dy * dy + dz * dz) < R.ML2) cgg++;
+      }
+    int t1, t2, t3;
+    int p1 = block_scan(cgi, s_int, &t1) + n_gi;
+    int p2 = block_scan(cgv, s_int, &t2) + n_gv;
+    int p3 = block_scan(cgg, s_int, &t3) + n_gg;
+    if (act && cgi)
+      for (int j = 0; j < n_ion; j++) {
+        const double *xj = R.atom_x + 3 * (size_t)j;
+        double dx = g[0] - xj[0], dy = g[1] - xj[1], dz = g[2] - xj[2];
+        if ((dx * dx + dy * dy + dz * dz) < R.ML2) {
+          if (p1 < R.cap_gi) { R.gi[2 * p1] = k; R.gi[2 * p1 + 1] = j; }
+          p1++;
+        }
+      }
+    if (act && cgv)
+      for (int l = 0; l < nv; l++) {
+        const double *h = mol + 3 * (size_t)R.in_vap[l];
+        double dx = g[0] - h[0], dy = g[1] - h[1], dz = g[2] - h[2];
+        if ((dx * dx + dy * dy + dz * dz) < R.ML2) {
+          if (p2 < R.cap_gv) { R.gv[2 * p2] = k; R.gv[2 * p2 + 1] = l; }
+          p2++;
+        }
+      }
+    if (act && cgg)
+      for (int l = k + 1; l < ng; l++) {
+        const double *h = mol + 3 * (size_t)R.in_gas[l];
+        double dx = g[0] - h[0], dy = g[1] - h[1], dz = g[2] - h[2];
+        if ((dx * dx + dy * dy + dz * dz) < R.ML2) {
+          if (p3 < R.cap_gg) { R.gg[2 * p3] = k; R.gg[2 * p3 + 1] = l; }
+          p3++;
+        }
+      }
+    n_gi += t1; n_gv += t2; n_gg += t3;
+  }
+  // v_max over every molecule (pairlist.cpp:38-46); max is order-free
+  double vm = 0.0;
+  for (int i = tid; i < R.n_mol; i += nt) {
+    const double *v = molv + 3 * (size_t)i;
+    double v2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+    if (vm < v2) vm = v2;
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    double y = __shfl_xor(vm, o);
+    vm = vm < y ? y : vm;
+  }
+  if ((tid & 63) == 0) s_dbl[tid >> 6] = vm;
+  __syncthreads();
+  if (tid == 0) {
+    double m2 = 0.0;
+    for (int w = 0; w < (nt + 63) / 64; w++) m2 = m2 < s_dbl[w] ? s_dbl[w] : m2;
+    double lu = R.margin / (sqrt(m2) * R.dt);
+    D->loop_update = lu >= 2147483647.0 ? 2147483647 : (int)lu;
+    D->loopPair = 0;
+    D->rebuild_count = rb + 1;
+    D->n_in_gas = ng;
+    D->n_in_vap = nv;
+    int st = 0;
+    if (n_gi > R.cap_gi) { st |= ST_CAP_GI; n_gi = R.cap_gi; }
+    if (n_gv > R.cap_gv) { st |= ST_CAP_GV; n_gv = R.cap_gv; }
+    if (n_gg > R.cap_gg) { st |= ST_CAP_GG; n_gg = R.cap_gg; }
+    D->n_gi = n_gi; D->n_gv = n_gv; D->n_gg = n_gg;
+    D->status |= st;
+  }
+  __syncthreads();
+}
+
+}  // namespace mdn

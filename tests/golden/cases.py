@@ -1,0 +1,28 @@
+"""Input files of the golden cases (shared by make_golden.py and the tests;
+md_nucleation_amd.configs writes them deterministically)."""
+import os
+
+from md_nucleation_amd import configs
+
+
+def write_case(name, d):
+    m = configs.molecule_files(d)
+    if name == "c1":
+        # box 162 A: molecules inside the domain, in the 100-110 A blend shell
+        # and outside it
+        return configs.Condition(m["na"], m["methanol"], 30, "N2", 600, pressure=4e5,
+                                 efield=(0.01, -0.02, 0.005)).write(os.path.join(d, "c1.cond"))
+    if name == "pep":
+        pep = configs.write_peptide(os.path.join(d, "pep40.txt"), seed=3, n=40)
+        return configs.Condition(pep, m["water"], 20, "N2", 200, pressure=1e6, ion_potential="AMBER").write(
+            os.path.join(d, "pep.cond"))
+    if name == "bmh":
+        nacl = configs.write_nacl_cluster(os.path.join(d, "nacl.txt"))
+        return configs.Condition(nacl, m["water"], 1, "He", 1, ion_potential="Born-Mayer-Huggins-NaCl",
+                                 interactions={}, shake_h=False).write(os.path.join(d, "bmh.cond"))
+    if name in ("tersoff", "sw"):
+        si = configs.write_si_cluster(os.path.join(d, "si.txt"))
+        pot = "Tersoff" if name == "tersoff" else "Stilinger-Weber"
+        return configs.Condition(si, m["water"], 1, "He", 1, ion_potential=pot, interactions={},
+                                 shake_h=False).write(os.path.join(d, "si_%s.cond" % name))
+    raise KeyError(name)

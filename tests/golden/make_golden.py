@@ -25,8 +25,8 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
 sys.path.insert(0, ROOT)
-from md_nucleation_amd import configs  # noqa: E402
 from oracle import oracle as O  # noqa: E402
+from tests.golden.cases import write_case  # noqa: E402
 
 OUT = os.path.dirname(os.path.abspath(__file__))
 STATE_KEYS = ("mol_x", "mol_v", "region", "atom_x", "atom_v", "atom_f", "atom_w")
@@ -86,21 +86,11 @@ def main():
     if not os.path.exists(O.REF_HARNESS):
         subprocess.check_call(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "ref"])
     with tempfile.TemporaryDirectory() as d:
-        m = configs.molecule_files(d)
-        # c1: box 162 A so that molecules sit inside, in the 100-110 A blend
-        # shell and outside the domain
-        c = configs.Condition(m["na"], m["methanol"], 30, "N2", 600, pressure=4e5, efield=(0.01, -0.02, 0.005)).write(
-            os.path.join(d, "c1.cond"))
-        case("c1", c, d, shake_mols=(601, 605))
-        pep = configs.write_peptide(os.path.join(d, "pep40.txt"), seed=3, n=40)
-        c = configs.Condition(pep, m["water"], 20, "N2", 200, pressure=1e6, ion_potential="AMBER").write(
-            os.path.join(d, "pep.cond"))
-        case("pep", c, d, shake_mols=(201, 210))
-        nacl = configs.write_nacl_cluster(os.path.join(d, "nacl.txt"))
+        case("c1", write_case("c1", d), d, shake_mols=(601, 605))
+        case("pep", write_case("pep", d), d, shake_mols=(201, 210))
         # the reference needs a gas line; its BMH table is indexed by ion-local
         # type, so the harness shifts the ion's types down by the gas type count
-        c = configs.Condition(nacl, m["water"], 1, "He", 1, ion_potential="Born-Mayer-Huggins-NaCl",
-                              interactions={}, shake_h=False).write(os.path.join(d, "bmh.cond"))
+        c = write_case("bmh", d)
         sysa = harness(c, None, "sys", cwd=d)
         init = harness(c, None, "init", cwd=d)
         st = {k: init[k] for k in STATE_KEYS}
@@ -112,15 +102,12 @@ def main():
         res["born_ion_f"] = r["ion_f"]
         res["born_U_ion"] = r["U_ion"]
         np.savez_compressed(os.path.join(OUT, "bmh.npz"), **res)
-        si = configs.write_si_cluster(os.path.join(d, "si.txt"))
         res = {}
-        for pot, op in (("Tersoff", "tersoff"), ("Stilinger-Weber", "sw")):
-            c = configs.Condition(si, m["water"], 1, "He", 1, ion_potential=pot, interactions={}, shake_h=False).write(
-                os.path.join(d, "si_%s.cond" % op))
+        for op in ("tersoff", "sw"):
+            c = write_case(op, d)
             sysa = harness(c, None, "sys", cwd=d)
             init = harness(c, None, "init", cwd=d)
             st = {k: init[k] for k in STATE_KEYS}
-            sp = os.path.join(d, "state.txt")
             O.write_arrays(sp, dict(st, loopPair=[0], loop_update=[10 ** 8], margin_length=[10.0]))
             r = harness(c, sp, op, cwd=d)
             res.update(pack(op + "_sys_", sysa))
