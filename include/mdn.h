@@ -47,7 +47,7 @@ typedef enum mdn_status {
   MDN_E_IO = -2,        /* input file missing or malformed */
   MDN_E_HIP = -3,       /* HIP runtime error */
   MDN_E_CAPACITY = -4,  /* a list or pool exceeded its capacity */
-  MDN_E_SHAKE = -5,     /* SHAKE did not converge within its iteration cap */
+  MDN_E_SHAKE = -5,     /* reserved (SHAKE failures are counted, see mdn_get_counters) */
   MDN_E_STATE = -6      /* call out of order (e.g. before mdn_commit) */
 } mdn_status;
 
@@ -121,6 +121,10 @@ const char *mdn_last_error(void);
 int mdn_sysdef_load(const char *cond_path, mdn_sysdef **out);
 int mdn_sysdef_view(const mdn_sysdef *d, mdn_system *out);
 int mdn_sysdef_sizes(const mdn_sysdef *d, int32_t *n_mol, int32_t *n_atoms);
+/* SHAKE mode: 0 off (the reference as shipped never calls shake(),
+ * verlet.cpp:24); 1 (default under "SHAKE H") SHAKE of shake.cpp:44-92 plus
+ * the velocity correction v = (x_shaken - x_old)/dt of constrained atoms;
+ * 2 shake.cpp's positions-only update.  max_iter caps the iteration. */
 int mdn_sysdef_set_shake(mdn_sysdef *d, int32_t shake, int32_t max_iter);
 int mdn_sysdef_set_flags(mdn_sysdef *d, uint32_t flags);
 int mdn_sysdef_init_state(const mdn_sysdef *d, uint64_t seed, mdn_state *st);
@@ -158,9 +162,13 @@ int mdn_run_batched(mdn_ctx *c, int32_t nsteps, int32_t eflag_every,
  * the autotuner (md_nucleation_amd/autotune.py); no reference counterpart. */
 int mdn_set_batch_shape(mdn_ctx *c, int32_t threads, int32_t lds_cap_atoms, int32_t grid);
 
-/* counters: in-domain atom-steps since commit, rebuilds, max SHAKE iters */
+/* counters: in-domain atom-steps since commit, rebuilds, max SHAKE
+ * iterations, SHAKE calls that failed (cap reached or diverged: that molecule
+ * then stays unconstrained for the step).  Returns MDN_E_CAPACITY when a
+ * list or the batched kernel's LDS capacity overflowed. */
 int mdn_get_counters(mdn_ctx *c, int32_t replica, int64_t *atom_steps,
-                     int64_t *rebuilds, int32_t *shake_iters_max);
+                     int64_t *rebuilds, int32_t *shake_iters_max,
+                     int64_t *shake_failures);
 /* device synchronisation and timing of the last mdn_run_batched /
  * mdn_verlet call (HIP events on the library's stream), milliseconds */
 int mdn_sync(mdn_ctx *c);

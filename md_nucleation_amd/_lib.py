@@ -76,7 +76,8 @@ _SIGS = {
     "mdn_verlet": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int64, C.c_int32]),
     "mdn_run_batched": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int64, C.c_int32]),
     "mdn_set_batch_shape": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32]),
-    "mdn_get_counters": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_int64), C.POINTER(C.c_int64), _ip]),
+    "mdn_get_counters": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_int64), C.POINTER(C.c_int64), _ip,
+                                   C.POINTER(C.c_int64)]),
     "mdn_sync": (C.c_int, [C.c_void_p]),
     "mdn_last_kernel_ms": (C.c_int, [C.c_void_p, _dp]),
 }
@@ -293,9 +294,11 @@ class Engine:
         check(lib().mdn_set_batch_shape(self.h, int(threads), int(lds_cap_atoms), int(grid)), "mdn_set_batch_shape")
 
     def counters(self, r):
-        a, b, s = C.c_int64(), C.c_int64(), C.c_int32()
-        check(lib().mdn_get_counters(self.h, int(r), C.byref(a), C.byref(b), C.byref(s)), "mdn_get_counters")
-        return a.value, b.value, s.value
+        """(in-domain atom-steps, rebuilds, max SHAKE iterations, SHAKE failures)"""
+        a, b, s, f = C.c_int64(), C.c_int64(), C.c_int32(), C.c_int64()
+        check(lib().mdn_get_counters(self.h, int(r), C.byref(a), C.byref(b), C.byref(s), C.byref(f)),
+              "mdn_get_counters")
+        return a.value, b.value, s.value, f.value
 
     def sync(self):
         check(lib().mdn_sync(self.h), "mdn_sync")

@@ -539,8 +539,9 @@ int mdn_sysdef_sizes(const mdn_sysdef *h, int32_t *n_mol, int32_t *n_atoms) {
 
 int mdn_sysdef_set_shake(mdn_sysdef *h, int32_t shake, int32_t max_iter) {
   if (!h || max_iter <= 0) return set_error(MDN_E_ARG, "mdn_sysdef_set_shake: bad argument");
+  if (shake < 0 || shake > 2) return set_error(MDN_E_ARG, "mdn_sysdef_set_shake: mode must be 0, 1 or 2");
   SysDef &d = *reinterpret_cast<SysDef *>(h);
-  d.shake = shake ? 1 : 0;
+  d.shake = shake;
   d.shake_max_iter = max_iter;
   return MDN_OK;
 }

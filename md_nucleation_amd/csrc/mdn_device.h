@@ -49,7 +49,8 @@ struct Dyn {
   int n_gi, n_gv, n_gg;
   int n_in_gas, n_in_vap;
   int status, shake_iters_max;
-  int pad[3];
+  int pad[2];
+  unsigned long long shake_fail;
 };
 
 struct DevReplica {

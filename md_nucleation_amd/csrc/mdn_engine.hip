@@ -103,12 +103,20 @@ __global__ void k_shake(const DevReplica *__restrict__ Rp) {
     else { sp = 1; o = R.off_gas + (R.in_gas[m - 1 - L.nv] - 1) * L.na_g; }
     const DevSpecies &S = R.sp[sp];
     if (!S.nrigid) continue;
-    double *x = R.atom_x + 3 * (size_t)o;
+    double *x = R.atom_x + 3 * (size_t)o, *v = R.atom_v + 3 * (size_t)o;
     const double *xo = R.atom_xo + 3 * (size_t)o;
-    int it = shake_molecule(R, S, [&](int i, int q) -> double & { return x[3 * i + q]; },
-                            [&](int i, int q) -> double { return xo[3 * i + q]; }, R.shake_max_iter);
-    if (it < 0) { atomicOr(&R.dyn->status, ST_SHAKE); it = -it; }
+    auto X = [&](int i, int q) -> double & { return x[3 * i + q]; };
+    auto XO = [&](int i, int q) -> double { return xo[3 * i + q]; };
+    auto V = [&](int i, int q) -> double & { return v[3 * i + q]; };
+    int it = shake_molecule(R, S, X, XO, R.shake_max_iter);
+    if (it < 0) {
+      atomicAdd(&R.dyn->shake_fail, 1ull);
+      atomicMax(&R.dyn->shake_iters_max, -it);
+      shake_undo(S, X, XO, V, R.dt);
+      continue;
+    }
     atomicMax(&R.dyn->shake_iters_max, it);
+    if (R.shake == 1) shake_velocity(S, X, XO, V, R.dt);
   }
 }
 
@@ -376,12 +384,20 @@ __global__ void __launch_bounds__(256) k_batched(const DevReplica *__restrict__ 
           else { sp = 1; base = L.gbase + (m - 1 - L.nv) * L.na_g; }
           const DevSpecies &S = R.sp[sp];
           if (!S.nrigid) continue;
-          double *x = B.x + 3 * base;
+          double *x = B.x + 3 * base, *v = B.v + 3 * base;
           const double *xo = B.xo + 3 * base;
-          int it = shake_molecule(R, S, [&](int i, int q) -> double & { return x[3 * i + q]; },
-                                  [&](int i, int q) -> double { return xo[3 * i + q]; }, R.shake_max_iter);
-          if (it < 0) { atomicOr(&D->status, ST_SHAKE); it = -it; }
+          auto X = [&](int i, int q) -> double & { return x[3 * i + q]; };
+          auto XO = [&](int i, int q) -> double { return xo[3 * i + q]; };
+          auto V = [&](int i, int q) -> double & { return v[3 * i + q]; };
+          int it = shake_molecule(R, S, X, XO, R.shake_max_iter);
+          if (it < 0) {
+            atomicAdd(&D->shake_fail, 1ull);
+            atomicMax(&D->shake_iters_max, -it);
+            shake_undo(S, X, XO, V, R.dt);
+            continue;
+          }
           atomicMax(&D->shake_iters_max, it);
+          if (R.shake == 1) shake_velocity(S, X, XO, V, R.dt);
         }
         __syncthreads();
       }
@@ -501,6 +517,7 @@ struct mdn_ctx {
   mdn::DevReplica *d_reps = nullptr;
   bool committed = false;
   int batch_cap = 0, batch_threads = 256, batch_grid = 0;
+  mdn::Dyn *d_dyn = nullptr;
   int any_shake = 0;
 };
 
@@ -590,6 +607,8 @@ static int upload_all(mdn_ctx *c) {
   };
   std::vector<Offs> offs(c->reps.size());
   c->any_shake = 0;
+  // every replica's Dyn in one block: the host reads all of them with one copy
+  size_t dyn_block = lay.zeros(sizeof(Dyn) * c->reps.size());
   for (size_t r = 0; r < c->reps.size(); r++) {
     HostReplica &h = c->reps[r];
     const SysDef &d = h.def;
@@ -650,7 +669,8 @@ static int upload_all(mdn_ctx *c) {
     in_lists(h, ig2, iv2);
     dyn.n_in_gas = (int)ig2.size();
     dyn.n_in_vap = (int)iv2.size();
-    o.dyn = lay.put(&dyn, sizeof(dyn), false);
+    o.dyn = dyn_block + sizeof(Dyn) * r;
+    memcpy(lay.host.data() + o.dyn, &dyn, sizeof(dyn));
     h.off_dyn = o.dyn;
   }
   size_t reps_off = lay.zeros(sizeof(DevReplica) * c->reps.size());
@@ -706,13 +726,13 @@ static int upload_all(mdn_ctx *c) {
   }
   memcpy(lay.host.data() + reps_off, drs.data(), sizeof(DevReplica) * drs.size());
   c->d_reps = at<DevReplica>(B, reps_off);
+  c->d_dyn = at<Dyn>(B, dyn_block);
   HIPCHK(hipMemcpy(c->dbuf, lay.host.data(), lay.host.size(), hipMemcpyHostToDevice));
   return MDN_OK;
 }
 
 static int status_to_code(int st) {
   if (st & (ST_CAP_GI | ST_CAP_GV | ST_CAP_GG | ST_LDS_CAP)) return MDN_E_CAPACITY;
-  if (st & ST_SHAKE) return MDN_E_SHAKE;
   return MDN_OK;
 }
 
@@ -975,11 +995,24 @@ int mdn_run_batched(mdn_ctx *c, int32_t nsteps, int32_t eflag_every, int64_t ste
   HIPCHK(hipSetDevice(c->device));
   int cap = c->batch_cap;
   if (cap <= 0) {
-    // LDS budget for two workgroups per CU (160 KB of LDS per CU)
-    int need = 16;
-    for (auto &h : c->reps) need = std::max(need, h.def.n_atoms);
-    cap = 32;
-    while (cap + 32 <= need + 31 && batch_lds_bytes(cap + 32, c->any_shake) <= 80 * 1024) cap += 32;
+    // size the LDS capacity from the replicas' current in-domain counts with
+    // headroom for molecules entering later; up to 160 KB (one workgroup per
+    // CU) when a replica needs it, otherwise <= 80 KB (two per CU)
+    std::vector<Dyn> dyn(c->reps.size());
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipMemcpy(dyn.data(), c->d_dyn, sizeof(Dyn) * dyn.size(), hipMemcpyDeviceToHost));
+    int need = 16, most = 16;
+    for (size_t r = 0; r < c->reps.size(); r++) {
+      const SysDef &d = c->reps[r].def;
+      int n_in = d.sp[0].natoms + dyn[r].n_in_vap * d.sp[2].natoms + dyn[r].n_in_gas * d.sp[1].natoms;
+      need = std::max(need, n_in);
+      most = std::max(most, d.n_atoms);
+    }
+    cap = std::min(most, (need * 3 / 2 + 64 + 31) / 32 * 32);
+    while (cap > need && batch_lds_bytes(cap, c->any_shake) > 160 * 1024) cap -= 32;
+    if (batch_lds_bytes(cap, c->any_shake) > 160 * 1024)
+      return set_error(MDN_E_CAPACITY, "mdn_run_batched: a replica's in-domain atoms exceed one workgroup's LDS; "
+                                       "use mdn_verlet for it");
   }
   size_t lds = batch_lds_bytes(cap, c->any_shake);
   if (lds > 160 * 1024) return set_error(MDN_E_ARG, "mdn_run_batched: LDS capacity too large");
@@ -1008,7 +1041,8 @@ int mdn_set_batch_shape(mdn_ctx *c, int32_t threads, int32_t lds_cap_atoms, int3
   return MDN_OK;
 }
 
-int mdn_get_counters(mdn_ctx *c, int32_t r, int64_t *atom_steps, int64_t *rebuilds, int32_t *shake_iters_max) {
+int mdn_get_counters(mdn_ctx *c, int32_t r, int64_t *atom_steps, int64_t *rebuilds, int32_t *shake_iters_max,
+                     int64_t *shake_failures) {
   int rc = check_replica(c, r);
   if (rc) return rc;
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -1017,6 +1051,7 @@ int mdn_get_counters(mdn_ctx *c, int32_t r, int64_t *atom_steps, int64_t *rebuil
   if (atom_steps) *atom_steps = dyn.atom_steps;
   if (rebuilds) *rebuilds = dyn.rebuild_count;
   if (shake_iters_max) *shake_iters_max = dyn.shake_iters_max;
+  if (shake_failures) *shake_failures = (int64_t)dyn.shake_fail;
   int code = status_to_code(dyn.status);
   return code ? set_error(code, "replica status flags set: " + std::to_string(dyn.status)) : MDN_OK;
 }

@@ -543,10 +543,31 @@ __device__ int shake_molecule(const DevReplica &R, const DevSpecies &Sp, XA x, R
       for (int q = 0; q < 3; q++) rn[q] = x(i, q) - x(j, q);
       rn2 = rn[0] * rn[0] + rn[1] * rn[1] + rn[2] * rn[2];
       double dr = rn2 - dk * dk;
+      if (!(dr * dr <= 1e30)) return -it;  // diverged (or non-finite)
       if (dr * dr > tolerance) converge = false;
     }
   }
   return it;
+}
+
+// a failed SHAKE leaves the molecule unconstrained for this step (the
+// reference, which never calls shake(), always does): x = x_old + v dt
+template <class XA, class RA, class VA>
+__device__ void shake_undo(const DevSpecies &Sp, XA x, RA r, VA v, double dt) {
+  for (int a = 0; a < Sp.natoms; a++)
+    for (int q = 0; q < 3; q++) x(a, q) = r(a, q) + v(a, q) * dt;
+}
+
+// SHAKE mode 1: every atom that takes part in a constraint gets the
+// constrained half-step velocity (x_shaken - x_old) / dt
+template <class XA, class RA, class VA>
+__device__ void shake_velocity(const DevSpecies &Sp, XA x, RA r, VA v, double dt) {
+  for (int a = 0; a < Sp.natoms; a++) {
+    bool cons = false;
+    for (int k = 0; k < Sp.nrigid && !cons; k++) cons = Sp.rigid[3 * k] == a || Sp.rigid[3 * k + 1] == a;
+    if (!cons) continue;
+    for (int q = 0; q < 3; q++) v(a, q) = (x(a, q) - r(a, q)) / dt;
+  }
 }
 
 // ------------------------------------------------------------------------

@@ -259,8 +259,8 @@ int orc_shake_molecule(orc_ctx *c, double *x, const double *r, int sp) {
       _rkx = x[3*i] - x[3*j]; _rky = x[3*i+1] - x[3*j+1]; _rkz = x[3*i+2] - x[3*j+2];
       _rk2 = _rkx * _rkx + _rky * _rky + _rkz * _rkz;
       double dr = _rk2 - dk * dk;
+      if (!(dr * dr <= 1e30)) return -it; /* diverged (or non-finite) */
       if (dr * dr > tolerance) converge = 0;
-      if (dr * dr > 1e4) converge = 0;
     }
   }
   return it;
@@ -288,8 +288,30 @@ void orc_update_position_shake(orc_ctx *c, orc_state *st) {
     }
     if (s->nrigid[sp]) {
       int it = orc_shake_molecule(c, st->atom_x + 3 * o, oldp, sp);
-      if (it < 0) { st->status = -3; it = -it; }
+      if (it < 0) {
+        /* a failed SHAKE leaves the molecule unconstrained for this step, as
+         * the reference (which never calls shake()) always does */
+        st->shake_failures++;
+        it = -it;
+        if (it > st->shake_iters_max) st->shake_iters_max = it;
+        for (int ii = 0; ii < 3 * na; ii++) st->atom_x[3 * o + ii] = oldp[ii] + st->atom_v[3 * o + ii] * dt;
+        free(heap);
+        continue;
+      }
       if (it > st->shake_iters_max) st->shake_iters_max = it;
+      /* mode 1: the half-step velocity of every constrained atom becomes the
+       * constrained displacement over dt (SHAKE in velocity Verlet); mode 2
+       * keeps shake.cpp's positions-only update */
+      if (s->shake == 1)
+        for (int ii = 0; ii < na; ii++) {
+          int cons = 0;
+          for (int k = 0; k < s->nrigid[sp]; k++)
+            if (s->rigid[sp][3 * k] == ii || s->rigid[sp][3 * k + 1] == ii) { cons = 1; break; }
+          if (!cons) continue;
+          double *x = st->atom_x + 3 * (o + ii), *v = st->atom_v + 3 * (o + ii);
+          const double *xo = oldp + 3 * ii;
+          v[0] = (x[0] - xo[0]) / dt; v[1] = (x[1] - xo[1]) / dt; v[2] = (x[2] - xo[2]) / dt;
+        }
     }
     free(heap);
   }

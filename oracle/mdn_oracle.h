@@ -79,7 +79,8 @@ typedef struct orc_system {
   int gas_spin;             /* gastype==2: N2 spin at entry */
   uint32_t flags;
   double efield[3];
-  int shake;                /* 0: as shipped (verlet.cpp:24), 1: SHAKE */
+  int shake;                /* 0: as shipped (verlet.cpp:24); 1: SHAKE with the
+                               velocity correction; 2: shake.cpp positions only */
   int shake_max_iter;
   uint64_t seed;
   uint32_t replica;
@@ -97,6 +98,7 @@ typedef struct orc_state {
   int *gi, *gv, *gg;        /* pair lists [cap*2] */
   double U[U_N];
   int shake_iters_max;      /* diagnostics: max SHAKE iterations seen */
+  int64_t shake_failures;   /* SHAKE calls that hit the cap or diverged */
   int status;               /* 0 ok, <0 error */
 } orc_state;
 

@@ -55,7 +55,7 @@ class OrcState(C.Structure):
         ("cap_gi", C.c_int), ("cap_gv", C.c_int), ("cap_gg", C.c_int),
         ("n_gi", C.c_int), ("n_gv", C.c_int), ("n_gg", C.c_int),
         ("gi", _ip), ("gv", _ip), ("gg", _ip),
-        ("U", C.c_double * 8), ("shake_iters_max", C.c_int), ("status", C.c_int),
+        ("U", C.c_double * 8), ("shake_iters_max", C.c_int), ("shake_failures", C.c_int64), ("status", C.c_int),
     ]
 
 

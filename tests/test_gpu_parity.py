@@ -165,7 +165,7 @@ def test_trajectory_fp64_vs_oracle(tmp_path, path, ion):
         assert x_err < 1e-8
         assert f_err < 1e-7
         np.testing.assert_allclose(st.U, S.U, rtol=1e-6, atol=1e-6)
-        a, rb, it = eng.counters(r)
+        a, rb, it, nfail = eng.counters(r)
         assert rb == S.c.rebuild_count
 
 
