@@ -168,6 +168,7 @@ def main():
         eng.sync()
         el = time.perf_counter() - t0
         kern_ms = eng.last_kernel_ms()
+        lds_cap, lds_bytes, n_heavy = eng.batch_info()
         a1 = total_atom_steps(eng, n)
         value = (a1 - a0) / el
         flops = 0
@@ -207,7 +208,8 @@ def main():
         "force_err": {"rel_rms": force_err, "max_abs": fmax, "vs": "FP64 path, identical states, %d replicas" % k_err},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
-                     "kernel": "k_batched<float>", "kernel_ms": kern_ms,
+                     "kernel": "k_batched<float>", "kernel_ms": kern_ms, "lds_bytes_per_wg": lds_bytes, "replicas_with_own_cu": n_heavy,
+                     "lds_cap_atoms": lds_cap,
                      "flops_per_step": flops},
         "cpu_baseline": cpu,
     }

@@ -156,11 +156,19 @@ int mdn_verlet(mdn_ctx *c, int32_t replica, int32_t nsteps,
 int mdn_run_batched(mdn_ctx *c, int32_t nsteps, int32_t eflag_every,
                     int64_t step0, int32_t precision);
 
-/* launch shape of mdn_run_batched: threads per workgroup (64/128/256), LDS
- * capacity in in-domain atoms per replica (0 = largest that keeps two
- * workgroups per CU), grid size (0 = one workgroup per replica).  Chosen by
- * the autotuner (md_nucleation_amd/autotune.py); no reference counterpart. */
-int mdn_set_batch_shape(mdn_ctx *c, int32_t threads, int32_t lds_cap_atoms, int32_t grid);
+/* launch shape of mdn_run_batched: threads per workgroup (64..512; 0 = split:
+ * the heaviest replicas in 512-thread workgroups, one per CU, the rest in
+ * 128-thread workgroups, the default), LDS
+ * capacity in in-domain atoms per replica (0 = sized from the current
+ * in-domain counts), grid size (0 = one workgroup per replica), steps per
+ * launch (0 = all; between launches the LDS capacity is re-sized).  No
+ * reference counterpart: the batched path's tuning knobs. */
+int mdn_set_batch_shape(mdn_ctx *c, int32_t threads, int32_t lds_cap_atoms, int32_t grid, int32_t chunk);
+
+/* LDS capacity (in-domain atoms per replica), LDS bytes per workgroup and the
+ * number of replicas given a CU of their own, of the last mdn_run_batched
+ * launch.  No reference counterpart. */
+int mdn_get_batch_info(mdn_ctx *c, int32_t *lds_cap_atoms, int32_t *lds_bytes, int32_t *heavy);
 
 /* counters: in-domain atom-steps since commit, rebuilds, max SHAKE
  * iterations, SHAKE calls that failed (cap reached or diverged: that molecule
@@ -169,8 +177,8 @@ int mdn_set_batch_shape(mdn_ctx *c, int32_t threads, int32_t lds_cap_atoms, int3
 int mdn_get_counters(mdn_ctx *c, int32_t replica, int64_t *atom_steps,
                      int64_t *rebuilds, int32_t *shake_iters_max,
                      int64_t *shake_failures);
-/* device synchronisation and timing of the last mdn_run_batched /
- * mdn_verlet call (HIP events on the library's stream), milliseconds */
+/* device synchronisation and timing of the last mdn_run_batched (sum of its
+ * launches) / mdn_verlet call (HIP events on the library's stream), ms */
 int mdn_sync(mdn_ctx *c);
 int mdn_last_kernel_ms(mdn_ctx *c, double *ms);
 

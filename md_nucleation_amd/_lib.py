@@ -75,9 +75,10 @@ _SIGS = {
     "mdn_update_position": (C.c_int, [C.c_void_p, C.c_int32]),
     "mdn_verlet": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int64, C.c_int32]),
     "mdn_run_batched": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int64, C.c_int32]),
-    "mdn_set_batch_shape": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32]),
+    "mdn_set_batch_shape": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32]),
     "mdn_get_counters": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_int64), C.POINTER(C.c_int64), _ip,
                                    C.POINTER(C.c_int64)]),
+    "mdn_get_batch_info": (C.c_int, [C.c_void_p, _ip, _ip, _ip]),
     "mdn_sync": (C.c_int, [C.c_void_p]),
     "mdn_last_kernel_ms": (C.c_int, [C.c_void_p, _dp]),
 }
@@ -290,8 +291,9 @@ class Engine:
         check(lib().mdn_run_batched(self.h, int(nsteps), int(eflag_every), int(step0), int(precision)),
               "mdn_run_batched")
 
-    def set_batch_shape(self, threads=256, lds_cap_atoms=0, grid=0):
-        check(lib().mdn_set_batch_shape(self.h, int(threads), int(lds_cap_atoms), int(grid)), "mdn_set_batch_shape")
+    def set_batch_shape(self, threads=0, lds_cap_atoms=0, grid=0, chunk=1000):
+        check(lib().mdn_set_batch_shape(self.h, int(threads), int(lds_cap_atoms), int(grid), int(chunk)),
+              "mdn_set_batch_shape")
 
     def counters(self, r):
         """(in-domain atom-steps, rebuilds, max SHAKE iterations, SHAKE failures)"""
@@ -299,6 +301,13 @@ class Engine:
         check(lib().mdn_get_counters(self.h, int(r), C.byref(a), C.byref(b), C.byref(s), C.byref(f)),
               "mdn_get_counters")
         return a.value, b.value, s.value, f.value
+
+    def batch_info(self):
+        """(LDS capacity in atoms, LDS bytes per workgroup, replicas with a CU of
+        their own) of the last batched launch"""
+        a, b, h = C.c_int32(), C.c_int32(), C.c_int32()
+        check(lib().mdn_get_batch_info(self.h, C.byref(a), C.byref(b), C.byref(h)), "mdn_get_batch_info")
+        return a.value, b.value, h.value
 
     def sync(self):
         check(lib().mdn_sync(self.h), "mdn_sync")

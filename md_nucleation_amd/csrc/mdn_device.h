@@ -38,6 +38,7 @@ struct DevSpecies {
   int natoms, nbond, nangle, ndih, nrigid;
   double molmass;
   const double *mass, *invmass, *charge, *tmpl;
+  const double *se, *sg;  // per atom sqrt(eps) and sigma of its type
   const int *type, *bond, *angle, *dih, *rigid;
 };
 
@@ -66,6 +67,8 @@ struct DevReplica {
   const int *dmulti;
   int n_long;
   const int *long_pairs;
+  int amber_words;              // words per row of amber_allow
+  const uint32_t *amber_allow;  // [n_ion][amber_words]: bit j of row i = pair (i,j) not excluded
   double *mol_x, *mol_v;
   uint8_t *region;
   double *atom_x, *atom_v, *atom_f, *atom_w, *atom_xo;
@@ -152,20 +155,38 @@ __device__ inline double blend_weight(double r2) {
 // Pair arithmetic.  R = double (plain) or float (mixed: the displacement is
 // formed in FP64 and rounded, the rest is FP32; the caller accumulates the
 // force in FP64).
+// 1/r^2 and 1/r: IEEE in FP64 (the reference's 1/rsq and sqrt(r2inv)); in
+// FP32 the hardware reciprocal square root (v_rsq_f32, about 1 ulp)
+template <typename R>
+__device__ inline void inv_r(R rsq, R &r2inv, R &rinv) {
+  r2inv = R(1) / rsq;
+  rinv = sqrt(r2inv);
+}
+template <>
+__device__ inline void inv_r<float>(float rsq, float &r2inv, float &rinv) {
+  rinv = __builtin_amdgcn_rsqf(rsq);
+  r2inv = rinv * rinv;
+}
+template <typename R>
+__device__ inline R inv_r2(R rsq) { return R(1) / rsq; }
+template <>
+__device__ inline float inv_r2<float>(float rsq) { return __builtin_amdgcn_rcpf(rsq); }
+
 // LJ + Coulomb, VaporIon.cpp:19-37 / VaporVapor.cpp:21-39 / computeLong.
 template <typename R>
 __device__ inline R ljcoul_fpair(R rsq, R c0, R c1, R qq, R w, bool eflag, double &e) {
-  R r2inv = R(1) / rsq;
+  R r2inv, rinv;
+  inv_r<R>(rsq, r2inv, rinv);
   R r6inv = r2inv * r2inv * r2inv;
   R flj = r6inv * (c0 * r6inv - c1);
-  R fc = qq * sqrt(r2inv);
+  R fc = qq * rinv;
   if (eflag) e += (double)(r6inv * (c0 / R(12) * r6inv - c1 / R(6))) + (double)fc;
   return (flj + fc) * r2inv * w;
 }
 // LJ only, GasIon.cpp:19-33 / VaporGas.cpp:19-35
 template <typename R>
 __device__ inline R lj_fpair(R rsq, R c0, R c1, R w, bool eflag, double &e) {
-  R r2inv = R(1) / rsq;
+  R r2inv = inv_r2<R>(rsq);
   R r6inv = r2inv * r2inv * r2inv;
   R flj = r6inv * (c0 * r6inv - c1);
   if (eflag) e += (double)(r6inv * (c0 / R(12) * r6inv - c1 / R(6)));

@@ -274,60 +274,236 @@ __global__ void __launch_bounds__(kVVTile) k_vv_tiled(const DevReplica *__restri
 }
 
 // ======================= batched persistent kernel ========================
+// LDS of one workgroup: per in-domain atom x, v, f (FP64), w, kick factor
+// (FP64), charge and LJ mixing parameters (pair precision), type; old
+// positions only for the ion + vapor atoms (the only constrained ones).
+// per-atom pair parameters packed for one LDS read: charge, sqrt(eps),
+// sigma, local vapor molecule id (-1: ion or gas) as a value
+template <typename Rl> struct Pack4;
+template <> struct Pack4<float> { typedef float4 T; };
+template <> struct Pack4<double> { typedef double4 T; };
+
+template <typename Rl>
 struct BatchLds {
-  double *x, *v, *f, *xo, *w, *q, *kc;
-  int *type;
-  double *red;   // [U_N + 6][16]
-  int *sint;     // [64]
-  double *sdbl;  // [16]
-  int *flag;
+  double *x, *v, *f, *xo;
+  double *kct;      // [kcap] kick factor 0.5 dt 4.184e-4 / m per species atom (ion | vapor | gas)
+  typename Pack4<Rl>::T *pk;  // packed (q, sqrt eps, sigma, molecule id)
+  Rl *wr;           // blend weight (pair precision)
+  int16_t *kidx;    // per atom: index into kct
+  uint32_t *allow;  // [acap] AMBER non-excluded ion-pair bitmask (rows of amber_words)
+  double *red;      // [U_N + 6][16]
+  double *sdbl;     // [16]
+  int *sint;        // [64]
 };
 
-__device__ inline BatchLds carve(char *base, int cap, int has_shake) {
-  BatchLds B;
+template <typename Rl>
+__device__ inline BatchLds<Rl> carve(char *base, int cap, int cap_h, int kcap, int acap) {
+  BatchLds<Rl> B;
   double *d = reinterpret_cast<double *>(base);
   B.x = d; d += 3 * cap;
   B.v = d; d += 3 * cap;
   B.f = d; d += 3 * cap;
-  B.xo = d; if (has_shake) d += 3 * cap;
-  B.w = d; d += cap;
-  B.q = d; d += cap;
-  B.kc = d; d += cap;
+  B.xo = d; d += 3 * cap_h;
+  B.kct = d; d += kcap;
   B.red = d; d += (U_N + 6) * 16;
   B.sdbl = d; d += 16;
-  int *ip = reinterpret_cast<int *>(d);
-  B.type = ip; ip += cap;
+  B.pk = reinterpret_cast<typename Pack4<Rl>::T *>(d);
+  d += (4 * sizeof(Rl) * (size_t)cap) / sizeof(double);
+  Rl *r = reinterpret_cast<Rl *>(d);
+  B.wr = r; r += cap;
+  int *ip = reinterpret_cast<int *>(r);
   B.sint = ip; ip += 64;
-  B.flag = ip; ip += 4;
+  B.allow = reinterpret_cast<uint32_t *>(ip); ip += acap;
+  int16_t *hp = reinterpret_cast<int16_t *>(ip);
+  B.kidx = hp; hp += cap;
   return B;
 }
 
-static size_t batch_lds_bytes(int cap, int has_shake) {
-  return sizeof(double) * ((size_t)(9 + (has_shake ? 3 : 0) + 3) * cap + (U_N + 6) * 16 + 16) +
-         sizeof(int) * ((size_t)cap + 64 + 4);
+static size_t batch_lds_bytes(int cap, int cap_h, int kcap, int acap, size_t rl) {
+  return sizeof(double) * ((size_t)9 * cap + 3 * (size_t)cap_h + kcap + (U_N + 6) * 16 + 16) +
+         rl * 5 * (size_t)cap + sizeof(int) * (64 + (size_t)acap) + sizeof(int16_t) * (size_t)cap;
 }
 
-__device__ inline void batch_load(const DevReplica &R, const LView &L, const BatchLds &B) {
+// LDS store with Lorentz-Berthelot mixing from per-atom parameters
+// (setCrossPotential.cpp:13-20: eps_ij = sqrt(eps_i eps_j), sigma_ij =
+// (sigma_i + sigma_j)/2, c0 = 48 eps sigma^12, c1 = 24 eps sigma^6)
+template <typename Rl>
+struct SStoreT {
+  LView L;
+  double *px, *pf;
+  const typename Pack4<Rl>::T *pk;
+  const Rl *wr;
+  const int *ptype;  // the ion's atom types (global memory; BMH is ion-only)
+  typedef int H;
+  __device__ H h(int a) const { return a; }
+  __device__ void pos(H a, double p[3]) const { p[0] = px[3 * a]; p[1] = px[3 * a + 1]; p[2] = px[3 * a + 2]; }
+  __device__ void fadd(H a, double fx, double fy, double fz) const {
+    atomicAdd(&pf[3 * a], fx); atomicAdd(&pf[3 * a + 1], fy); atomicAdd(&pf[3 * a + 2], fz);
+  }
+  __device__ double w(H a) const { return (double)wr[a]; }
+  __device__ double q(H a) const { return (double)pk[a].x; }
+  __device__ int type(H a) const { return ptype[a]; }
+  template <typename R2>
+  __device__ void coeff(const DevReplica &, H a, H b, R2 &c0, R2 &c1) const {
+    R2 e = (R2)pk[a].y * (R2)pk[b].y;
+    R2 sg = (R2)0.5 * ((R2)pk[a].z + (R2)pk[b].z);
+    R2 s2 = sg * sg, s6 = s2 * s2 * s2;
+    c1 = (R2)24 * e * s6;
+    c0 = (R2)2 * c1 * s6;
+  }
+};
+
+// AMBER non-excluded ion pairs (computeLong, compute_AMBER.cpp:21-49), vi
+// (VaporIon.cpp:5-43) and vv (VaporVapor.cpp:5-45) as one pass over the ion
+// + in-domain vapor atoms, each unordered pair evaluated once (Newton's third
+// law).  The atoms are cut into 64-atom blocks; a wave takes a block pair
+// (bi <= bj) with lane l owning atom i = 64 bi + l and, at step k, partner
+// j = 64 bj + (l + k) mod 64, so the 64 lanes always touch 64 different j:
+// F_i accumulates in FP64 registers and F_j goes to LDS with conflict-free
+// ds_add_f64.  A diagonal block pair runs k = 1..32 (k = 32 on half the
+// lanes), which visits each of its pairs once.  A per-lane mask selects the
+// pairs the reference evaluates: ion-ion by the AMBER exclusion bitmask,
+// ion-vapor by vi, vapor-vapor of distinct molecules by vv.
+template <typename Rl, bool EF>
+__device__ inline void dense_n3(const DevReplica &R, const BatchLds<Rl> &B, int nh, int ni, bool am, bool vi, bool vv,
+                                int t, int nt, double *e) {
+  const Rl qq = (Rl)R.qqrd2e;
+  const int lane = t & 63, wave = t >> 6, nw = nt >> 6;
+  const int NB = (nh + 63) >> 6, units = NB * (NB + 1) / 2, W = R.amber_words;
+  const double *__restrict__ px = B.x;
+  const typename Pack4<Rl>::T *__restrict__ pk = B.pk;
+  const Rl *__restrict__ wr = B.wr;
+  for (int u = wave; u < units; u += nw) {
+    int bi = 0, rem = u;
+    while (rem >= NB - bi) { rem -= NB - bi; bi++; }
+    const int bj = bi + rem;
+    const bool diag = bi == bj;
+    const int i = 64 * bi + lane;
+    const bool i_ok = i < nh;
+    const int ic = i_ok ? i : 0;
+    const double xi0 = px[3 * ic], xi1 = px[3 * ic + 1], xi2 = px[3 * ic + 2];
+    const typename Pack4<Rl>::T pi = pk[ic];
+    const Rl qi = qq * pi.x, sei = pi.y, sgi = pi.z, mi = pi.w, wi = wr[ic];
+    const bool ion_i = ic < ni;
+    const uint32_t *arow = B.allow + (size_t)(ion_i ? ic : 0) * W;
+    double fx = 0, fy = 0, fz = 0, e_ion = 0, e_vi = 0, e_vv = 0;
+    const int kbeg = diag ? 1 : 0, kend = diag ? 33 : 64;
+#pragma unroll 4
+    for (int k = kbeg; k < kend; k++) {
+      const int j = 64 * bj + ((lane + k) & 63);
+      const bool valid = i_ok && j < nh && !(diag && k == 32 && lane >= 32);
+      const int jc = valid ? j : 0;
+      const typename Pack4<Rl>::T pj = pk[jc];
+      const double dx = xi0 - px[3 * jc], dy = xi1 - px[3 * jc + 1], dz = xi2 - px[3 * jc + 2];
+      const Rl wj = wr[jc];
+      const bool ion_j = jc < ni;
+      const bool both = ion_i && ion_j, cross = ion_i != ion_j;
+      const uint32_t aw = both && am ? arow[jc >> 5] : 0u;
+      const bool on = valid && (both ? ((aw >> (jc & 31)) & 1u) != 0 : (cross ? vi : (vv && pj.w != mi)));
+      const Rl rx = (Rl)dx, ry = (Rl)dy, rz = (Rl)dz;
+      const Rl rsq = on ? rx * rx + ry * ry + rz * rz : (Rl)1;
+      const Rl ee = sei * pj.y;
+      const Rl sg = (Rl)0.5 * (sgi + pj.z);
+      const Rl s2 = sg * sg, s6 = s2 * s2 * s2;
+      const Rl c1 = (Rl)24 * ee * s6, c0 = (Rl)2 * c1 * s6;
+      const Rl w = both ? (Rl)1 : wi * wj;
+      Rl r2inv, rinv;
+      inv_r<Rl>(rsq, r2inv, rinv);
+      const Rl r6inv = r2inv * r2inv * r2inv;
+      const Rl fc = qi * pj.x * rinv;
+      const Rl fp = on ? (r6inv * (c0 * r6inv - c1) + fc) * r2inv * w : (Rl)0;
+      const double gx = (double)(fp * rx), gy = (double)(fp * ry), gz = (double)(fp * rz);
+      fx += gx;
+      fy += gy;
+      fz += gz;
+      if (on) {
+        atomicAdd(&B.f[3 * jc], -gx);
+        atomicAdd(&B.f[3 * jc + 1], -gy);
+        atomicAdd(&B.f[3 * jc + 2], -gz);
+      }
+      if (EF) {
+        const double ep =
+            on ? (double)(r6inv * (c0 * (Rl)(1.0 / 12.0) * r6inv - c1 * (Rl)(1.0 / 6.0))) + (double)fc : 0.0;
+        e_ion += both ? ep : 0.0;
+        e_vi += cross ? ep : 0.0;
+        e_vv += (!both && !cross) ? ep : 0.0;
+      }
+    }
+    if (i_ok) {
+      atomicAdd(&B.f[3 * i], fx);
+      atomicAdd(&B.f[3 * i + 1], fy);
+      atomicAdd(&B.f[3 * i + 2], fz);
+    }
+    if (EF) { e[U_ION] += e_ion; e[U_VI] += e_vi; e[U_VV] += e_vv; }
+  }
+}
+
+template <typename Rl>
+__device__ void dense_heavy(const DevReplica &R, const BatchLds<Rl> &B, const LView &L, int t, int nt, bool eflag,
+                            double *e) {
+  const int nh = L.gbase, ni = L.n_ion;
+  const bool am = (R.flags & B_AMBER) != 0, vi = (R.flags & B_VI) != 0, vv = (R.flags & B_VV) != 0;
+  if (!(am || vi || vv) || nh == 0) return;
+  if (eflag)
+    dense_n3<Rl, true>(R, B, nh, ni, am, vi, vv, t, nt, e);
+  else
+    dense_n3<Rl, false>(R, B, nh, ni, am, vi, vv, t, nt, e);
+}
+
+// the batched path's force phase: registration order of setPotential.cpp
+// with vi, vv and the AMBER non-bonded pairs in the dense pass
+template <typename Rl>
+__device__ void batch_terms(const DevReplica &R, const SStoreT<Rl> &s, const BatchLds<Rl> &B, int t, int nt,
+                            bool eflag, double *e) {
+  const uint32_t bits = R.flags;
+  if (bits & B_GI) term_gi<Rl>(R, s, t, nt, eflag, e);
+  dense_heavy<Rl>(R, B, s.L, t, nt, eflag, e);
+  if (bits & B_GG) term_gg<Rl>(R, s, t, nt, eflag, e);
+  if (bits & B_GV) term_gv<Rl>(R, s, t, nt, eflag, e);
+  if (bits & B_EFIELD) term_efield(R, s, t, nt);
+  if (bits & B_TERSOFF) term_tersoff(R, s, t, nt, eflag, e);
+  if (bits & B_SW) term_sw(R, s, t, nt, eflag, e);
+  if (bits & B_AMBER) term_amber_bonded(R, s, t, nt, eflag, e);
+  if (bits & B_BMH) term_born(R, s, t, nt, eflag, e);
+  if (bits & B_VINTRA) term_vintra(R, s, t, nt, eflag, e);
+  if (bits & B_GINTRA) term_gintra(R, s, t, nt, eflag, e);
+}
+
+template <typename Rl>
+__device__ inline void batch_load(const DevReplica &R, const LView &L, const BatchLds<Rl> &B) {
   const double C = 0.5 * R.dt * kKickUnit;
+  if (R.flags & B_AMBER)
+    for (int k = threadIdx.x; k < L.n_ion * R.amber_words; k += blockDim.x) B.allow[k] = R.amber_allow[k];
+  const int nkt = L.n_ion + L.na_v + L.na_g;
+  for (int k = threadIdx.x; k < nkt; k += blockDim.x) {
+    int sp = k < L.n_ion ? 0 : (k < L.n_ion + L.na_v ? 2 : 1);
+    int idx = sp == 0 ? k : (sp == 2 ? k - L.n_ion : k - L.n_ion - L.na_v);
+    B.kct[k] = C / R.sp[sp].mass[idx];
+  }
   for (int a = threadIdx.x; a < L.n_in; a += blockDim.x) {
-    int sp, idx;
-    if (a < L.n_ion) { sp = 0; idx = a; }
-    else if (a < L.gbase) { sp = 2; idx = (a - L.vbase) % L.na_v; }
-    else { sp = 1; idx = (a - L.gbase) % L.na_g; }
+    int sp, idx, kk;
+    if (a < L.n_ion) { sp = 0; idx = a; kk = a; }
+    else if (a < L.gbase) { sp = 2; idx = (a - L.vbase) % L.na_v; kk = L.n_ion + idx; }
+    else { sp = 1; idx = (a - L.gbase) % L.na_g; kk = L.n_ion + L.na_v + idx; }
     size_t g = (size_t)gatom(R, L, a);
     for (int q = 0; q < 3; q++) {
       B.x[3 * a + q] = R.atom_x[3 * g + q];
       B.v[3 * a + q] = R.atom_v[3 * g + q];
       B.f[3 * a + q] = R.atom_f[3 * g + q];
     }
-    B.w[a] = R.atom_w[g];
-    B.q[a] = R.sp[sp].charge[idx];
-    B.type[a] = R.sp[sp].type[idx];
-    B.kc[a] = C / R.sp[sp].mass[idx];
+    const DevSpecies &S = R.sp[sp];
+    B.kidx[a] = (int16_t)kk;
+    const int m = sp == 2 ? (a - L.vbase) / L.na_v : -1;
+    B.pk[a].x = (Rl)S.charge[idx];
+    B.pk[a].y = (Rl)S.se[idx];
+    B.pk[a].z = (Rl)S.sg[idx];
+    B.pk[a].w = (Rl)m;
+    B.wr[a] = (Rl)R.atom_w[g];
   }
 }
 
-__device__ inline void batch_store(const DevReplica &R, const LView &L, const BatchLds &B) {
+template <typename Rl>
+__device__ inline void batch_store(const DevReplica &R, const LView &L, const BatchLds<Rl> &B) {
   for (int a = threadIdx.x; a < L.n_in; a += blockDim.x) {
     size_t g = (size_t)gatom(R, L, a);
     for (int q = 0; q < 3; q++) {
@@ -335,22 +511,25 @@ __device__ inline void batch_store(const DevReplica &R, const LView &L, const Ba
       R.atom_v[3 * g + q] = B.v[3 * a + q];
       R.atom_f[3 * g + q] = B.f[3 * a + q];
     }
-    R.atom_w[g] = B.w[a];
+    R.atom_w[g] = (double)B.wr[a];
   }
 }
 
 template <typename Rl>
-__global__ void __launch_bounds__(256) k_batched(const DevReplica *__restrict__ reps, int n_rep, int nsteps,
-                                                 int eflag_every, long long step0, int cap, int has_shake) {
+__global__ void __launch_bounds__(512) k_batched(const DevReplica *__restrict__ reps,
+                                                 const int *__restrict__ order, int n_rep, int nsteps,
+                                                 int eflag_every, long long step0, int cap, int cap_h, int kcap,
+                                                 int acap) {
   extern __shared__ __align__(16) char smem[];
-  BatchLds B = carve(smem, cap, has_shake);
+  BatchLds<Rl> B = carve<Rl>(smem, cap, cap_h, kcap, acap);
   const int tid = threadIdx.x, nt = blockDim.x;
-  for (int r = blockIdx.x; r < n_rep; r += gridDim.x) {
-    const DevReplica &R = reps[r];
+  for (int rr = blockIdx.x; rr < n_rep; rr += gridDim.x) {
+    const DevReplica &R = reps[order[rr]];
     Dyn *D = R.dyn;
     if (D->status & ST_LDS_CAP) continue;
     LView L = view_from_dyn(R);
-    if (L.n_in > cap) {
+    if (L.n_in > cap || L.gbase > cap_h || L.n_ion + L.na_v + L.na_g > kcap ||
+        ((R.flags & B_AMBER) && L.n_ion * R.amber_words > acap)) {
       if (tid == 0) atomicOr(&D->status, ST_LDS_CAP);
       continue;
     }
@@ -358,30 +537,28 @@ __global__ void __launch_bounds__(256) k_batched(const DevReplica *__restrict__ 
     int lp = D->loopPair, lu = D->loop_update;
     long long acc_steps = 0;
     const DevSpecies &I = R.sp[0];
-    const bool shake = R.shake && has_shake;
+    const bool shake = R.shake != 0;
     __syncthreads();
     bool dead = false;
     for (int s = 0; s < nsteps && !dead; s++) {
       const bool eflag = eflag_every > 0 && ((step0 + s + 1) % eflag_every) == 0;
       // first half-kick + drift + force clear (verlet.cpp:21-25)
       for (int a = tid; a < L.n_in; a += nt) {
-        double kc = B.kc[a];
+        double kc = B.kct[B.kidx[a]];
+        bool keep = shake && a < L.gbase;
         for (int q = 0; q < 3; q++) {
           double v = B.v[3 * a + q] + B.f[3 * a + q] * kc;
           B.v[3 * a + q] = v;
-          if (shake) B.xo[3 * a + q] = B.x[3 * a + q];
+          if (keep) B.xo[3 * a + q] = B.x[3 * a + q];
           B.x[3 * a + q] += v * R.dt;
           B.f[3 * a + q] = 0.0;
         }
       }
       __syncthreads();
       if (shake) {
-        int nm = 1 + L.nv + L.ng;
-        for (int m = tid; m < nm; m += nt) {
-          int sp, base;
-          if (m == 0) { sp = 0; base = 0; }
-          else if (m <= L.nv) { sp = 2; base = L.vbase + (m - 1) * L.na_v; }
-          else { sp = 1; base = L.gbase + (m - 1 - L.nv) * L.na_g; }
+        // constrained molecules: the ion and the vapor (gas has no rigid pairs)
+        for (int m = tid; m <= L.nv; m += nt) {
+          int sp = m == 0 ? 0 : 2, base = m == 0 ? 0 : L.vbase + (m - 1) * L.na_v;
           const DevSpecies &S = R.sp[sp];
           if (!S.nrigid) continue;
           double *x = B.x + 3 * base, *v = B.v + 3 * base;
@@ -424,7 +601,7 @@ __global__ void __launch_bounds__(256) k_batched(const DevReplica *__restrict__ 
       // blend weights (verlet.cpp:109-134)
       for (int a = L.n_ion + tid; a < L.n_in; a += nt) {
         double dx = B.x[3 * a] - cx, dy = B.x[3 * a + 1] - cy, dz = B.x[3 * a + 2] - cz;
-        B.w[a] = blend_weight(dx * dx + dy * dy + dz * dz);
+        B.wr[a] = (Rl)blend_weight(dx * dx + dy * dy + dz * dz);
       }
       // check_pairlist (pairlist.cpp:58-63)
       lp++;
@@ -438,7 +615,7 @@ __global__ void __launch_bounds__(256) k_batched(const DevReplica *__restrict__ 
         L = view_from_dyn(R);
         lp = D->loopPair;
         lu = D->loop_update;
-        if (L.n_in > cap) {
+        if (L.n_in > cap || L.gbase > cap_h) {
           if (tid == 0) atomicOr(&D->status, ST_LDS_CAP);
           dead = true;
           break;
@@ -447,14 +624,15 @@ __global__ void __launch_bounds__(256) k_batched(const DevReplica *__restrict__ 
       }
       __syncthreads();
       // forces of every registered potential (verlet.cpp:31-32)
-      SStore ss;
-      ss.L = L; ss.px = B.x; ss.pf = B.f; ss.pw = B.w; ss.pq = B.q; ss.ptype = B.type;
+      SStoreT<Rl> ss;
+      ss.L = L; ss.px = B.x; ss.pf = B.f; ss.pk = B.pk; ss.wr = B.wr;
+      ss.ptype = R.sp[0].type;
       double e[U_N] = {0, 0, 0, 0, 0, 0, 0, 0};
-      all_terms<Rl>(R, ss, R.flags, tid, nt, eflag, e);
+      batch_terms<Rl>(R, ss, B, tid, nt, eflag, e);
       __syncthreads();
       // second half-kick (verlet.cpp:34)
       for (int a = tid; a < L.n_in; a += nt) {
-        double kc = B.kc[a];
+        double kc = B.kct[B.kidx[a]];
         for (int q = 0; q < 3; q++) B.v[3 * a + q] += B.f[3 * a + q] * kc;
       }
       acc_steps += L.n_in;
@@ -518,6 +696,12 @@ struct mdn_ctx {
   bool committed = false;
   int batch_cap = 0, batch_threads = 256, batch_grid = 0;
   mdn::Dyn *d_dyn = nullptr;
+  int last_cap = 0, last_lds = 0, batch_chunk = 1000, batch_split = 1, n_cu = 256;
+  hipStream_t stream2 = nullptr;
+  hipEvent_t evj = nullptr, evk = nullptr;
+  int *d_order = nullptr;
+  double last_ms = 0;
+  int last_heavy = 0;
   int any_shake = 0;
 };
 
@@ -601,8 +785,8 @@ static int lists_to_local(const HostReplica &h, std::vector<int> &gi, std::vecto
 static int upload_all(mdn_ctx *c) {
   Layout lay;
   struct Offs {
-    size_t sp[3][10];
-    size_t pc, bt, ct, dc, dm, lp;
+    size_t sp[3][12];
+    size_t pc, bt, ct, dc, dm, lp, am;
     size_t mol_x, mol_v, region, ax, av, af, aw, axo, ing, inv, gi, gv, gg, dyn;
   };
   std::vector<Offs> offs(c->reps.size());
@@ -626,6 +810,13 @@ static int upload_all(mdn_ctx *c) {
       o.sp[s][6] = lay.vec(S.angle, true);
       o.sp[s][7] = lay.vec(S.dih, true);
       o.sp[s][8] = lay.vec(S.rigid, true);
+      std::vector<double> se(S.type.size()), sg(S.type.size());
+      for (size_t k = 0; k < S.type.size(); k++) {
+        se[k] = sqrt(d.atypes[S.type[k]].eps);
+        sg[k] = d.atypes[S.type[k]].sigma;
+      }
+      o.sp[s][9] = lay.vec(se, true);
+      o.sp[s][10] = lay.vec(sg, true);
       if (d.shake && S.rigid.size()) c->any_shake = 1;
     }
     o.pc = lay.vec(d.pair_coeff, true);
@@ -635,6 +826,16 @@ static int upload_all(mdn_ctx *c) {
     o.dm = lay.vec(d.dmulti, true);
     h.long_pairs = (d.flags & MDN_ION_AMBER) ? amber_long_pairs(d.sp[0]) : std::vector<int>();
     o.lp = lay.vec(h.long_pairs, true);
+    {
+      int n = d.sp[0].natoms, W = (n + 31) / 32;
+      std::vector<uint32_t> allow((size_t)std::max(1, n * W), 0u);
+      for (size_t k = 0; k + 1 < h.long_pairs.size(); k += 2) {
+        int i = h.long_pairs[k], j = h.long_pairs[k + 1];
+        allow[(size_t)i * W + (j >> 5)] |= 1u << (j & 31);
+        allow[(size_t)j * W + (i >> 5)] |= 1u << (i & 31);
+      }
+      o.am = lay.vec(allow, true);
+    }
     o.mol_x = lay.vec(h.mol_x, false);
     o.mol_v = lay.vec(h.mol_v, false);
     o.region = lay.vec(h.region, false);
@@ -674,6 +875,7 @@ static int upload_all(mdn_ctx *c) {
     h.off_dyn = o.dyn;
   }
   size_t reps_off = lay.zeros(sizeof(DevReplica) * c->reps.size());
+  size_t order_off = lay.zeros(sizeof(int) * c->reps.size());
   if (c->dbuf) { hipFree(c->dbuf); c->dbuf = nullptr; }
   HIPCHK(hipSetDevice(c->device));
   HIPCHK(hipMalloc(&c->dbuf, lay.host.size()));
@@ -697,6 +899,7 @@ static int upload_all(mdn_ctx *c) {
       D.charge = at<double>(B, o.sp[s][2]); D.tmpl = at<double>(B, o.sp[s][3]);
       D.type = at<int>(B, o.sp[s][4]); D.bond = at<int>(B, o.sp[s][5]); D.angle = at<int>(B, o.sp[s][6]);
       D.dih = at<int>(B, o.sp[s][7]); D.rigid = at<int>(B, o.sp[s][8]);
+      D.se = at<double>(B, o.sp[s][9]); D.sg = at<double>(B, o.sp[s][10]);
     }
     R.n_gas = d.n_gas; R.n_vap = d.n_vap; R.n_mol = d.n_mol; R.n_atoms = d.n_atoms;
     R.n_types = (int)d.atypes.size();
@@ -715,6 +918,7 @@ static int upload_all(mdn_ctx *c) {
     R.pair_coeff = at<double>(B, o.pc); R.btype = at<double>(B, o.bt); R.ctype = at<double>(B, o.ct);
     R.dcoeff = at<double>(B, o.dc); R.dmulti = at<int>(B, o.dm);
     R.n_long = (int)h.long_pairs.size() / 2; R.long_pairs = at<int>(B, o.lp);
+    R.amber_words = (d.sp[0].natoms + 31) / 32; R.amber_allow = at<uint32_t>(B, o.am);
     R.mol_x = at<double>(B, o.mol_x); R.mol_v = at<double>(B, o.mol_v); R.region = at<uint8_t>(B, o.region);
     R.atom_x = at<double>(B, o.ax); R.atom_v = at<double>(B, o.av); R.atom_f = at<double>(B, o.af);
     R.atom_w = at<double>(B, o.aw); R.atom_xo = at<double>(B, o.axo);
@@ -727,6 +931,7 @@ static int upload_all(mdn_ctx *c) {
   memcpy(lay.host.data() + reps_off, drs.data(), sizeof(DevReplica) * drs.size());
   c->d_reps = at<DevReplica>(B, reps_off);
   c->d_dyn = at<Dyn>(B, dyn_block);
+  c->d_order = at<int>(B, order_off);
   HIPCHK(hipMemcpy(c->dbuf, lay.host.data(), lay.host.size(), hipMemcpyHostToDevice));
   return MDN_OK;
 }
@@ -795,6 +1000,12 @@ int mdn_create(int device, mdn_ctx **out) {
   HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   HIPCHK(hipEventCreate(&c->ev0));
   HIPCHK(hipEventCreate(&c->ev1));
+  HIPCHK(hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking));
+  HIPCHK(hipEventCreateWithFlags(&c->evj, hipEventDisableTiming));
+  HIPCHK(hipEventCreateWithFlags(&c->evk, hipEventDisableTiming));
+  hipDeviceProp_t prop;
+  HIPCHK(hipGetDeviceProperties(&prop, device));
+  c->n_cu = prop.multiProcessorCount;
   *out = c;
   return MDN_OK;
 }
@@ -806,6 +1017,9 @@ void mdn_destroy(mdn_ctx *c) {
   if (c->dbuf) hipFree(c->dbuf);
   if (c->ev0) hipEventDestroy(c->ev0);
   if (c->ev1) hipEventDestroy(c->ev1);
+  if (c->evj) hipEventDestroy(c->evj);
+  if (c->evk) hipEventDestroy(c->evk);
+  if (c->stream2) { hipStreamSynchronize(c->stream2); hipStreamDestroy(c->stream2); }
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
 }
@@ -986,58 +1200,153 @@ int mdn_verlet(mdn_ctx *c, int32_t r, int32_t nsteps, int32_t eflag_every, int64
     if (rc) return rc;
   }
   HIPCHK(hipEventRecord(c->ev1, c->stream));
+  c->last_ms = -1;  // read from the events on demand
   return MDN_OK;
 }
+
+}  // extern "C"
+
+namespace mdn {
+// LDS capacities for a set of replicas from their current in-domain counts,
+// with headroom for molecules that enter during a launch
+struct BatchCaps { int cap, cap_h, kcap, acap; size_t lds; };
+static BatchCaps batch_caps(const mdn_ctx *c, const std::vector<Dyn> &dyn, const std::vector<int> &set, size_t rl,
+                            size_t budget) {
+  int need = 16, need_h = 16, most = 16, most_h = 16, kcap = 8, acap = 1;
+  for (int r : set) {
+    const SysDef &d = c->reps[r].def;
+    int nh = d.sp[0].natoms + dyn[r].n_in_vap * d.sp[2].natoms;
+    need_h = std::max(need_h, nh);
+    need = std::max(need, nh + dyn[r].n_in_gas * d.sp[1].natoms);
+    most = std::max(most, d.n_atoms);
+    most_h = std::max(most_h, d.sp[0].natoms + d.n_vap * d.sp[2].natoms);
+    kcap = std::max(kcap, d.sp[0].natoms + d.sp[1].natoms + d.sp[2].natoms);
+    if (d.flags & MDN_ION_AMBER) acap = std::max(acap, d.sp[0].natoms * ((d.sp[0].natoms + 31) / 32));
+  }
+  auto r32 = [](int v) { return (v + 31) / 32 * 32; };
+  int lo_h = std::min(most_h, r32(need_h + 32)), hi_h = std::min(most_h, r32(need_h * 3 / 2 + 32));
+  int lo = std::min(most, r32(need + need / 8 + 64)), hi = std::min(most, r32(need * 3 / 2 + 64));
+  BatchCaps b;
+  b.kcap = kcap; b.acap = acap;
+  if (c->batch_cap > 0) {
+    b.cap = c->batch_cap;
+    b.cap_h = std::min(b.cap, hi_h);
+  } else {
+    b.cap = hi; b.cap_h = hi_h;
+    while (batch_lds_bytes(b.cap, b.cap_h, kcap, acap, rl) > budget && (b.cap > lo || b.cap_h > lo_h)) {
+      if (b.cap > lo) b.cap -= 32;
+      if (b.cap_h > lo_h) b.cap_h -= 32;
+    }
+    b.cap_h = std::min(b.cap_h, b.cap);
+  }
+  b.lds = batch_lds_bytes(b.cap, b.cap_h, kcap, acap, rl);
+  return b;
+}
+
+template <typename Rl>
+static int launch_batched(mdn_ctx *c, hipStream_t st, const int *order, int n, int threads, int ns, int eflag_every,
+                          long long step0, const BatchCaps &b) {
+  if (n <= 0) return MDN_OK;
+  if (b.lds > 160 * 1024)
+    return set_error(MDN_E_CAPACITY, "mdn_run_batched: a replica's in-domain atoms exceed one workgroup's LDS; "
+                                     "use mdn_verlet for it");
+  HIPCHK(hipFuncSetAttribute((const void *)k_batched<Rl>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)b.lds));
+  int grid = c->batch_grid > 0 ? std::min(c->batch_grid, n) : n;
+  hipLaunchKernelGGL(k_batched<Rl>, dim3(grid), dim3(threads), b.lds, st, c->d_reps, order, n, ns, eflag_every,
+                     step0, b.cap, b.cap_h, b.kcap, b.acap);
+  HIPCHK(hipGetLastError());
+  return MDN_OK;
+}
+}  // namespace mdn
+
+extern "C" {
 
 int mdn_run_batched(mdn_ctx *c, int32_t nsteps, int32_t eflag_every, int64_t step0, int32_t precision) {
   if (!c) return set_error(MDN_E_ARG, "null context");
   if (!c->committed) return set_error(MDN_E_STATE, "call mdn_commit first");
   HIPCHK(hipSetDevice(c->device));
-  int cap = c->batch_cap;
-  if (cap <= 0) {
-    // size the LDS capacity from the replicas' current in-domain counts with
-    // headroom for molecules entering later; up to 160 KB (one workgroup per
-    // CU) when a replica needs it, otherwise <= 80 KB (two per CU)
-    std::vector<Dyn> dyn(c->reps.size());
+  for (auto &h : c->reps)
+    if (h.def.shake && h.def.sp[1].rigid.size())
+      return set_error(MDN_E_ARG, "mdn_run_batched: constrained gas molecules are not supported; use mdn_verlet");
+  const size_t rl = precision == MDN_MIXED ? sizeof(float) : sizeof(double);
+  const int n = (int)c->reps.size();
+  const int chunk = c->batch_chunk > 0 ? c->batch_chunk : nsteps;
+  std::vector<Dyn> dyn(c->reps.size());
+  std::vector<int> order(n);
+  c->last_ms = 0;
+  for (int done = 0; done < nsteps; done += chunk) {
+    const int ns = std::min(chunk, nsteps - done);
     HIPCHK(hipStreamSynchronize(c->stream));
     HIPCHK(hipMemcpy(dyn.data(), c->d_dyn, sizeof(Dyn) * dyn.size(), hipMemcpyDeviceToHost));
-    int need = 16, most = 16;
-    for (size_t r = 0; r < c->reps.size(); r++) {
+    // Work per step grows with the square of the ion + vapor atom count.  The
+    // heaviest replicas get a whole CU each (512-thread workgroups: at 256
+    // VGPRs one fills a CU) and the rest share CUs four at a time
+    // (128-thread workgroups), so that every replica runs in the first round.
+    std::vector<double> cost(n);
+    for (int r = 0; r < n; r++) {
       const SysDef &d = c->reps[r].def;
-      int n_in = d.sp[0].natoms + dyn[r].n_in_vap * d.sp[2].natoms + dyn[r].n_in_gas * d.sp[1].natoms;
-      need = std::max(need, n_in);
-      most = std::max(most, d.n_atoms);
+      double nh = d.sp[0].natoms + dyn[r].n_in_vap * d.sp[2].natoms;
+      double nin = nh + dyn[r].n_in_gas * d.sp[1].natoms;
+      cost[r] = nh * nh + 40.0 * nin;
+      order[r] = r;
     }
-    cap = std::min(most, (need * 3 / 2 + 64 + 31) / 32 * 32);
-    while (cap > need && batch_lds_bytes(cap, c->any_shake) > 160 * 1024) cap -= 32;
-    if (batch_lds_bytes(cap, c->any_shake) > 160 * 1024)
-      return set_error(MDN_E_CAPACITY, "mdn_run_batched: a replica's in-domain atoms exceed one workgroup's LDS; "
-                                       "use mdn_verlet for it");
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b2) { return cost[a] > cost[b2]; });
+    int H = 0;
+    if (c->batch_split && c->batch_threads == 256 && n > c->n_cu) {
+      // heavy count: H + ceil((n - H) / 4) <= CUs, and only replicas well
+      // above the median cost
+      double med = cost[order[n / 2]];
+      while (H < n && cost[order[H]] > 2.0 * med && H + 1 + (n - H - 1 + 3) / 4 <= c->n_cu) H++;
+    }
+    std::vector<int> heavy(order.begin(), order.begin() + H), light(order.begin() + H, order.end());
+    HIPCHK(hipMemcpyAsync(c->d_order, order.data(), sizeof(int) * n, hipMemcpyHostToDevice, c->stream));
+    const long long s0 = (long long)(step0 + done);
+    HIPCHK(hipEventRecord(c->ev0, c->stream));
+    int rc;
+    if (H > 0) {
+      BatchCaps bh = batch_caps(c, dyn, heavy, rl, 160 * 1024);
+      BatchCaps bl = batch_caps(c, dyn, light, rl, 40 * 1024);
+      HIPCHK(hipEventRecord(c->evj, c->stream));
+      HIPCHK(hipStreamWaitEvent(c->stream2, c->evj, 0));
+      rc = precision == MDN_MIXED
+               ? launch_batched<float>(c, c->stream, c->d_order, H, 512, ns, eflag_every, s0, bh)
+               : launch_batched<double>(c, c->stream, c->d_order, H, 512, ns, eflag_every, s0, bh);
+      if (rc) return rc;
+      rc = precision == MDN_MIXED
+               ? launch_batched<float>(c, c->stream2, c->d_order + H, n - H, 128, ns, eflag_every, s0, bl)
+               : launch_batched<double>(c, c->stream2, c->d_order + H, n - H, 128, ns, eflag_every, s0, bl);
+      if (rc) return rc;
+      HIPCHK(hipEventRecord(c->evk, c->stream2));
+      HIPCHK(hipStreamWaitEvent(c->stream, c->evk, 0));
+      c->last_cap = std::max(bh.cap, bl.cap);
+      c->last_lds = (int)std::max(bh.lds, bl.lds);
+    } else {
+      BatchCaps b = batch_caps(c, dyn, order, rl, 80 * 1024);
+      rc = precision == MDN_MIXED
+               ? launch_batched<float>(c, c->stream, c->d_order, n, c->batch_threads, ns, eflag_every, s0, b)
+               : launch_batched<double>(c, c->stream, c->d_order, n, c->batch_threads, ns, eflag_every, s0, b);
+      if (rc) return rc;
+      c->last_cap = b.cap;
+      c->last_lds = (int)b.lds;
+    }
+    c->last_heavy = H;
+    HIPCHK(hipEventRecord(c->ev1, c->stream));
+    float ms = 0;
+    HIPCHK(hipEventSynchronize(c->ev1));
+    HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->last_ms += ms;
   }
-  size_t lds = batch_lds_bytes(cap, c->any_shake);
-  if (lds > 160 * 1024) return set_error(MDN_E_ARG, "mdn_run_batched: LDS capacity too large");
-  int n = (int)c->reps.size();
-  int grid = c->batch_grid > 0 ? std::min(c->batch_grid, n) : n;
-  HIPCHK(hipEventRecord(c->ev0, c->stream));
-  if (precision == MDN_MIXED) {
-    HIPCHK(hipFuncSetAttribute((const void *)k_batched<float>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL(k_batched<float>, dim3(grid), dim3(c->batch_threads), lds, c->stream, c->d_reps, n, nsteps,
-                       eflag_every, (long long)step0, cap, c->any_shake);
-  } else {
-    HIPCHK(hipFuncSetAttribute((const void *)k_batched<double>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL(k_batched<double>, dim3(grid), dim3(c->batch_threads), lds, c->stream, c->d_reps, n, nsteps,
-                       eflag_every, (long long)step0, cap, c->any_shake);
-  }
-  HIPCHK(hipGetLastError());
-  HIPCHK(hipEventRecord(c->ev1, c->stream));
   return MDN_OK;
 }
 
-int mdn_set_batch_shape(mdn_ctx *c, int32_t threads, int32_t lds_cap_atoms, int32_t grid) {
-  if (!c || (threads != 64 && threads != 128 && threads != 256)) return set_error(MDN_E_ARG, "mdn_set_batch_shape: threads must be 64, 128 or 256");
-  c->batch_threads = threads;
+int mdn_set_batch_shape(mdn_ctx *c, int32_t threads, int32_t lds_cap_atoms, int32_t grid, int32_t chunk) {
+  if (!c || (threads != 0 && threads != 64 && threads != 128 && threads != 256 && threads != 512))
+    return set_error(MDN_E_ARG, "mdn_set_batch_shape: threads must be 0 (split), 64, 128, 256 or 512");
+  c->batch_split = threads == 0 ? 1 : 0;
+  c->batch_threads = threads == 0 ? 256 : threads;
   c->batch_cap = lds_cap_atoms;
   c->batch_grid = grid;
+  c->batch_chunk = chunk;
   return MDN_OK;
 }
 
@@ -1056,6 +1365,14 @@ int mdn_get_counters(mdn_ctx *c, int32_t r, int64_t *atom_steps, int64_t *rebuil
   return code ? set_error(code, "replica status flags set: " + std::to_string(dyn.status)) : MDN_OK;
 }
 
+int mdn_get_batch_info(mdn_ctx *c, int32_t *lds_cap_atoms, int32_t *lds_bytes, int32_t *heavy) {
+  if (!c) return set_error(MDN_E_ARG, "null context");
+  if (heavy) *heavy = c->last_heavy;
+  if (lds_cap_atoms) *lds_cap_atoms = c->last_cap;
+  if (lds_bytes) *lds_bytes = c->last_lds;
+  return MDN_OK;
+}
+
 int mdn_sync(mdn_ctx *c) {
   if (!c) return set_error(MDN_E_ARG, "null context");
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -1064,6 +1381,10 @@ int mdn_sync(mdn_ctx *c) {
 
 int mdn_last_kernel_ms(mdn_ctx *c, double *ms) {
   if (!c || !ms) return set_error(MDN_E_ARG, "null argument");
+  if (c->last_ms >= 0) {
+    *ms = c->last_ms;
+    return MDN_OK;
+  }
   float f = 0;
   HIPCHK(hipEventSynchronize(c->ev1));
   HIPCHK(hipEventElapsedTime(&f, c->ev0, c->ev1));

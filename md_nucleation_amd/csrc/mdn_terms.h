@@ -53,22 +53,13 @@ struct GStore {
   __device__ double w(const H &q) const { return R->atom_w[q.g]; }
   __device__ double q(const H &h_) const { return R->sp[h_.sp].charge[h_.idx]; }
   __device__ int type(const H &h_) const { return R->sp[h_.sp].type[h_.idx]; }
-};
-
-// LDS store (one workgroup per replica)
-struct SStore {
-  LView L;
-  double *px, *pf, *pw, *pq;
-  const int *ptype;
-  typedef int H;
-  __device__ H h(int a) const { return a; }
-  __device__ void pos(H a, double p[3]) const { p[0] = px[3 * a]; p[1] = px[3 * a + 1]; p[2] = px[3 * a + 2]; }
-  __device__ void fadd(H a, double fx, double fy, double fz) const {
-    atomicAdd(&pf[3 * a], fx); atomicAdd(&pf[3 * a + 1], fy); atomicAdd(&pf[3 * a + 2], fz);
+  // pair_coeff table of setCrossPotentials (setCrossPotential.cpp:4-23)
+  template <typename R2>
+  __device__ void coeff(const DevReplica &Rr, const H &a, const H &b, R2 &c0, R2 &c1) const {
+    const double *p = Rr.pair_coeff + 2 * ((size_t)type(a) * Rr.n_types + type(b));
+    c0 = (R2)p[0];
+    c1 = (R2)p[1];
   }
-  __device__ double w(H a) const { return pw[a]; }
-  __device__ double q(H a) const { return pq[a]; }
-  __device__ int type(H a) const { return ptype[a]; }
 };
 
 __device__ inline void pc(const DevReplica &R, int t1, int t2, double &c0, double &c1) {
@@ -84,16 +75,16 @@ __device__ inline void pair_ab(const DevReplica &R, const S &s, const typename S
   s.pos(ha, xa);
   s.pos(hb, xb);
   double dx = xa[0] - xb[0], dy = xa[1] - xb[1], dz = xa[2] - xb[2];
-  double c0, c1;
-  pc(R, s.type(ha), s.type(hb), c0, c1);
+  Rl c0, c1;
+  s.template coeff<Rl>(R, ha, hb, c0, c1);
   Rl rx = (Rl)dx, ry = (Rl)dy, rz = (Rl)dz;
   Rl rsq = rx * rx + ry * ry + rz * rz;
   Rl w = weighted ? (Rl)(s.w(ha) * s.w(hb)) : Rl(1);
   Rl fp;
   if (COUL)
-    fp = ljcoul_fpair<Rl>(rsq, (Rl)c0, (Rl)c1, (Rl)(R.qqrd2e * s.q(ha) * s.q(hb)), w, eflag, e);
+    fp = ljcoul_fpair<Rl>(rsq, c0, c1, (Rl)(R.qqrd2e * s.q(ha) * s.q(hb)), w, eflag, e);
   else
-    fp = lj_fpair<Rl>(rsq, (Rl)c0, (Rl)c1, w, eflag, e);
+    fp = lj_fpair<Rl>(rsq, c0, c1, w, eflag, e);
   double fx = (double)(fp * rx), fy = (double)(fp * ry), fz = (double)(fp * rz);
   s.fadd(ha, fx, fy, fz);
   s.fadd(hb, -fx, -fy, -fz);
@@ -221,14 +212,18 @@ __device__ inline void dihedral_term(const DevReplica &R, const S &s, int base, 
 // ---- AMBER ion: PotentialAMBER::compute, compute_AMBER.cpp:10-253
 // (non-excluded pairs from initialAMBER, unweighted LJcoul; bonds, angles,
 // dihedrals; all into Uion) ----
-template <typename Rl, class S>
-__device__ void term_amber(const DevReplica &R, const S &s, int t, int nt, bool eflag, double *e) {
-  for (int p = t; p < R.n_long; p += nt)
-    pair_ab<Rl, true>(R, s, s.h(R.long_pairs[2 * p]), s.h(R.long_pairs[2 * p + 1]), false, eflag, e[U_ION]);
+template <class S>
+__device__ void term_amber_bonded(const DevReplica &R, const S &s, int t, int nt, bool eflag, double *e) {
   const DevSpecies &I = R.sp[0];
   for (int p = t; p < I.nbond; p += nt) bond_term(R, s, 0, I.bond + 3 * p, eflag, e[U_ION]);
   for (int p = t; p < I.nangle; p += nt) angle_term(R, s, 0, I.angle + 4 * p, eflag, e[U_ION]);
   for (int p = t; p < I.ndih; p += nt) dihedral_term(R, s, 0, I.dih + 5 * p, eflag, e[U_ION]);
+}
+template <typename Rl, class S>
+__device__ void term_amber(const DevReplica &R, const S &s, int t, int nt, bool eflag, double *e) {
+  for (int p = t; p < R.n_long; p += nt)
+    pair_ab<Rl, true>(R, s, s.h(R.long_pairs[2 * p]), s.h(R.long_pairs[2 * p + 1]), false, eflag, e[U_ION]);
+  term_amber_bonded(R, s, t, nt, eflag, e);
 }
 
 // ---- vapor intra: PotentialVaporIntra::compute, compute_intra.cpp:45-245
@@ -265,12 +260,13 @@ __device__ void term_gintra(const DevReplica &R, const S &s, int t, int nt, bool
 // type (0 = Na, 1 = Cl) ----
 __device__ inline void born_coeff(int t1, int t2, double B[5]) {
   int k = t1 + t2;  // 0 NaNa, 1 NaCl, 2 ClCl
-  const double A[3] = {25.4435, 20.3548, 15.2661}, C[3] = {101.1719, 674.4793, 6985.6786},
-               D[3] = {48.1771, 837.077, 14031.5785}, sg[3] = {2.340, 2.755, 3.170};
-  B[0] = A[k] * 3.1546 / 4.184;
-  B[1] = 6 * C[k] / 4.184;
-  B[2] = 8 * D[k] / 4.184;
-  B[3] = sg[k];
+  double A = k == 0 ? 25.4435 : (k == 1 ? 20.3548 : 15.2661);
+  double C = k == 0 ? 101.1719 : (k == 1 ? 674.4793 : 6985.6786);
+  double D = k == 0 ? 48.1771 : (k == 1 ? 837.077 : 14031.5785);
+  B[0] = A * 3.1546 / 4.184;
+  B[1] = 6 * C / 4.184;
+  B[2] = 8 * D / 4.184;
+  B[3] = k == 0 ? 2.340 : (k == 1 ? 2.755 : 3.170);
   B[4] = 3.1546;
 }
 template <class S>
@@ -518,7 +514,7 @@ __device__ void all_terms(const DevReplica &R, const S &s, uint32_t bits, int t,
 // with an iteration cap.  x: current positions (updated), r: positions before
 // the drift.  Returns iterations used, negative when the cap was hit.
 template <class XA, class RA>
-__device__ int shake_molecule(const DevReplica &R, const DevSpecies &Sp, XA x, RA r, int max_iter) {
+__device__ int shake_molecule_generic(const DevReplica &R, const DevSpecies &Sp, XA x, RA r, int max_iter) {
   const double tolerance = 1e-20, rho_zero = 0.01;
   int it = 0;
   bool converge = false;
@@ -550,6 +546,64 @@ __device__ int shake_molecule(const DevReplica &R, const DevSpecies &Sp, XA x, R
   return it;
 }
 
+// Same iteration with the constraint data (atoms, old bond vector, d^2,
+// inverse masses) read once into registers: up to kMaxRigid constraints per
+// molecule (methanol under SHAKE H has 4, water 2); more take the generic loop.
+constexpr int kMaxRigid = 4;
+template <class XA, class RA>
+__device__ int shake_molecule(const DevReplica &R, const DevSpecies &Sp, XA x, RA r, int max_iter) {
+  const int nr = Sp.nrigid;
+  if (nr > kMaxRigid) return shake_molecule_generic(R, Sp, x, r, max_iter);
+  const double tolerance = 1e-20, rho_zero = 0.01;
+  int ci_[kMaxRigid], cj_[kMaxRigid];
+  double rk[kMaxRigid][3], dk2[kMaxRigid], mi[kMaxRigid], mj[kMaxRigid];
+#pragma unroll
+  for (int k = 0; k < kMaxRigid; k++) {
+    if (k >= nr) break;
+    int i = Sp.rigid[3 * k], j = Sp.rigid[3 * k + 1];
+    double dk = R.btype[2 * Sp.rigid[3 * k + 2] + 1];
+    ci_[k] = i; cj_[k] = j;
+    dk2[k] = dk * dk;
+    mi[k] = Sp.invmass[i]; mj[k] = Sp.invmass[j];
+    for (int q = 0; q < 3; q++) rk[k][q] = r(i, q) - r(j, q);
+  }
+  int it = 0;
+  bool converge = false;
+  while (!converge) {
+    if (it >= max_iter) return -it;
+    it++;
+    converge = true;
+#pragma unroll
+    for (int k = 0; k < kMaxRigid; k++) {
+      if (k >= nr) break;
+      const int i = ci_[k], j = cj_[k];
+      double rn[3];
+      for (int q = 0; q < 3; q++) rn[q] = x(i, q) - x(j, q);
+      double rn2 = rn[0] * rn[0] + rn[1] * rn[1] + rn[2] * rn[2];
+      double dot = rk[k][0] * rn[0] + rk[k][1] * rn[1] + rk[k][2] * rn[2];
+      double mred_inv = mi[k] + mj[k];
+      double ramk = (dk2[k] - rn2) / 2 * mred_inv;
+      if (dot > 1e-10) ramk /= dot;
+      else {
+        double rk2 = rk[k][0] * rk[k][0] + rk[k][1] * rk[k][1] + rk[k][2] * rk[k][2];
+        ramk = 0.5 * rho_zero + ramk / (rk2 * rho_zero) * mred_inv;
+      }
+      double cfi = ramk * mi[k], cfj = ramk * mj[k];
+      for (int q = 0; q < 3; q++) {
+        double ni = x(i, q) + rk[k][q] * cfi, nj = x(j, q) - rk[k][q] * cfj;
+        x(i, q) = ni;
+        x(j, q) = nj;
+        rn[q] = ni - nj;
+      }
+      rn2 = rn[0] * rn[0] + rn[1] * rn[1] + rn[2] * rn[2];
+      double dr = rn2 - dk2[k];
+      if (!(dr * dr <= 1e30)) return -it;  // diverged (or non-finite)
+      if (dr * dr > tolerance) converge = false;
+    }
+  }
+  return it;
+}
+
 // a failed SHAKE leaves the molecule unconstrained for this step (the
 // reference, which never calls shake(), always does): x = x_old + v dt
 template <class XA, class RA, class VA>
@@ -562,10 +616,20 @@ __device__ void shake_undo(const DevSpecies &Sp, XA x, RA r, VA v, double dt) {
 // constrained half-step velocity (x_shaken - x_old) / dt
 template <class XA, class RA, class VA>
 __device__ void shake_velocity(const DevSpecies &Sp, XA x, RA r, VA v, double dt) {
+  uint64_t cons = 0;  // constrained atoms of molecules with <= 64 atoms
+  for (int k = 0; k < Sp.nrigid; k++) {
+    int i = Sp.rigid[3 * k], j = Sp.rigid[3 * k + 1];
+    if (i < 64) cons |= 1ull << i;
+    if (j < 64) cons |= 1ull << j;
+  }
   for (int a = 0; a < Sp.natoms; a++) {
-    bool cons = false;
-    for (int k = 0; k < Sp.nrigid && !cons; k++) cons = Sp.rigid[3 * k] == a || Sp.rigid[3 * k + 1] == a;
-    if (!cons) continue;
+    bool c;
+    if (a < 64) c = (cons >> a) & 1ull;
+    else {
+      c = false;
+      for (int k = 0; k < Sp.nrigid && !c; k++) c = Sp.rigid[3 * k] == a || Sp.rigid[3 * k + 1] == a;
+    }
+    if (!c) continue;
     for (int q = 0; q < 3; q++) v(a, q) = (x(a, q) - r(a, q)) / dt;
   }
 }

@@ -219,7 +219,7 @@ def test_lds_capacity_overflow_is_an_error(tmp_path):
     eng.add(sd, st)
     eng.commit()
     eng.make_pair(0)
-    eng.set_batch_shape(256, 32, 0)
+    eng.set_batch_shape(256, 32, 0, 0)
     eng.run_batched(5)
     eng.sync()
     with pytest.raises(M.MdnError, match="status"):
