@@ -122,9 +122,12 @@ int mdn_sysdef_load(const char *cond_path, mdn_sysdef **out);
 int mdn_sysdef_view(const mdn_sysdef *d, mdn_system *out);
 int mdn_sysdef_sizes(const mdn_sysdef *d, int32_t *n_mol, int32_t *n_atoms);
 /* SHAKE mode: 0 off (the reference as shipped never calls shake(),
- * verlet.cpp:24); 1 (default under "SHAKE H") SHAKE of shake.cpp:44-92 plus
- * the velocity correction v = (x_shaken - x_old)/dt of constrained atoms;
- * 2 shake.cpp's positions-only update.  max_iter caps the iteration. */
+ * verlet.cpp:24); 1 (default under "SHAKE H") Gauss-Seidel SHAKE over the
+ * rigid pairs with the textbook update (shake.cpp:67 multiplies by the
+ * inverse reduced mass where SHAKE divides; both reach the same constraint
+ * solution) plus the velocity correction v = (x_shaken - x_old)/dt of
+ * constrained atoms; 2 shake.cpp's own update, positions only.  max_iter caps
+ * the iteration. */
 int mdn_sysdef_set_shake(mdn_sysdef *d, int32_t shake, int32_t max_iter);
 int mdn_sysdef_set_flags(mdn_sysdef *d, uint32_t flags);
 int mdn_sysdef_init_state(const mdn_sysdef *d, uint64_t seed, mdn_state *st);

@@ -150,6 +150,18 @@ __device__ inline double blend_weight(double r2) {
   double C = cos(kPI / kOverlap * 0.5 * dr);
   return C * C;
 }
+// the weight in pair precision: FP32 sqrt/cos in mixed mode (the weight only
+// scales FP32 pair forces there)
+template <typename R>
+__device__ inline R blend_weight_r(double r2) { return (R)blend_weight(r2); }
+template <>
+__device__ inline float blend_weight_r<float>(double r2) {
+  if (r2 < kRi2) return 1.0f;
+  if (r2 > kRI2) return 0.0f;
+  float dr = sqrtf((float)r2) - (float)kRcenter;
+  float C = __cosf((float)(kPI / kOverlap * 0.5) * dr);
+  return C * C;
+}
 
 // ------------------------------------------------------------------------
 // Pair arithmetic.  R = double (plain) or float (mixed: the displacement is

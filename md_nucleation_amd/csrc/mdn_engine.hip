@@ -601,7 +601,7 @@ __global__ void __launch_bounds__(512) k_batched(const DevReplica *__restrict__ 
       // blend weights (verlet.cpp:109-134)
       for (int a = L.n_ion + tid; a < L.n_in; a += nt) {
         double dx = B.x[3 * a] - cx, dy = B.x[3 * a + 1] - cy, dz = B.x[3 * a + 2] - cz;
-        B.wr[a] = (Rl)blend_weight(dx * dx + dy * dy + dz * dz);
+        B.wr[a] = blend_weight_r<Rl>(dx * dx + dy * dy + dz * dz);
       }
       // check_pairlist (pairlist.cpp:58-63)
       lp++;

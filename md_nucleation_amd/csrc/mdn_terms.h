@@ -530,8 +530,12 @@ __device__ int shake_molecule_generic(const DevReplica &R, const DevSpecies &Sp,
       double rk2 = rk[0] * rk[0] + rk[1] * rk[1] + rk[2] * rk[2];
       double rn2 = rn[0] * rn[0] + rn[1] * rn[1] + rn[2] * rn[2];
       double dot = rk[0] * rn[0] + rk[1] * rn[1] + rk[2] * rn[2];
+      if (R.shake != 2) {  // mode 1 tests each constraint before its update
+        double d0 = rn2 - dk * dk;
+        if (d0 * d0 > tolerance) converge = false;
+      }
       double mred_inv = Sp.invmass[i] + Sp.invmass[j];
-      double ramk = (dk * dk - rn2) / 2 * mred_inv;
+      double ramk = R.shake == 2 ? (dk * dk - rn2) / 2 * mred_inv : (dk * dk - rn2) / 2 / mred_inv;
       if (dot > 1e-10) ramk /= dot;
       else ramk = 0.5 * rho_zero + ramk / (rk2 * rho_zero) * mred_inv;
       double ci = ramk * Sp.invmass[i], cj = ramk * Sp.invmass[j];
@@ -540,7 +544,7 @@ __device__ int shake_molecule_generic(const DevReplica &R, const DevSpecies &Sp,
       rn2 = rn[0] * rn[0] + rn[1] * rn[1] + rn[2] * rn[2];
       double dr = rn2 - dk * dk;
       if (!(dr * dr <= 1e30)) return -it;  // diverged (or non-finite)
-      if (dr * dr > tolerance) converge = false;
+      if (R.shake == 2 && dr * dr > tolerance) converge = false;
     }
   }
   return it;
@@ -581,8 +585,14 @@ __device__ int shake_molecule(const DevReplica &R, const DevSpecies &Sp, XA x, R
       for (int q = 0; q < 3; q++) rn[q] = x(i, q) - x(j, q);
       double rn2 = rn[0] * rn[0] + rn[1] * rn[1] + rn[2] * rn[2];
       double dot = rk[k][0] * rn[0] + rk[k][1] * rn[1] + rk[k][2] * rn[2];
+      if (R.shake != 2) {  // mode 1 tests each constraint before its update
+        double d0 = rn2 - dk2[k];
+        if (d0 * d0 > tolerance) converge = false;
+      }
       double mred_inv = mi[k] + mj[k];
-      double ramk = (dk2[k] - rn2) / 2 * mred_inv;
+      // mode 2: shake.cpp:67 multiplies by the inverse reduced mass; mode 1
+      // divides by it (textbook SHAKE); both reach the same fixed point
+      double ramk = R.shake == 2 ? (dk2[k] - rn2) / 2 * mred_inv : (dk2[k] - rn2) / 2 / mred_inv;
       if (dot > 1e-10) ramk /= dot;
       else {
         double rk2 = rk[k][0] * rk[k][0] + rk[k][1] * rk[k][1] + rk[k][2] * rk[k][2];
@@ -598,7 +608,7 @@ __device__ int shake_molecule(const DevReplica &R, const DevSpecies &Sp, XA x, R
       rn2 = rn[0] * rn[0] + rn[1] * rn[1] + rn[2] * rn[2];
       double dr = rn2 - dk2[k];
       if (!(dr * dr <= 1e30)) return -it;  // diverged (or non-finite)
-      if (dr * dr > tolerance) converge = false;
+      if (R.shake == 2 && dr * dr > tolerance) converge = false;
     }
   }
   return it;

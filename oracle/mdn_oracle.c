@@ -249,8 +249,16 @@ int orc_shake_molecule(orc_ctx *c, double *x, const double *r, int sp) {
       double _rkx = x[3*i] - x[3*j], _rky = x[3*i+1] - x[3*j+1], _rkz = x[3*i+2] - x[3*j+2];
       double _rk2 = _rkx * _rkx + _rky * _rky + _rkz * _rkz;
       double dot = rkx * _rkx + rky * _rky + rkz * _rkz;
+      /* mode 1 tests each constraint before its update (a later update in
+       * the sweep can break it again); mode 2 tests after it, as shake.cpp */
+      if (s->shake != 2) {
+        double d0 = _rk2 - dk * dk;
+        if (d0 * d0 > tolerance) converge = 0;
+      }
       double mred_inv = 1 / m[i] + 1 / m[j];
-      double ramk = (dk * dk - _rk2) / 2 * mred_inv;
+      /* mode 2 restates shake.cpp:67 (multiplies by the inverse reduced
+       * mass); mode 1 divides by it (textbook SHAKE), same fixed point */
+      double ramk = s->shake == 2 ? (dk * dk - _rk2) / 2 * mred_inv : (dk * dk - _rk2) / 2 / mred_inv;
       if (dot > 1e-10) ramk /= dot;
       else ramk = 0.5 * rho_zero + ramk / (rk2 * rho_zero) * mred_inv;
       double coeffi = ramk / m[i], coeffj = ramk / m[j];
@@ -260,7 +268,7 @@ int orc_shake_molecule(orc_ctx *c, double *x, const double *r, int sp) {
       _rk2 = _rkx * _rkx + _rky * _rky + _rkz * _rkz;
       double dr = _rk2 - dk * dk;
       if (!(dr * dr <= 1e30)) return -it; /* diverged (or non-finite) */
-      if (dr * dr > tolerance) converge = 0;
+      if (s->shake == 2 && dr * dr > tolerance) converge = 0;
     }
   }
   return it;

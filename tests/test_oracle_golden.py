@@ -70,7 +70,8 @@ def test_make_pair_deterministic_part(name):
 def test_shake_fixed_point():
     for name in CASES:
         fx = G.load(name)
-        orc, sysd = G.oracle_for(fx, shake=1)
+        # mode 2 is shake.cpp's own iteration
+        orc, sysd = G.oracle_for(fx, shake=2)
         keys = [k for k in fx if k.startswith("shake") and k.endswith("_out")]
         assert keys
         for k in keys:
@@ -80,6 +81,22 @@ def test_shake_fixed_point():
             assert it > 0
             # same iteration as the reference -> same fixed point to rounding
             np.testing.assert_allclose(new, out, rtol=0, atol=1e-12)
+
+
+def test_shake_modes_share_the_fixed_point():
+    """Mode 1 (textbook update) converges to the constraint solution of mode 2
+    (shake.cpp's update) and needs fewer sweeps."""
+    fx = G.load("c1")
+    o1, _ = G.oracle_for(fx, shake=1)
+    o2, _ = G.oracle_for(fx, shake=2)
+    for k in [k for k in fx if k.startswith("shake") and k.endswith("_out")]:
+        mol = k[len("shake"):-len("_out")]
+        old = fx["shake%s_old" % mol]
+        a, b = fx["shake%s_new" % mol].copy(), fx["shake%s_new" % mol].copy()
+        i1 = O.lib().orc_shake_molecule(o1.h, a.ctypes.data_as(O._dp), old.ctypes.data_as(O._dp), 2)
+        i2 = O.lib().orc_shake_molecule(o2.h, b.ctypes.data_as(O._dp), old.ctypes.data_as(O._dp), 2)
+        assert 0 < i1 < i2
+        np.testing.assert_allclose(a, b, rtol=0, atol=1e-9)
 
 
 def test_born_mayer_huggins():
