@@ -80,6 +80,17 @@ def total_atom_steps(eng, n):
     return sum(eng.counters(r)[0] for r in range(n))
 
 
+def _oracle_for(sd, seed):
+    """The FP64 oracle on the same parsed system (CPU-baseline leg only)."""
+    from oracle import oracle as O
+    v = sd.view
+    sysd = {k: v[k] for k in ("pair_coeff", "btype", "ctype", "dmulti", "dcoeff", "species", "n_gas", "n_vap",
+                              "ion_type_base", "dt", "T", "d_size", "margin", "ML2", "m_gas_kg", "m_vap_kg",
+                              "gas_spin", "flags", "efield", "shake", "shake_max_iter")}
+    sysd.update(seed=seed, replica=0)
+    return O.Oracle(sysd)
+
+
 def cpu_baseline(reps, seconds=12.0):
     """FP64 oracle on the host cores: one replica per OpenMP thread."""
     from oracle import oracle as O
@@ -87,12 +98,7 @@ def cpu_baseline(reps, seconds=12.0):
     sample = reps[:threads]
     orcs, states = [], []
     for sd, st, seed, _ in sample:
-        v = sd.view
-        sysd = {k: v[k] for k in ("pair_coeff", "btype", "ctype", "dmulti", "dcoeff", "species", "n_gas", "n_vap",
-                                  "ion_type_base", "dt", "T", "d_size", "margin", "ML2", "m_gas_kg", "m_vap_kg",
-                                  "gas_spin", "flags", "efield", "shake", "shake_max_iter")}
-        sysd.update(seed=seed, replica=0)
-        orc = O.Oracle(sysd)
+        orc = _oracle_for(sd, seed)
         S = orc.new_state(st.as_dict())
         orc.make_pair(S)
         orcs.append(orc)
@@ -110,6 +116,74 @@ def cpu_baseline(reps, seconds=12.0):
                       % (len(sample), len(sample), nsteps, el, threads)}
 
 
+def copy_check(nbytes):
+    """A plain device-to-device copy of known size (the reference point for
+    rocprofv3's FETCH_SIZE / WRITE_SIZE counters)."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(nbytes)) == 0
+    assert hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3) == 0  # device to device
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print("copy_check: %d bytes device-to-device" % nbytes, file=sys.stderr, flush=True)
+
+
+def cluster_bench(d, steps=40, cpu_steps=3, with_cpu=True):
+    """The 12,001-atom cluster (Na+ + 2,000 methanol, 100,000 Ar) through the
+    per-call path: mixed and FP64 throughput, and the all-pairs vv kernel
+    alone against the FP32 VALU peak."""
+    from md_nucleation_amd import _lib as M
+    from md_nucleation_amd import workloads as W
+    sd, st0, seed = W.cluster(d)
+    st0.atom_w[:] = 1.0  # all atoms within 100 A of the ion
+    out = {"workload": "cluster: Na+ + 2000 methanol at 0.79 g/cm3 (12,001 atoms, all-pairs vv/vi) + 100,000 Ar"}
+    for name, prec in (("mixed", M.MIXED), ("fp64", M.FP64)):
+        eng = M.Engine(0)
+        eng.add(sd, st0, seed=seed)
+        eng.commit()
+        eng.make_pair(0)
+        eng.verlet(0, 2, precision=prec)
+        eng.sync()
+        a0 = eng.counters(0)[0]
+        t0 = time.perf_counter()
+        eng.verlet(0, steps, precision=prec)
+        eng.sync()
+        el = time.perf_counter() - t0
+        out[name] = {"value": (eng.counters(0)[0] - a0) / el, "unit": "atom-steps/s", "ms_per_step": el * 1e3 / steps}
+        # the vv kernel alone (one launch, 20 repeats)
+        st = eng.get_state(0)
+        nv = int(st.region[1 + sd.view["n_gas"]:].sum())
+        na = sd.view["species"][2]["mass"].size
+        pairs = nv * (nv - 1) // 2 * na * na
+        ms = []
+        for _ in range(20):
+            eng.compute(0, M.MDN_VV, 0, prec)
+            ms.append(eng.last_kernel_ms())
+        t = float(np.median(ms)) * 1e-3
+        peak = PEAK_FP32_TFLOPS if prec == M.MIXED else PEAK_FP64_TFLOPS
+        ach = pairs * FLOP["ljcoul_w"] / t / 1e12
+        out[name]["vv_kernel"] = {"kernel": "k_vv_tiled<%s>" % ("float" if prec == M.MIXED else "double"),
+                                  "unique_pairs": pairs, "ms": t * 1e3, "achieved_tflops": ach, "peak_tflops": peak,
+                                  "frac": ach / peak}
+        eng.close()
+    if with_cpu:
+        orc = _oracle_for(sd, seed)
+        S = orc.new_state(st0.as_dict())
+        orc.make_pair(S)
+        t0 = time.perf_counter()
+        rc, acc = orc.verlet(S, cpu_steps)
+        el = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": acc / el, "unit": "atom-steps/s", "cores": 1, "kind": "port",
+                               "sample": "%d steps of the cluster, FP64 oracle, one thread (one replica)" % cpu_steps}
+        out["mixed_vs_cpu"] = out["mixed"]["value"] / out["cpu_baseline"]["value"]
+    out["mixed_vs_fp64"] = out["mixed"]["value"] / out["fp64"]["value"]
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -120,6 +194,9 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=300)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-cluster", action="store_true", help="skip the 12,001-atom cluster measurement")
+    ap.add_argument("--copy-check", action="store_true",
+                    help="also run a 256 MiB device-to-device copy (calibrates HBM byte counters under rocprofv3 --pmc)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; replicas would shard across GPUs with no exchange "
@@ -192,6 +269,9 @@ def main():
         ep.close()
 
         cpu = None if args.no_cpu else cpu_baseline(reps, args.cpu_seconds)
+        cl = None if args.no_cluster else cluster_bench(os.path.join(d, "cluster"), with_cpu=not args.no_cpu)
+        if args.copy_check:
+            copy_check(256 << 20)
 
     n_in = (a1 - a0) / args.steps / n
     out = {
@@ -212,6 +292,7 @@ def main():
                      "lds_cap_atoms": lds_cap,
                      "flops_per_step": flops},
         "cpu_baseline": cpu,
+        "cluster": cl,
     }
     if cpu:
         out["vs_cpu"] = value / cpu["value"]

@@ -953,6 +953,26 @@ static int grid_for(int n, int blk = kBlk, int cap = 2048) {
   return std::max(1, std::min(g, cap));
 }
 
+// every selected potential of one replica through the per-call kernels; vv
+// goes to the tiled all-pairs kernel when the vapor's types fit its table
+template <typename Rl>
+static int launch_terms(mdn_ctx *c, const HostReplica &h, const DevReplica *R, uint32_t bits, bool eflag) {
+  const SysDef &d = h.def;
+  int tv0 = d.sp[2].natoms ? *std::min_element(d.sp[2].type.begin(), d.sp[2].type.end()) : 0;
+  int ntv = d.sp[2].natoms ? *std::max_element(d.sp[2].type.begin(), d.sp[2].type.end()) - tv0 + 1 : 1;
+  bool tiled = ntv <= 8;
+  uint32_t rest = tiled ? bits & ~(uint32_t)MDN_VV : bits;
+  if (rest) hipLaunchKernelGGL(k_terms<Rl>, dim3(512), dim3(kBlk), 0, c->stream, R, rest, eflag ? 1 : 0);
+  if ((bits & MDN_VV) && tiled) {
+    int nva = d.n_vap * d.sp[2].natoms;  // bound on the in-domain vapor atoms
+    int gx = std::max(1, (nva + kVVTile - 1) / kVVTile);
+    int gy = std::max(1, std::min(64, (4 * c->n_cu + gx - 1) / gx));
+    hipLaunchKernelGGL(k_vv_tiled<Rl>, dim3(gx, gy), dim3(kVVTile), 0, c->stream, R, eflag ? 1 : 0, tv0, ntv);
+  }
+  HIPCHK(hipGetLastError());
+  return MDN_OK;
+}
+
 // one MD::verlet step of one replica through the per-call kernels
 template <typename Rl>
 static int launch_step(mdn_ctx *c, const HostReplica &h, const DevReplica *R, bool eflag) {
@@ -964,18 +984,8 @@ static int launch_step(mdn_ctx *c, const HostReplica &h, const DevReplica *R, bo
   if (sh) hipLaunchKernelGGL(k_shake, dim3(grid_for(d.n_mol, 64)), dim3(64), 0, c->stream, R);
   hipLaunchKernelGGL(k_com_weights, dim3(1), dim3(1024), 0, c->stream, R);
   hipLaunchKernelGGL(k_rebuild, dim3(1), dim3(1024), 0, c->stream, R, 0);
-  uint32_t bits = d.flags;
-  int tv0 = d.sp[2].natoms ? *std::min_element(d.sp[2].type.begin(), d.sp[2].type.end()) : 0;
-  int ntv = d.sp[2].natoms ? *std::max_element(d.sp[2].type.begin(), d.sp[2].type.end()) - tv0 + 1 : 1;
-  bool tiled = ntv <= 8;
-  hipLaunchKernelGGL(k_terms<Rl>, dim3(512), dim3(kBlk), 0, c->stream, R, tiled ? bits & ~(uint32_t)MDN_VV : bits,
-                     eflag ? 1 : 0);
-  if ((bits & MDN_VV) && tiled) {
-    int nva = d.n_vap * d.sp[2].natoms;
-    int gx = std::max(1, (nva + kVVTile - 1) / kVVTile);
-    int gy = std::max(1, std::min(64, 2048 / gx));
-    hipLaunchKernelGGL(k_vv_tiled<Rl>, dim3(gx, gy), dim3(kVVTile), 0, c->stream, R, eflag ? 1 : 0, tv0, ntv);
-  }
+  int rc = launch_terms<Rl>(c, h, R, d.flags, eflag);
+  if (rc) return rc;
   hipLaunchKernelGGL(k_kick, dim3(g), dim3(kBlk), 0, c->stream, R, 0, 1);
   HIPCHK(hipGetLastError());
   return MDN_OK;
@@ -1159,12 +1169,12 @@ int mdn_compute(mdn_ctx *c, int32_t r, uint32_t bits, int32_t eflag, int32_t pre
   if (rc) return rc;
   HIPCHK(hipSetDevice(c->device));
   const HostReplica &h = c->reps[r];
-  if (precision == MDN_MIXED)
-    hipLaunchKernelGGL(k_terms<float>, dim3(512), dim3(kBlk), 0, c->stream, c->d_reps + r, bits, eflag);
-  else
-    hipLaunchKernelGGL(k_terms<double>, dim3(512), dim3(kBlk), 0, c->stream, c->d_reps + r, bits, eflag);
-  HIPCHK(hipGetLastError());
-  return MDN_OK;
+  HIPCHK(hipEventRecord(c->ev0, c->stream));
+  rc = precision == MDN_MIXED ? launch_terms<float>(c, h, c->d_reps + r, bits, eflag != 0)
+                              : launch_terms<double>(c, h, c->d_reps + r, bits, eflag != 0);
+  HIPCHK(hipEventRecord(c->ev1, c->stream));
+  c->last_ms = -1;
+  return rc;
 }
 
 int mdn_velocity_calculation(mdn_ctx *c, int32_t r) {

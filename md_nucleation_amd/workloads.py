@@ -33,6 +33,14 @@ def batched_replicas(d, n_rep=512, n_gas=5000, seed=2024):
     return out
 
 
+def cluster(d, n_vap=2000, n_gas=100000, seed=3):
+    """Na+ with 2,000 methanol packed at 0.79 g/cm3 around it (12,001 atoms,
+    all-pairs vv/vi) and 100,000 Ar: returns (SysDef, State, seed)."""
+    os.makedirs(d, exist_ok=True)
+    sd = M.SysDef(configs.config3(d, n_gas=n_gas, n_vap=n_vap))
+    return sd, sd.init_state(seed), seed
+
+
 def build_engine(reps, device=0):
     eng = M.Engine(device)
     for r, (sd, st, s, _) in enumerate(reps):
