@@ -235,9 +235,15 @@ def main():
         e64.close()
         e32.close()
 
-        # batched mixed precision: warmup, then the timed region
+        # batched mixed precision: warmup (the launch-shape autotuner's trials
+        # are the warmup steps), then the timed region
+        tuned = None
         if args.warmup:
-            eng.run_batched(args.warmup, precision=M.MIXED)
+            from md_nucleation_amd import autotune
+            per = max(10, args.warmup // len(autotune.DEFAULT_SHAPES))
+            best, res = autotune.tune_batched(eng, n, steps=per, precision=M.MIXED)
+            tuned = {"shape": list(best), "trials": {str(list(k)): v for k, v in res.items()},
+                     "steps_per_trial": per}
         eng.sync()
         a0 = total_atom_steps(eng, n)
         t0 = time.perf_counter()
@@ -292,6 +298,7 @@ def main():
                      "lds_cap_atoms": lds_cap,
                      "flops_per_step": flops},
         "cpu_baseline": cpu,
+        "autotune": tuned,
         "cluster": cl,
     }
     if cpu:

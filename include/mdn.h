@@ -155,6 +155,13 @@ int mdn_update_position(mdn_ctx *c, int32_t replica);
 int mdn_verlet(mdn_ctx *c, int32_t replica, int32_t nsteps,
                int32_t eflag_every, int64_t step0, int32_t precision);
 
+/* mdn_verlet with each step's kernel sequence captured once into a hipGraph
+ * (per replica, precision and energy flag) and replayed; the same arithmetic
+ * as mdn_verlet.  No reference counterpart: launch-overhead relief for a
+ * single large replica. */
+int mdn_verlet_graph(mdn_ctx *c, int32_t replica, int32_t nsteps,
+                     int32_t eflag_every, int64_t step0, int32_t precision);
+
 /* batched path: all replicas, nsteps in one persistent launch */
 int mdn_run_batched(mdn_ctx *c, int32_t nsteps, int32_t eflag_every,
                     int64_t step0, int32_t precision);

@@ -74,6 +74,7 @@ _SIGS = {
     "mdn_velocity_calculation": (C.c_int, [C.c_void_p, C.c_int32]),
     "mdn_update_position": (C.c_int, [C.c_void_p, C.c_int32]),
     "mdn_verlet": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int64, C.c_int32]),
+    "mdn_verlet_graph": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int64, C.c_int32]),
     "mdn_run_batched": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int64, C.c_int32]),
     "mdn_set_batch_shape": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32]),
     "mdn_get_counters": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_int64), C.POINTER(C.c_int64), _ip,
@@ -286,6 +287,10 @@ class Engine:
     def verlet(self, r, nsteps, eflag_every=0, step0=0, precision=FP64):
         check(lib().mdn_verlet(self.h, int(r), int(nsteps), int(eflag_every), int(step0), int(precision)),
               "mdn_verlet")
+
+    def verlet_graph(self, r, nsteps, eflag_every=0, step0=0, precision=FP64):
+        check(lib().mdn_verlet_graph(self.h, int(r), int(nsteps), int(eflag_every), int(step0), int(precision)),
+              "mdn_verlet_graph")
 
     def run_batched(self, nsteps, eflag_every=0, step0=0, precision=MIXED):
         check(lib().mdn_run_batched(self.h, int(nsteps), int(eflag_every), int(step0), int(precision)),

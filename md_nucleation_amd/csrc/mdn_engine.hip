@@ -538,6 +538,9 @@ __global__ void __launch_bounds__(512) k_batched(const DevReplica *__restrict__ 
     long long acc_steps = 0;
     const DevSpecies &I = R.sp[0];
     const bool shake = R.shake != 0;
+    // SHAKE diagnostics accumulate in LDS (sint[32]: max iterations,
+    // sint[33]: failures) and reach the replica's counters once per launch
+    if (tid == 0) { B.sint[32] = 0; B.sint[33] = 0; }
     __syncthreads();
     bool dead = false;
     for (int s = 0; s < nsteps && !dead; s++) {
@@ -568,12 +571,12 @@ __global__ void __launch_bounds__(512) k_batched(const DevReplica *__restrict__ 
           auto V = [&](int i, int q) -> double & { return v[3 * i + q]; };
           int it = shake_molecule(R, S, X, XO, R.shake_max_iter);
           if (it < 0) {
-            atomicAdd(&D->shake_fail, 1ull);
-            atomicMax(&D->shake_iters_max, -it);
+            atomicAdd(&B.sint[33], 1);
+            atomicMax(&B.sint[32], -it);
             shake_undo(S, X, XO, V, R.dt);
             continue;
           }
-          atomicMax(&D->shake_iters_max, it);
+          atomicMax(&B.sint[32], it);
           if (R.shake == 1) shake_velocity(S, X, XO, V, R.dt);
         }
         __syncthreads();
@@ -655,7 +658,11 @@ __global__ void __launch_bounds__(512) k_batched(const DevReplica *__restrict__ 
       if (tid < 3) { R.mol_x[tid] = B.sdbl[tid]; R.mol_v[tid] = B.sdbl[3 + tid]; }
       if (tid == 0) D->loopPair = lp;
     }
-    if (tid == 0) D->atom_steps += acc_steps;
+    if (tid == 0) {
+      D->atom_steps += acc_steps;
+      atomicMax(&D->shake_iters_max, B.sint[32]);
+      if (B.sint[33]) atomicAdd(&D->shake_fail, (unsigned long long)B.sint[33]);
+    }
     __syncthreads();
   }
 }
@@ -681,6 +688,7 @@ struct HostReplica {
   std::vector<int> long_pairs;
   DevReplica dev;
   size_t off_dyn = 0;
+  hipGraphExec_t graph[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};  // [precision][eflag]
 };
 
 }  // namespace mdn
@@ -782,7 +790,15 @@ static int lists_to_local(const HostReplica &h, std::vector<int> &gi, std::vecto
   return 0;
 }
 
+static void drop_graphs(mdn_ctx *c) {
+  for (auto &h : c->reps)
+    for (int p = 0; p < 2; p++)
+      for (int e = 0; e < 2; e++)
+        if (h.graph[p][e]) { (void)hipGraphExecDestroy(h.graph[p][e]); h.graph[p][e] = nullptr; }
+}
+
 static int upload_all(mdn_ctx *c) {
+  drop_graphs(c);
   Layout lay;
   struct Offs {
     size_t sp[3][12];
@@ -1023,6 +1039,7 @@ int mdn_create(int device, mdn_ctx **out) {
 void mdn_destroy(mdn_ctx *c) {
   if (!c) return;
   hipSetDevice(c->device);
+  drop_graphs(c);
   if (c->stream) hipStreamSynchronize(c->stream);
   if (c->dbuf) hipFree(c->dbuf);
   if (c->ev0) hipEventDestroy(c->ev0);
@@ -1211,6 +1228,35 @@ int mdn_verlet(mdn_ctx *c, int32_t r, int32_t nsteps, int32_t eflag_every, int64
   }
   HIPCHK(hipEventRecord(c->ev1, c->stream));
   c->last_ms = -1;  // read from the events on demand
+  return MDN_OK;
+}
+
+// mdn_verlet with each step's launch sequence captured once into a hipGraph
+// (per replica, precision and energy flag) and replayed
+int mdn_verlet_graph(mdn_ctx *c, int32_t r, int32_t nsteps, int32_t eflag_every, int64_t step0, int32_t precision) {
+  int rc = check_replica(c, r);
+  if (rc) return rc;
+  HIPCHK(hipSetDevice(c->device));
+  HostReplica &h = c->reps[r];
+  const int p = precision == MDN_MIXED ? 1 : 0;
+  HIPCHK(hipEventRecord(c->ev0, c->stream));
+  for (int s = 0; s < nsteps; s++) {
+    const int ef = eflag_every > 0 && ((step0 + s + 1) % eflag_every) == 0;
+    if (!h.graph[p][ef]) {
+      hipGraph_t g = nullptr;
+      HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      rc = p ? launch_step<float>(c, h, c->d_reps + r, ef != 0) : launch_step<double>(c, h, c->d_reps + r, ef != 0);
+      hipError_t e2 = hipStreamEndCapture(c->stream, &g);
+      if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+      if (e2 != hipSuccess) return set_error(MDN_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e2));
+      hipError_t e3 = hipGraphInstantiate(&h.graph[p][ef], g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (e3 != hipSuccess) return set_error(MDN_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e3));
+    }
+    HIPCHK(hipGraphLaunch(h.graph[p][ef], c->stream));
+  }
+  HIPCHK(hipEventRecord(c->ev1, c->stream));
+  c->last_ms = -1;
   return MDN_OK;
 }
 

@@ -224,3 +224,27 @@ def test_lds_capacity_overflow_is_an_error(tmp_path):
     eng.sync()
     with pytest.raises(M.MdnError, match="status"):
         eng.counters(0)
+
+
+def test_graph_replays_match_eager(tmp_path):
+    """The captured per-call step, replayed many times with eager work and
+    synchronisations in between, gives the eager path's trajectory."""
+    sd = M.SysDef(_cond(tmp_path, margin=2.0))
+    st0 = sd.init_state(21)
+    engs = []
+    for _ in range(2):
+        e = M.Engine(0)
+        e.add(sd, st0, seed=21)
+        e.commit()
+        e.make_pair(0)
+        engs.append(e)
+    eager, graphed = engs
+    for chunk in range(4):
+        eager.verlet(0, 25, eflag_every=10, step0=25 * chunk, precision=M.FP64)
+        graphed.verlet_graph(0, 25, eflag_every=10, step0=25 * chunk, precision=M.FP64)
+        graphed.sync()
+        a, b = eager.get_state(0), graphed.get_state(0)
+        np.testing.assert_array_equal(a.region, b.region)
+        assert np.abs(a.atom_x - b.atom_x).max() < 1e-9, chunk
+        np.testing.assert_allclose(a.U, b.U, rtol=1e-9, atol=1e-9)
+        assert a.c.rebuild_count == b.c.rebuild_count
