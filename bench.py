@@ -24,6 +24,10 @@ import tempfile
 import time
 
 import numpy as np
+# torch (used by the differentiable-term section) brings its own HIP runtime;
+# importing it before libmdn loads makes libmdn bind to that same runtime, so
+# the process holds one HIP runtime rather than two
+import torch  # noqa: F401
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -184,6 +188,36 @@ def cluster_bench(d, steps=40, cpu_steps=3, with_cpu=True):
     return out
 
 
+def adjoint_bench(d, n_snap=256, repeats=5):
+    """The differentiable vi term on the force-matching workload: 256 FP64
+    frames of the 150-atom chain with 200 water packed around it; forward
+    (E, F) and adjoint (gradients of a force-matching loss) wall time per
+    call, host buffers in and out included."""
+    from md_nucleation_amd import _lib as M
+    from md_nucleation_amd import adjoint as A
+    from md_nucleation_amd import configs
+    sd = M.SysDef(configs.config5(d, n_gas=5000))
+    snaps = A.snapshots(sd, sd.init_state(11), 11, n=n_snap, every=10)
+    sig, eps, q = (t.numpy() for t in A.ion_params(sd))
+    rng = np.random.default_rng(5)
+    Ft = A.vi_forward(snaps, *[a * (1 + rng.uniform(-0.1, 0.1, a.size)) for a in (sig, eps, q)])[1]
+    tf, ta = [], []
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        _, F = A.vi_forward(snaps, sig, eps, q)
+        t1 = time.perf_counter()
+        A.vi_adjoint(snaps, sig, eps, q, gF=2 * (F - Ft))
+        t2 = time.perf_counter()
+        tf.append(t1 - t0)
+        ta.append(t2 - t1)
+    pairs = int(snaps.vap_in.sum()) * snaps.n_ion
+    f, a = float(np.median(tf)), float(np.median(ta))
+    return {"workload": "%d FP64 frames: 150-atom AMBER chain + 200 packed water (750 atoms)" % n_snap,
+            "pairs_per_pass": pairs, "forward_ms": f * 1e3, "adjoint_ms": a * 1e3,
+            "forward_pairs_per_s": pairs / f, "adjoint_pairs_per_s": pairs / a,
+            "note": "wall time per C-ABI call including host-buffer copies"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -195,6 +229,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-cluster", action="store_true", help="skip the 12,001-atom cluster measurement")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable vi term measurement")
     ap.add_argument("--copy-check", action="store_true",
                     help="also run a 256 MiB device-to-device copy (calibrates HBM byte counters under rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -276,6 +311,7 @@ def main():
 
         cpu = None if args.no_cpu else cpu_baseline(reps, args.cpu_seconds)
         cl = None if args.no_cluster else cluster_bench(os.path.join(d, "cluster"), with_cpu=not args.no_cpu)
+        adj = None if args.no_adjoint else adjoint_bench(os.path.join(d, "fm"))
         if args.copy_check:
             copy_check(256 << 20)
 
@@ -300,6 +336,7 @@ def main():
         "cpu_baseline": cpu,
         "autotune": tuned,
         "cluster": cl,
+        "adjoint": adj,
     }
     if cpu:
         out["vs_cpu"] = value / cpu["value"]

@@ -180,6 +180,34 @@ int mdn_set_batch_shape(mdn_ctx *c, int32_t threads, int32_t lds_cap_atoms, int3
  * launch.  No reference counterpart. */
 int mdn_get_batch_info(mdn_ctx *c, int32_t *lds_cap_atoms, int32_t *lds_bytes, int32_t *heavy);
 
+/* ---- differentiable vi term (fitting ion-vapor parameters) ----
+ * PotentialVaporIon::compute (src/potential/VaporIon.cpp:5-43) over S
+ * snapshots with the ion's per-atom sigma, eps, q as inputs (mixing of
+ * setCrossPotential.cpp:13-20), FP64 on the device.  Host pointers (copied
+ * in and out): x [S][n_ion+n_vap][3] (ion atoms first), w [S][n_ion+n_vap]
+ * blend weights, vap_in [S][n_vap] 1 where the vapor atom's molecule is in
+ * the domain (region flag; NULL = all), E [S], F [S][n_ion+n_vap][3].
+ * mdn_vi_adjoint writes
+ * dL/dsigma, dL/deps, dL/dq of the ion atoms given dL/dE (gE, may be NULL)
+ * and dL/dF (gF, may be NULL), by a hand-written reverse pass.  No reference
+ * counterpart (the reference has no gradients): the fitting path of the
+ * force-matching workload. */
+int mdn_vi_forward(const double *x, const double *w, int32_t S, int32_t n_ion,
+                   int32_t n_vap, const double *ion_sigma,
+                   const double *ion_eps, const double *ion_q,
+                   const double *vap_sigma, const double *vap_eps,
+                   const double *vap_q, const uint8_t *vap_in, double *E,
+                   double *F);
+int mdn_vi_adjoint(const double *x, const double *w, int32_t S, int32_t n_ion,
+                   int32_t n_vap, const double *ion_sigma,
+                   const double *ion_eps, const double *ion_q,
+                   const double *vap_sigma, const double *vap_eps,
+                   const double *vap_q, const uint8_t *vap_in,
+                   const double *gE, const double *gF,
+                   double *g_sigma, double *g_eps, double *g_q);
+/* Coulomb constant of the reference (constants.hpp:50), kcal A/(mol e^2) */
+double mdn_qqrd2e(void);
+
 /* counters: in-domain atom-steps since commit, rebuilds, max SHAKE
  * iterations, SHAKE calls that failed (cap reached or diverged: that molecule
  * then stays unconstrained for the step).  Returns MDN_E_CAPACITY when a

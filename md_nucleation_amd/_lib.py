@@ -80,6 +80,9 @@ _SIGS = {
     "mdn_get_counters": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_int64), C.POINTER(C.c_int64), _ip,
                                    C.POINTER(C.c_int64)]),
     "mdn_get_batch_info": (C.c_int, [C.c_void_p, _ip, _ip, _ip]),
+    "mdn_vi_forward": (C.c_int, [_dp, _dp, C.c_int32, C.c_int32, C.c_int32] + [_dp] * 6 + [C.c_void_p, _dp, _dp]),
+    "mdn_vi_adjoint": (C.c_int, [_dp, _dp, C.c_int32, C.c_int32, C.c_int32] + [_dp] * 6 + [C.c_void_p] + [_dp] * 5),
+    "mdn_qqrd2e": (C.c_double, []),
     "mdn_sync": (C.c_int, [C.c_void_p]),
     "mdn_last_kernel_ms": (C.c_int, [C.c_void_p, _dp]),
 }

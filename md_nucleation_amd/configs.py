@@ -284,6 +284,16 @@ def config3(d, n_gas=100000, n_vap=2000, density=0.79, radius=45.0):
                      cluster=(radius, density)).write(os.path.join(d, "c3.cond"))
 
 
+def config5(d, n_gas=100000, n_vap=200, seed=1, radius=40.0, density=1.0):
+    """Force-matching system: the 150-atom AMBER chain with 200 water packed
+    around it (lattice sites nearest the chain at ``density`` g/cm3, all in
+    the effective domain, 750 interacting ion + vapor atoms) + N2."""
+    m = molecule_files(d)
+    pep = write_peptide(os.path.join(d, "peptide_%d.txt" % seed), seed)
+    return Condition(pep, m["water"], n_vap, "N2", n_gas, total_steps=10000, ion_potential="AMBER",
+                     cluster=(radius, density)).write(os.path.join(d, "c5_%d.cond" % seed))
+
+
 def config4_replica(d, r, rng, n_gas=5000):
     """One replica of the batched workload: ion in {Na+, Cl-, chain},
     vapor U[10,400] methanol or water, 5,000 N2."""
