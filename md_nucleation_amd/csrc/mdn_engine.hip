@@ -364,78 +364,98 @@ struct SStoreT {
 // lanes), which visits each of its pairs once.  A per-lane mask selects the
 // pairs the reference evaluates: ion-ion by the AMBER exclusion bitmask,
 // ion-vapor by vi, vapor-vapor of distinct molecules by vv.
+//
+// Work items are runs of L consecutive k of one block pair (L = 32, or 16 /
+// 8 when there are fewer items than waves), dealt round-robin to the waves,
+// so the waves finish together.  The LDS atom arrays are padded to whole
+// 64-atom blocks with finite values (batch_load), so every lane loads and
+// adds unconditionally and the loop body has no branches: masked pairs carry
+// a zero force.
 template <typename Rl, bool EF>
 __device__ inline void dense_n3(const DevReplica &R, const BatchLds<Rl> &B, int nh, int ni, bool am, bool vi, bool vv,
                                 int t, int nt, double *e) {
   const Rl qq = (Rl)R.qqrd2e;
   const int lane = t & 63, wave = t >> 6, nw = nt >> 6;
-  const int NB = (nh + 63) >> 6, units = NB * (NB + 1) / 2, W = R.amber_words;
+  const int NB = (nh + 63) >> 6, n_off = NB * (NB - 1) / 2;
+  int L = 32;
+  while (L > 8 && NB * (32 / L) + n_off * (64 / L) < nw) L >>= 1;
+  const int W = R.amber_words, wmax = W > 0 ? W - 1 : 0;
   const double *__restrict__ px = B.x;
   const typename Pack4<Rl>::T *__restrict__ pk = B.pk;
   const Rl *__restrict__ wr = B.wr;
-  for (int u = wave; u < units; u += nw) {
-    int bi = 0, rem = u;
-    while (rem >= NB - bi) { rem -= NB - bi; bi++; }
-    const int bj = bi + rem;
-    const bool diag = bi == bj;
+  double *__restrict__ pf = B.f;
+  double e_ion = 0, e_vi = 0, e_vv = 0;
+  int item = 0;
+  for (int bi = 0; bi < NB; bi++) {
     const int i = 64 * bi + lane;
-    const bool i_ok = i < nh;
-    const int ic = i_ok ? i : 0;
-    const double xi0 = px[3 * ic], xi1 = px[3 * ic + 1], xi2 = px[3 * ic + 2];
-    const typename Pack4<Rl>::T pi = pk[ic];
-    const Rl qi = qq * pi.x, sei = pi.y, sgi = pi.z, mi = pi.w, wi = wr[ic];
-    const bool ion_i = ic < ni;
-    const uint32_t *arow = B.allow + (size_t)(ion_i ? ic : 0) * W;
-    double fx = 0, fy = 0, fz = 0, e_ion = 0, e_vi = 0, e_vv = 0;
-    const int kbeg = diag ? 1 : 0, kend = diag ? 33 : 64;
+    const bool i_ok = i < nh, ion_i = i < ni;
+    for (int bj = bi; bj < NB; bj++) {
+      const bool diag = bi == bj;
+      const int np = (diag ? 32 : 64) / L;
+      for (int part = 0; part < np; part++, item++) {
+        if (item % nw != wave) continue;
+        const double xi0 = px[3 * i], xi1 = px[3 * i + 1], xi2 = px[3 * i + 2];
+        const typename Pack4<Rl>::T pi = pk[i];
+        const Rl qi = qq * pi.x, sei = pi.y, sgi = pi.z, mi = pi.w, wi = wr[i];
+        const uint32_t *arow = B.allow + (size_t)(ion_i ? i : 0) * W;
+        double fx = 0, fy = 0, fz = 0;
+        const int kb = (diag ? 1 : 0) + part * L;
+        // pair selection as integer masks (no short-circuit control flow)
+        const uint32_t ok_i = i_ok ? 1u : 0u, io_i = ion_i ? 1u : 0u;
+        const uint32_t amu = am ? 1u : 0u, viu = vi ? 1u : 0u, vvu = vv ? 1u : 0u;
+        const uint32_t half_ok = (!diag || lane < 32) ? 1u : 0u;
 #pragma unroll 4
-    for (int k = kbeg; k < kend; k++) {
-      const int j = 64 * bj + ((lane + k) & 63);
-      const bool valid = i_ok && j < nh && !(diag && k == 32 && lane >= 32);
-      const int jc = valid ? j : 0;
-      const typename Pack4<Rl>::T pj = pk[jc];
-      const double dx = xi0 - px[3 * jc], dy = xi1 - px[3 * jc + 1], dz = xi2 - px[3 * jc + 2];
-      const Rl wj = wr[jc];
-      const bool ion_j = jc < ni;
-      const bool both = ion_i && ion_j, cross = ion_i != ion_j;
-      const uint32_t aw = both && am ? arow[jc >> 5] : 0u;
-      const bool on = valid && (both ? ((aw >> (jc & 31)) & 1u) != 0 : (cross ? vi : (vv && pj.w != mi)));
-      const Rl rx = (Rl)dx, ry = (Rl)dy, rz = (Rl)dz;
-      const Rl rsq = on ? rx * rx + ry * ry + rz * rz : (Rl)1;
-      const Rl ee = sei * pj.y;
-      const Rl sg = (Rl)0.5 * (sgi + pj.z);
-      const Rl s2 = sg * sg, s6 = s2 * s2 * s2;
-      const Rl c1 = (Rl)24 * ee * s6, c0 = (Rl)2 * c1 * s6;
-      const Rl w = both ? (Rl)1 : wi * wj;
-      Rl r2inv, rinv;
-      inv_r<Rl>(rsq, r2inv, rinv);
-      const Rl r6inv = r2inv * r2inv * r2inv;
-      const Rl fc = qi * pj.x * rinv;
-      const Rl fp = on ? (r6inv * (c0 * r6inv - c1) + fc) * r2inv * w : (Rl)0;
-      const double gx = (double)(fp * rx), gy = (double)(fp * ry), gz = (double)(fp * rz);
-      fx += gx;
-      fy += gy;
-      fz += gz;
-      if (on) {
-        atomicAdd(&B.f[3 * jc], -gx);
-        atomicAdd(&B.f[3 * jc + 1], -gy);
-        atomicAdd(&B.f[3 * jc + 2], -gz);
-      }
-      if (EF) {
-        const double ep =
-            on ? (double)(r6inv * (c0 * (Rl)(1.0 / 12.0) * r6inv - c1 * (Rl)(1.0 / 6.0))) + (double)fc : 0.0;
-        e_ion += both ? ep : 0.0;
-        e_vi += cross ? ep : 0.0;
-        e_vv += (!both && !cross) ? ep : 0.0;
+        for (int k = kb; k < kb + L; k++) {
+          const int j = 64 * bj + ((lane + k) & 63);
+          const typename Pack4<Rl>::T pj = pk[j];
+          const double dx = xi0 - px[3 * j], dy = xi1 - px[3 * j + 1], dz = xi2 - px[3 * j + 2];
+          const Rl wj = wr[j];
+          const uint32_t abit = (arow[min(j >> 5, wmax)] >> (j & 31)) & 1u;
+          const uint32_t io_j = j < ni ? 1u : 0u;
+          const uint32_t both = io_i & io_j, cross = io_i ^ io_j;
+          const uint32_t diff_mol = pj.w != mi ? 1u : 0u;
+          const uint32_t pass = (both & amu & abit) | (cross & viu) | ((1u ^ (both | cross)) & vvu & diff_mol);
+          const uint32_t valid = ok_i & (j < nh ? 1u : 0u) & (k != 32 ? 1u : half_ok);
+          const uint32_t on = valid & pass;
+          const Rl m = on ? (Rl)1 : (Rl)0;
+          const Rl rx = (Rl)dx, ry = (Rl)dy, rz = (Rl)dz;
+          const Rl r2 = rx * rx + ry * ry + rz * rz;
+          const Rl rsq = on ? r2 : (Rl)1;
+          const Rl ee = sei * pj.y;
+          const Rl sg = (Rl)0.5 * (sgi + pj.z);
+          const Rl s2 = sg * sg, s6 = s2 * s2 * s2;
+          const Rl c1 = (Rl)24 * ee * s6, c0 = (Rl)2 * c1 * s6;
+          const Rl w = both ? (Rl)1 : wi * wj;
+          Rl r2inv, rinv;
+          inv_r<Rl>(rsq, r2inv, rinv);
+          const Rl r6inv = r2inv * r2inv * r2inv;
+          const Rl fc = qi * pj.x * rinv;
+          // minus the pair force factor (zero for masked pairs): F_j gets
+          // +g, F_i gets -g
+          const Rl fn = -((r6inv * (c0 * r6inv - c1) + fc) * r2inv * (w * m));
+          const double gx = (double)(fn * rx), gy = (double)(fn * ry), gz = (double)(fn * rz);
+          fx -= gx;
+          fy -= gy;
+          fz -= gz;
+          atomicAdd(&pf[3 * j], gx);
+          atomicAdd(&pf[3 * j + 1], gy);
+          atomicAdd(&pf[3 * j + 2], gz);
+          if (EF) {
+            const double ep = (double)((r6inv * (c0 * (Rl)(1.0 / 12.0) * r6inv - c1 * (Rl)(1.0 / 6.0)) + fc) * m);
+            e_ion += both ? ep : 0.0;
+            e_vi += cross ? ep : 0.0;
+            e_vv += (both | cross) ? 0.0 : ep;
+          }
+        }
+        if (i_ok) {
+          atomicAdd(&pf[3 * i], fx);
+          atomicAdd(&pf[3 * i + 1], fy);
+          atomicAdd(&pf[3 * i + 2], fz);
+        }
       }
     }
-    if (i_ok) {
-      atomicAdd(&B.f[3 * i], fx);
-      atomicAdd(&B.f[3 * i + 1], fy);
-      atomicAdd(&B.f[3 * i + 2], fz);
-    }
-    if (EF) { e[U_ION] += e_ion; e[U_VI] += e_vi; e[U_VV] += e_vv; }
   }
+  if (EF) { e[U_ION] += e_ion; e[U_VI] += e_vi; e[U_VV] += e_vv; }
 }
 
 template <typename Rl>
@@ -499,6 +519,12 @@ __device__ inline void batch_load(const DevReplica &R, const LView &L, const Bat
     B.pk[a].z = (Rl)S.sg[idx];
     B.pk[a].w = (Rl)m;
     B.wr[a] = (Rl)R.atom_w[g];
+  }
+  // finite padding to the end of the last 64-atom block (read by dense_n3)
+  for (int a = L.n_in + threadIdx.x; a < ((L.n_in + 63) & ~63); a += blockDim.x) {
+    for (int q = 0; q < 3; q++) { B.x[3 * a + q] = 0.0; B.f[3 * a + q] = 0.0; }
+    B.pk[a].x = (Rl)0; B.pk[a].y = (Rl)0; B.pk[a].z = (Rl)0; B.pk[a].w = (Rl)-1;
+    B.wr[a] = (Rl)0;
   }
 }
 
@@ -1279,18 +1305,21 @@ static BatchCaps batch_caps(const mdn_ctx *c, const std::vector<Dyn> &dyn, const
     kcap = std::max(kcap, d.sp[0].natoms + d.sp[1].natoms + d.sp[2].natoms);
     if (d.flags & MDN_ION_AMBER) acap = std::max(acap, d.sp[0].natoms * ((d.sp[0].natoms + 31) / 32));
   }
-  auto r32 = [](int v) { return (v + 31) / 32 * 32; };
-  int lo_h = std::min(most_h, r32(need_h + 32)), hi_h = std::min(most_h, r32(need_h * 3 / 2 + 32));
-  int lo = std::min(most, r32(need + need / 8 + 64)), hi = std::min(most, r32(need * 3 / 2 + 64));
+  // the atom capacity is a whole number of 64-atom blocks (dense_n3 reads
+  // whole blocks)
+  auto r64 = [](int v) { return (v + 63) / 64 * 64; };
+  most = r64(most);
+  int lo_h = std::min(most_h, r64(need_h + 32)), hi_h = std::min(most_h, r64(need_h * 3 / 2 + 32));
+  int lo = std::min(most, r64(need + need / 8 + 64)), hi = std::min(most, r64(need * 3 / 2 + 64));
   BatchCaps b;
   b.kcap = kcap; b.acap = acap;
   if (c->batch_cap > 0) {
-    b.cap = c->batch_cap;
+    b.cap = r64(c->batch_cap);
     b.cap_h = std::min(b.cap, hi_h);
   } else {
     b.cap = hi; b.cap_h = hi_h;
     while (batch_lds_bytes(b.cap, b.cap_h, kcap, acap, rl) > budget && (b.cap > lo || b.cap_h > lo_h)) {
-      if (b.cap > lo) b.cap -= 32;
+      if (b.cap > lo) b.cap -= 64;
       if (b.cap_h > lo_h) b.cap_h -= 32;
     }
     b.cap_h = std::min(b.cap_h, b.cap);
