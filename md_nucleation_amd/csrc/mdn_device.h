@@ -21,6 +21,7 @@ constexpr double kRcenter = kRinter - kOverlap * 0.5;
 constexpr double kRi2 = kRinter * kRinter;
 constexpr double kRI2 = (kRinter + kOverlap) * (kRinter + kOverlap);
 constexpr double kPI = 3.14159265358979323846;
+constexpr double kSqrt24 = 4.89897948556635619639;  // sqrt(24)
 constexpr double kKickUnit = 4.184e-4;
 
 // interaction bits (same values as mdn.h)

@@ -279,6 +279,15 @@ __global__ void __launch_bounds__(kVVTile) k_vv_tiled(const DevReplica *__restri
 // positions only for the ion + vapor atoms (the only constrained ones).
 // per-atom pair parameters packed for one LDS read: charge, sqrt(eps),
 // sigma, local vapor molecule id (-1: ion or gas) as a value
+// every lane takes the value of lane (l + 1) mod 64 (DPP wave_rol:1; all 64
+// lanes must be active)
+__device__ inline double wave_rol1(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_mov_dpp((int)b, 0x134, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), 0x134, 0xf, 0xf, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
 template <typename Rl> struct Pack4;
 template <> struct Pack4<float> { typedef float4 T; };
 template <> struct Pack4<double> { typedef double4 T; };
@@ -342,13 +351,14 @@ struct SStoreT {
   }
   __device__ double w(H a) const { return (double)wr[a]; }
   __device__ double q(H a) const { return (double)pk[a].x; }
+  // pk.y = sqrt(24 eps), pk.z = sigma / 2 (batch_load)
   __device__ int type(H a) const { return ptype[a]; }
   template <typename R2>
   __device__ void coeff(const DevReplica &, H a, H b, R2 &c0, R2 &c1) const {
-    R2 e = (R2)pk[a].y * (R2)pk[b].y;
-    R2 sg = (R2)0.5 * ((R2)pk[a].z + (R2)pk[b].z);
+    R2 e24 = (R2)pk[a].y * (R2)pk[b].y;
+    R2 sg = (R2)pk[a].z + (R2)pk[b].z;
     R2 s2 = sg * sg, s6 = s2 * s2 * s2;
-    c1 = (R2)24 * e * s6;
+    c1 = e24 * s6;
     c0 = (R2)2 * c1 * s6;
   }
 };
@@ -358,10 +368,13 @@ struct SStoreT {
 // + in-domain vapor atoms, each unordered pair evaluated once (Newton's third
 // law).  The atoms are cut into 64-atom blocks; a wave takes a block pair
 // (bi <= bj) with lane l owning atom i = 64 bi + l and, at step k, partner
-// j = 64 bj + (l + k) mod 64, so the 64 lanes always touch 64 different j:
-// F_i accumulates in FP64 registers and F_j goes to LDS with conflict-free
-// ds_add_f64.  A diagonal block pair runs k = 1..32 (k = 32 on half the
-// lanes), which visits each of its pairs once.  A per-lane mask selects the
+// j = 64 bj + (l + k) mod 64, so the 64 lanes always touch 64 different j.
+// F_i accumulates in FP64 registers.  F_j accumulates in FP64 registers that
+// travel with their atom: the partner of lane l at step k + 1 is the partner
+// of lane l + 1 at step k, so after each step every lane takes the
+// accumulator of lane l + 1 (a DPP wave rotate), and each accumulator is
+// added to LDS once at the end of a work item.  A diagonal block pair runs
+// k = 1..32 (k = 32 on half the lanes), which visits each of its pairs once.  A per-lane mask selects the
 // pairs the reference evaluates: ion-ion by the AMBER exclusion bitmask,
 // ion-vapor by vi, vapor-vapor of distinct molecules by vv.
 //
@@ -375,11 +388,16 @@ template <typename Rl, bool EF>
 __device__ inline void dense_n3(const DevReplica &R, const BatchLds<Rl> &B, int nh, int ni, bool am, bool vi, bool vv,
                                 int t, int nt, double *e) {
   const Rl qq = (Rl)R.qqrd2e;
-  const int lane = t & 63, wave = t >> 6, nw = nt >> 6;
+  // the wave index and the counts are uniform within a wave: say so, so that
+  // the item loops and their trip counts stay scalar
+  const int lane = t & 63, wave = __builtin_amdgcn_readfirstlane(t >> 6);
+  const int nw = __builtin_amdgcn_readfirstlane(nt >> 6);
+  nh = __builtin_amdgcn_readfirstlane(nh);
+  ni = __builtin_amdgcn_readfirstlane(ni);
   const int NB = (nh + 63) >> 6, n_off = NB * (NB - 1) / 2;
   int L = 32;
   while (L > 8 && NB * (32 / L) + n_off * (64 / L) < nw) L >>= 1;
-  const int W = R.amber_words, wmax = W > 0 ? W - 1 : 0;
+  const int W = __builtin_amdgcn_readfirstlane(R.amber_words), wmax = W > 0 ? W - 1 : 0;
   const double *__restrict__ px = B.x;
   const typename Pack4<Rl>::T *__restrict__ pk = B.pk;
   const Rl *__restrict__ wr = B.wr;
@@ -404,13 +422,30 @@ __device__ inline void dense_n3(const DevReplica &R, const BatchLds<Rl> &B, int 
         const uint32_t ok_i = i_ok ? 1u : 0u, io_i = ion_i ? 1u : 0u;
         const uint32_t amu = am ? 1u : 0u, viu = vi ? 1u : 0u, vvu = vv ? 1u : 0u;
         const uint32_t half_ok = (!diag || lane < 32) ? 1u : 0u;
-#pragma unroll 4
+        // partner data is loaded one step ahead; the prefetch after the
+        // last step reads an in-range partner, whose index then names the
+        // atom of each lane's F_j accumulator
+        double ajx = 0, ajy = 0, ajz = 0;
+        int jn = 64 * bj + ((lane + kb) & 63);
+        typename Pack4<Rl>::T pn = pk[jn];
+        double xn0 = px[3 * jn], xn1 = px[3 * jn + 1], xn2 = px[3 * jn + 2];
+        Rl wn = wr[jn];
+        uint32_t an = arow[min(jn >> 5, wmax)];
+#pragma unroll 2
         for (int k = kb; k < kb + L; k++) {
-          const int j = 64 * bj + ((lane + k) & 63);
-          const typename Pack4<Rl>::T pj = pk[j];
-          const double dx = xi0 - px[3 * j], dy = xi1 - px[3 * j + 1], dz = xi2 - px[3 * j + 2];
-          const Rl wj = wr[j];
-          const uint32_t abit = (arow[min(j >> 5, wmax)] >> (j & 31)) & 1u;
+          const int j = jn;
+          const typename Pack4<Rl>::T pj = pn;
+          const double dx = xi0 - xn0, dy = xi1 - xn1, dz = xi2 - xn2;
+          const Rl wj = wn;
+          const uint32_t abit = (an >> (j & 31)) & 1u;
+          jn = 64 * bj + ((lane + k + 1) & 63);
+          pn = pk[jn];
+          {
+            const double *xq = px + __umul24((unsigned)jn, 3u);
+            xn0 = xq[0]; xn1 = xq[1]; xn2 = xq[2];
+          }
+          wn = wr[jn];
+          an = arow[min(jn >> 5, wmax)];
           const uint32_t io_j = j < ni ? 1u : 0u;
           const uint32_t both = io_i & io_j, cross = io_i ^ io_j;
           const uint32_t diff_mol = pj.w != mi ? 1u : 0u;
@@ -421,32 +456,39 @@ __device__ inline void dense_n3(const DevReplica &R, const BatchLds<Rl> &B, int 
           const Rl rx = (Rl)dx, ry = (Rl)dy, rz = (Rl)dz;
           const Rl r2 = rx * rx + ry * ry + rz * rz;
           const Rl rsq = on ? r2 : (Rl)1;
-          const Rl ee = sei * pj.y;
-          const Rl sg = (Rl)0.5 * (sgi + pj.z);
-          const Rl s2 = sg * sg, s6 = s2 * s2 * s2;
-          const Rl c1 = (Rl)24 * ee * s6, c0 = (Rl)2 * c1 * s6;
+          // 24 eps_ij and sigma_ij from the packed per-atom sqrt(24 eps)
+          // and sigma / 2; with u3 = (sigma_ij / r)^6 the reference's
+          // r^-6 (c0 r^-6 - c1) is 24 eps u3 (2 u3 - 1)
+          const Rl e24 = sei * pj.y;
+          const Rl sg = sgi + pj.z;
           const Rl w = both ? (Rl)1 : wi * wj;
           Rl r2inv, rinv;
           inv_r<Rl>(rsq, r2inv, rinv);
-          const Rl r6inv = r2inv * r2inv * r2inv;
+          const Rl u = sg * sg * r2inv, u3 = u * u * u;
+          const Rl flj = e24 * u3 * ((Rl)2 * u3 - (Rl)1);
           const Rl fc = qi * pj.x * rinv;
           // minus the pair force factor (zero for masked pairs): F_j gets
           // +g, F_i gets -g
-          const Rl fn = -((r6inv * (c0 * r6inv - c1) + fc) * r2inv * (w * m));
+          const Rl fn = -((flj + fc) * r2inv * (w * m));
           const double gx = (double)(fn * rx), gy = (double)(fn * ry), gz = (double)(fn * rz);
           fx -= gx;
           fy -= gy;
           fz -= gz;
-          atomicAdd(&pf[3 * j], gx);
-          atomicAdd(&pf[3 * j + 1], gy);
-          atomicAdd(&pf[3 * j + 2], gz);
+          ajx = wave_rol1(ajx + gx);
+          ajy = wave_rol1(ajy + gy);
+          ajz = wave_rol1(ajz + gz);
           if (EF) {
-            const double ep = (double)((r6inv * (c0 * (Rl)(1.0 / 12.0) * r6inv - c1 * (Rl)(1.0 / 6.0)) + fc) * m);
+            const double ep = (double)((e24 * (Rl)(1.0 / 6.0) * u3 * (u3 - (Rl)1) + fc) * m);
             e_ion += both ? ep : 0.0;
             e_vi += cross ? ep : 0.0;
             e_vv += (both | cross) ? 0.0 : ep;
           }
         }
+        // 64 distinct atoms of block bj (zero sums for padding and masked
+        // partners)
+        atomicAdd(&pf[3 * jn], ajx);
+        atomicAdd(&pf[3 * jn + 1], ajy);
+        atomicAdd(&pf[3 * jn + 2], ajz);
         if (i_ok) {
           atomicAdd(&pf[3 * i], fx);
           atomicAdd(&pf[3 * i + 1], fy);
@@ -515,8 +557,8 @@ __device__ inline void batch_load(const DevReplica &R, const LView &L, const Bat
     B.kidx[a] = (int16_t)kk;
     const int m = sp == 2 ? (a - L.vbase) / L.na_v : -1;
     B.pk[a].x = (Rl)S.charge[idx];
-    B.pk[a].y = (Rl)S.se[idx];
-    B.pk[a].z = (Rl)S.sg[idx];
+    B.pk[a].y = (Rl)(S.se[idx] * kSqrt24);
+    B.pk[a].z = (Rl)(0.5 * S.sg[idx]);
     B.pk[a].w = (Rl)m;
     B.wr[a] = (Rl)R.atom_w[g];
   }

@@ -535,7 +535,9 @@ __device__ int shake_molecule_generic(const DevReplica &R, const DevSpecies &Sp,
         if (d0 * d0 > tolerance) converge = false;
       }
       double mred_inv = Sp.invmass[i] + Sp.invmass[j];
-      double ramk = R.shake == 2 ? (dk * dk - rn2) / 2 * mred_inv : (dk * dk - rn2) / 2 / mred_inv;
+      // mode 2: shake.cpp:67 multiplies by the inverse reduced mass; mode 1
+      // divides by it (textbook SHAKE), through the precomputed 0.5 / mred_inv
+      double ramk = R.shake == 2 ? (dk * dk - rn2) / 2 * mred_inv : (dk * dk - rn2) * (0.5 / mred_inv);
       if (dot > 1e-10) ramk /= dot;
       else ramk = 0.5 * rho_zero + ramk / (rk2 * rho_zero) * mred_inv;
       double ci = ramk * Sp.invmass[i], cj = ramk * Sp.invmass[j];
@@ -560,7 +562,7 @@ __device__ int shake_molecule(const DevReplica &R, const DevSpecies &Sp, XA x, R
   if (nr > kMaxRigid) return shake_molecule_generic(R, Sp, x, r, max_iter);
   const double tolerance = 1e-20, rho_zero = 0.01;
   int ci_[kMaxRigid], cj_[kMaxRigid];
-  double rk[kMaxRigid][3], dk2[kMaxRigid], mi[kMaxRigid], mj[kMaxRigid];
+  double rk[kMaxRigid][3], dk2[kMaxRigid], mi[kMaxRigid], mj[kMaxRigid], hm[kMaxRigid];
 #pragma unroll
   for (int k = 0; k < kMaxRigid; k++) {
     if (k >= nr) break;
@@ -569,6 +571,9 @@ __device__ int shake_molecule(const DevReplica &R, const DevSpecies &Sp, XA x, R
     ci_[k] = i; cj_[k] = j;
     dk2[k] = dk * dk;
     mi[k] = Sp.invmass[i]; mj[k] = Sp.invmass[j];
+    // the factor multiplying (d^2 - r'^2): mode 2 (shake.cpp:67) 0.5 mred_inv,
+    // mode 1 (textbook) 0.5 / mred_inv
+    hm[k] = R.shake == 2 ? 0.5 * (mi[k] + mj[k]) : 0.5 / (mi[k] + mj[k]);
     for (int q = 0; q < 3; q++) rk[k][q] = r(i, q) - r(j, q);
   }
   int it = 0;
@@ -591,8 +596,9 @@ __device__ int shake_molecule(const DevReplica &R, const DevSpecies &Sp, XA x, R
       }
       double mred_inv = mi[k] + mj[k];
       // mode 2: shake.cpp:67 multiplies by the inverse reduced mass; mode 1
-      // divides by it (textbook SHAKE); both reach the same fixed point
-      double ramk = R.shake == 2 ? (dk2[k] - rn2) / 2 * mred_inv : (dk2[k] - rn2) / 2 / mred_inv;
+      // divides by it (textbook SHAKE); both reach the same fixed point.
+      // (x / 2 * m equals x * (0.5 m) exactly, so mode 2 keeps its rounding)
+      double ramk = (dk2[k] - rn2) * hm[k];
       if (dot > 1e-10) ramk /= dot;
       else {
         double rk2 = rk[k][0] * rk[k][0] + rk[k][1] * rk[k][1] + rk[k][2] * rk[k][2];

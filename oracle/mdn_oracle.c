@@ -257,8 +257,9 @@ int orc_shake_molecule(orc_ctx *c, double *x, const double *r, int sp) {
       }
       double mred_inv = 1 / m[i] + 1 / m[j];
       /* mode 2 restates shake.cpp:67 (multiplies by the inverse reduced
-       * mass); mode 1 divides by it (textbook SHAKE), same fixed point */
-      double ramk = s->shake == 2 ? (dk * dk - _rk2) / 2 * mred_inv : (dk * dk - _rk2) / 2 / mred_inv;
+       * mass); mode 1 divides by it (textbook SHAKE, as a product with
+       * 0.5 / mred_inv), same fixed point */
+      double ramk = s->shake == 2 ? (dk * dk - _rk2) / 2 * mred_inv : (dk * dk - _rk2) * (0.5 / mred_inv);
       if (dot > 1e-10) ramk /= dot;
       else ramk = 0.5 * rho_zero + ramk / (rk2 * rho_zero) * mred_inv;
       double coeffi = ramk / m[i], coeffj = ramk / m[j];
