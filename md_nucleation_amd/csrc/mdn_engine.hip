@@ -296,7 +296,10 @@ template <typename Rl>
 struct BatchLds {
   double *x, *v, *f, *xo;
   double *kct;      // [kcap] kick factor 0.5 dt 4.184e-4 / m per species atom (ion | vapor | gas)
-  typename Pack4<Rl>::T *pk;  // packed (q, sqrt eps, sigma, molecule id)
+  // packed (q, sqrt(24 eps), sigma / 2, vapor molecule id or -1): pk per
+  // atom of the ion + vapor range [cap_h], kpk per species atom [kcap]
+  // (what gas atoms use)
+  typename Pack4<Rl>::T *pk, *kpk;
   Rl *wr;           // blend weight (pair precision)
   int16_t *kidx;    // per atom: index into kct
   uint32_t *allow;  // [acap] AMBER non-excluded ion-pair bitmask (rows of amber_words)
@@ -317,7 +320,9 @@ __device__ inline BatchLds<Rl> carve(char *base, int cap, int cap_h, int kcap, i
   B.red = d; d += (U_N + 6) * 16;
   B.sdbl = d; d += 16;
   B.pk = reinterpret_cast<typename Pack4<Rl>::T *>(d);
-  d += (4 * sizeof(Rl) * (size_t)cap) / sizeof(double);
+  d += (4 * sizeof(Rl) * (size_t)cap_h) / sizeof(double);
+  B.kpk = reinterpret_cast<typename Pack4<Rl>::T *>(d);
+  d += (4 * sizeof(Rl) * (size_t)kcap + sizeof(double) - 1) / sizeof(double);
   Rl *r = reinterpret_cast<Rl *>(d);
   B.wr = r; r += cap;
   int *ip = reinterpret_cast<int *>(r);
@@ -330,7 +335,8 @@ __device__ inline BatchLds<Rl> carve(char *base, int cap, int cap_h, int kcap, i
 
 static size_t batch_lds_bytes(int cap, int cap_h, int kcap, int acap, size_t rl) {
   return sizeof(double) * ((size_t)9 * cap + 3 * (size_t)cap_h + kcap + (U_N + 6) * 16 + 16) +
-         rl * 5 * (size_t)cap + sizeof(int) * (64 + (size_t)acap) + sizeof(int16_t) * (size_t)cap;
+         ((4 * rl * (size_t)kcap + sizeof(double) - 1) / sizeof(double)) * sizeof(double) +
+         rl * (4 * (size_t)cap_h + (size_t)cap) + sizeof(int) * (64 + (size_t)acap) + sizeof(int16_t) * (size_t)cap;
 }
 
 // LDS store with Lorentz-Berthelot mixing from per-atom parameters
@@ -340,23 +346,28 @@ template <typename Rl>
 struct SStoreT {
   LView L;
   double *px, *pf;
-  const typename Pack4<Rl>::T *pk;
+  const typename Pack4<Rl>::T *pk, *kpk;
+  const int16_t *kidx;
   const Rl *wr;
   const int *ptype;  // the ion's atom types (global memory; BMH is ion-only)
   typedef int H;
   __device__ H h(int a) const { return a; }
+  // per-atom parameters: the ion + vapor range has its own, gas atoms read
+  // their species atom's
+  __device__ typename Pack4<Rl>::T par(H a) const { return a < L.gbase ? pk[a] : kpk[kidx[a]]; }
   __device__ void pos(H a, double p[3]) const { p[0] = px[3 * a]; p[1] = px[3 * a + 1]; p[2] = px[3 * a + 2]; }
   __device__ void fadd(H a, double fx, double fy, double fz) const {
     atomicAdd(&pf[3 * a], fx); atomicAdd(&pf[3 * a + 1], fy); atomicAdd(&pf[3 * a + 2], fz);
   }
   __device__ double w(H a) const { return (double)wr[a]; }
-  __device__ double q(H a) const { return (double)pk[a].x; }
-  // pk.y = sqrt(24 eps), pk.z = sigma / 2 (batch_load)
+  __device__ double q(H a) const { return (double)par(a).x; }
+  // .y = sqrt(24 eps), .z = sigma / 2 (batch_load)
   __device__ int type(H a) const { return ptype[a]; }
   template <typename R2>
   __device__ void coeff(const DevReplica &, H a, H b, R2 &c0, R2 &c1) const {
-    R2 e24 = (R2)pk[a].y * (R2)pk[b].y;
-    R2 sg = (R2)pk[a].z + (R2)pk[b].z;
+    const typename Pack4<Rl>::T pa = par(a), pb = par(b);
+    R2 e24 = (R2)pa.y * (R2)pb.y;
+    R2 sg = (R2)pa.z + (R2)pb.z;
     R2 s2 = sg * sg, s6 = s2 * s2 * s2;
     c1 = e24 * s6;
     c0 = (R2)2 * c1 * s6;
@@ -374,16 +385,72 @@ struct SStoreT {
 // of lane l + 1 at step k, so after each step every lane takes the
 // accumulator of lane l + 1 (a DPP wave rotate), and each accumulator is
 // added to LDS once at the end of a work item.  A diagonal block pair runs
-// k = 1..32 (k = 32 on half the lanes), which visits each of its pairs once.  A per-lane mask selects the
+// k = 1..32 (k = 32 on half the lanes), which visits each of its pairs once;
+// an off-diagonal one runs k and k + 32 together (two independent chains per
+// lane), k = 0..31.  A per-lane mask selects the
 // pairs the reference evaluates: ion-ion by the AMBER exclusion bitmask,
 // ion-vapor by vi, vapor-vapor of distinct molecules by vv.
 //
-// Work items are runs of L consecutive k of one block pair (L = 32, or 16 /
+// Work items are runs of L pairs per lane of one block pair (L = 32, or 16 /
 // 8 when there are fewer items than waves), dealt round-robin to the waves,
 // so the waves finish together.  The LDS atom arrays are padded to whole
 // 64-atom blocks with finite values (batch_load), so every lane loads and
 // adds unconditionally and the loop body has no branches: masked pairs carry
 // a zero force.
+
+// the work item's atom i (lane) and pair-selection flags
+template <typename Rl>
+struct DenseI {
+  double x0, x1, x2;
+  Rl qi, sei, sgi, mi, wi;
+  const uint32_t *arow;
+  uint32_t ok, ion, am, vi, vv;
+  int ni, nh, wmax;
+};
+
+// one pair (i, j): g = minus the pair force vector (F_j += g, F_i -= g),
+// zero for masked pairs; energies by reference label
+template <typename Rl, bool EF>
+__device__ inline void dense_pair(const BatchLds<Rl> &B, const DenseI<Rl> &c, int j, uint32_t valid_k, double &gx,
+                                  double &gy, double &gz, double &e_ion, double &e_vi, double &e_vv) {
+  const typename Pack4<Rl>::T pj = B.pk[j];
+  const double *xq = B.x + __umul24((unsigned)j, 3u);
+  const double dx = c.x0 - xq[0], dy = c.x1 - xq[1], dz = c.x2 - xq[2];
+  const Rl wj = B.wr[j];
+  const uint32_t abit = (c.arow[min(j >> 5, c.wmax)] >> (j & 31)) & 1u;
+  // pair selection as integer masks (no short-circuit control flow)
+  const uint32_t io_j = j < c.ni ? 1u : 0u;
+  const uint32_t both = c.ion & io_j, cross = c.ion ^ io_j;
+  const uint32_t diff_mol = pj.w != c.mi ? 1u : 0u;
+  const uint32_t pass = (both & c.am & abit) | (cross & c.vi) | ((1u ^ (both | cross)) & c.vv & diff_mol);
+  const uint32_t on = c.ok & (j < c.nh ? 1u : 0u) & valid_k & pass;
+  const Rl m = on ? (Rl)1 : (Rl)0;
+  const Rl rx = (Rl)dx, ry = (Rl)dy, rz = (Rl)dz;
+  const Rl r2 = rx * rx + ry * ry + rz * rz;
+  const Rl rsq = on ? r2 : (Rl)1;
+  // 24 eps_ij and sigma_ij from the packed per-atom sqrt(24 eps) and
+  // sigma / 2; with u3 = (sigma_ij / r)^6 the reference's r^-6 (c0 r^-6 -
+  // c1) is 24 eps u3 (2 u3 - 1)
+  const Rl e24 = c.sei * pj.y;
+  const Rl sg = c.sgi + pj.z;
+  const Rl w = both ? (Rl)1 : c.wi * wj;
+  Rl r2inv, rinv;
+  inv_r<Rl>(rsq, r2inv, rinv);
+  const Rl u = sg * sg * r2inv, u3 = u * u * u;
+  const Rl flj = e24 * u3 * ((Rl)2 * u3 - (Rl)1);
+  const Rl fc = c.qi * pj.x * rinv;
+  const Rl fn = -((flj + fc) * r2inv * (w * m));
+  gx = (double)(fn * rx);
+  gy = (double)(fn * ry);
+  gz = (double)(fn * rz);
+  if (EF) {
+    const double ep = (double)((e24 * (Rl)(1.0 / 6.0) * u3 * (u3 - (Rl)1) + fc) * m);
+    e_ion += both ? ep : 0.0;
+    e_vi += cross ? ep : 0.0;
+    e_vv += (both | cross) ? 0.0 : ep;
+  }
+}
+
 template <typename Rl, bool EF>
 __device__ inline void dense_n3(const DevReplica &R, const BatchLds<Rl> &B, int nh, int ni, bool am, bool vi, bool vv,
                                 int t, int nt, double *e) {
@@ -397,99 +464,71 @@ __device__ inline void dense_n3(const DevReplica &R, const BatchLds<Rl> &B, int 
   const int NB = (nh + 63) >> 6, n_off = NB * (NB - 1) / 2;
   int L = 32;
   while (L > 8 && NB * (32 / L) + n_off * (64 / L) < nw) L >>= 1;
-  const int W = __builtin_amdgcn_readfirstlane(R.amber_words), wmax = W > 0 ? W - 1 : 0;
-  const double *__restrict__ px = B.x;
-  const typename Pack4<Rl>::T *__restrict__ pk = B.pk;
-  const Rl *__restrict__ wr = B.wr;
+  const int W = __builtin_amdgcn_readfirstlane(R.amber_words);
   double *__restrict__ pf = B.f;
   double e_ion = 0, e_vi = 0, e_vv = 0;
   int item = 0;
   for (int bi = 0; bi < NB; bi++) {
     const int i = 64 * bi + lane;
-    const bool i_ok = i < nh, ion_i = i < ni;
     for (int bj = bi; bj < NB; bj++) {
       const bool diag = bi == bj;
       const int np = (diag ? 32 : 64) / L;
       for (int part = 0; part < np; part++, item++) {
         if (item % nw != wave) continue;
-        const double xi0 = px[3 * i], xi1 = px[3 * i + 1], xi2 = px[3 * i + 2];
-        const typename Pack4<Rl>::T pi = pk[i];
-        const Rl qi = qq * pi.x, sei = pi.y, sgi = pi.z, mi = pi.w, wi = wr[i];
-        const uint32_t *arow = B.allow + (size_t)(ion_i ? i : 0) * W;
+        DenseI<Rl> c;
+        c.x0 = B.x[3 * i]; c.x1 = B.x[3 * i + 1]; c.x2 = B.x[3 * i + 2];
+        const typename Pack4<Rl>::T pi = B.pk[i];
+        c.qi = qq * pi.x; c.sei = pi.y; c.sgi = pi.z; c.mi = pi.w; c.wi = B.wr[i];
+        c.arow = B.allow + (size_t)(i < ni ? i : 0) * W;
+        c.ok = i < nh ? 1u : 0u; c.ion = i < ni ? 1u : 0u;
+        c.am = am ? 1u : 0u; c.vi = vi ? 1u : 0u; c.vv = vv ? 1u : 0u;
+        c.ni = ni; c.nh = nh; c.wmax = W > 0 ? W - 1 : 0;
         double fx = 0, fy = 0, fz = 0;
-        const int kb = (diag ? 1 : 0) + part * L;
-        // pair selection as integer masks (no short-circuit control flow)
-        const uint32_t ok_i = i_ok ? 1u : 0u, io_i = ion_i ? 1u : 0u;
-        const uint32_t amu = am ? 1u : 0u, viu = vi ? 1u : 0u, vvu = vv ? 1u : 0u;
-        const uint32_t half_ok = (!diag || lane < 32) ? 1u : 0u;
-        // partner data is loaded one step ahead; the prefetch after the
-        // last step reads an in-range partner, whose index then names the
-        // atom of each lane's F_j accumulator
-        double ajx = 0, ajy = 0, ajz = 0;
-        int jn = 64 * bj + ((lane + kb) & 63);
-        typename Pack4<Rl>::T pn = pk[jn];
-        double xn0 = px[3 * jn], xn1 = px[3 * jn + 1], xn2 = px[3 * jn + 2];
-        Rl wn = wr[jn];
-        uint32_t an = arow[min(jn >> 5, wmax)];
-#pragma unroll 2
-        for (int k = kb; k < kb + L; k++) {
-          const int j = jn;
-          const typename Pack4<Rl>::T pj = pn;
-          const double dx = xi0 - xn0, dy = xi1 - xn1, dz = xi2 - xn2;
-          const Rl wj = wn;
-          const uint32_t abit = (an >> (j & 31)) & 1u;
-          jn = 64 * bj + ((lane + k + 1) & 63);
-          pn = pk[jn];
-          {
-            const double *xq = px + __umul24((unsigned)jn, 3u);
-            xn0 = xq[0]; xn1 = xq[1]; xn2 = xq[2];
+        int jend0, jend1 = 0;
+        double a0x = 0, a0y = 0, a0z = 0, a1x = 0, a1y = 0, a1z = 0;
+        if (diag) {
+          // k = 1..32 in runs of L; k = 32 only on lanes 0..31
+          const int kb = 1 + part * L;
+          const uint32_t half_ok = lane < 32 ? 1u : 0u;
+          for (int k = kb; k < kb + L; k++) {
+            double gx, gy, gz;
+            dense_pair<Rl, EF>(B, c, 64 * bj + ((lane + k) & 63), k != 32 ? 1u : half_ok, gx, gy, gz, e_ion, e_vi,
+                               e_vv);
+            fx -= gx; fy -= gy; fz -= gz;
+            a0x = wave_rol1(a0x + gx);
+            a0y = wave_rol1(a0y + gy);
+            a0z = wave_rol1(a0z + gz);
           }
-          wn = wr[jn];
-          an = arow[min(jn >> 5, wmax)];
-          const uint32_t io_j = j < ni ? 1u : 0u;
-          const uint32_t both = io_i & io_j, cross = io_i ^ io_j;
-          const uint32_t diff_mol = pj.w != mi ? 1u : 0u;
-          const uint32_t pass = (both & amu & abit) | (cross & viu) | ((1u ^ (both | cross)) & vvu & diff_mol);
-          const uint32_t valid = ok_i & (j < nh ? 1u : 0u) & (k != 32 ? 1u : half_ok);
-          const uint32_t on = valid & pass;
-          const Rl m = on ? (Rl)1 : (Rl)0;
-          const Rl rx = (Rl)dx, ry = (Rl)dy, rz = (Rl)dz;
-          const Rl r2 = rx * rx + ry * ry + rz * rz;
-          const Rl rsq = on ? r2 : (Rl)1;
-          // 24 eps_ij and sigma_ij from the packed per-atom sqrt(24 eps)
-          // and sigma / 2; with u3 = (sigma_ij / r)^6 the reference's
-          // r^-6 (c0 r^-6 - c1) is 24 eps u3 (2 u3 - 1)
-          const Rl e24 = sei * pj.y;
-          const Rl sg = sgi + pj.z;
-          const Rl w = both ? (Rl)1 : wi * wj;
-          Rl r2inv, rinv;
-          inv_r<Rl>(rsq, r2inv, rinv);
-          const Rl u = sg * sg * r2inv, u3 = u * u * u;
-          const Rl flj = e24 * u3 * ((Rl)2 * u3 - (Rl)1);
-          const Rl fc = qi * pj.x * rinv;
-          // minus the pair force factor (zero for masked pairs): F_j gets
-          // +g, F_i gets -g
-          const Rl fn = -((flj + fc) * r2inv * (w * m));
-          const double gx = (double)(fn * rx), gy = (double)(fn * ry), gz = (double)(fn * rz);
-          fx -= gx;
-          fy -= gy;
-          fz -= gz;
-          ajx = wave_rol1(ajx + gx);
-          ajy = wave_rol1(ajy + gy);
-          ajz = wave_rol1(ajz + gz);
-          if (EF) {
-            const double ep = (double)((e24 * (Rl)(1.0 / 6.0) * u3 * (u3 - (Rl)1) + fc) * m);
-            e_ion += both ? ep : 0.0;
-            e_vi += cross ? ep : 0.0;
-            e_vv += (both | cross) ? 0.0 : ep;
+          jend0 = 64 * bj + ((lane + kb + L) & 63);
+        } else {
+          // two independent partners per step, k and k + 32, each with its
+          // own travelling F_j accumulator
+          const int hL = L >> 1, kb = part * hL;
+          for (int k = kb; k < kb + hL; k++) {
+            double gx, gy, gz, hx, hy, hz;
+            dense_pair<Rl, EF>(B, c, 64 * bj + ((lane + k) & 63), 1u, gx, gy, gz, e_ion, e_vi, e_vv);
+            dense_pair<Rl, EF>(B, c, 64 * bj + ((lane + k + 32) & 63), 1u, hx, hy, hz, e_ion, e_vi, e_vv);
+            fx -= gx + hx; fy -= gy + hy; fz -= gz + hz;
+            a0x = wave_rol1(a0x + gx);
+            a0y = wave_rol1(a0y + gy);
+            a0z = wave_rol1(a0z + gz);
+            a1x = wave_rol1(a1x + hx);
+            a1y = wave_rol1(a1y + hy);
+            a1z = wave_rol1(a1z + hz);
           }
+          jend0 = 64 * bj + ((lane + kb + hL) & 63);
+          jend1 = 64 * bj + ((lane + kb + hL + 32) & 63);
+          atomicAdd(&pf[3 * jend1], a1x);
+          atomicAdd(&pf[3 * jend1 + 1], a1y);
+          atomicAdd(&pf[3 * jend1 + 2], a1z);
         }
-        // 64 distinct atoms of block bj (zero sums for padding and masked
-        // partners)
-        atomicAdd(&pf[3 * jn], ajx);
-        atomicAdd(&pf[3 * jn + 1], ajy);
-        atomicAdd(&pf[3 * jn + 2], ajz);
-        if (i_ok) {
+        // each accumulator now belongs to the partner one step past the run
+        // (64 distinct atoms of block bj per accumulator set; zero sums for
+        // padding and masked partners)
+        atomicAdd(&pf[3 * jend0], a0x);
+        atomicAdd(&pf[3 * jend0 + 1], a0y);
+        atomicAdd(&pf[3 * jend0 + 2], a0z);
+        if (i < nh) {
           atomicAdd(&pf[3 * i], fx);
           atomicAdd(&pf[3 * i + 1], fy);
           atomicAdd(&pf[3 * i + 2], fz);
@@ -540,7 +579,12 @@ __device__ inline void batch_load(const DevReplica &R, const LView &L, const Bat
   for (int k = threadIdx.x; k < nkt; k += blockDim.x) {
     int sp = k < L.n_ion ? 0 : (k < L.n_ion + L.na_v ? 2 : 1);
     int idx = sp == 0 ? k : (sp == 2 ? k - L.n_ion : k - L.n_ion - L.na_v);
-    B.kct[k] = C / R.sp[sp].mass[idx];
+    const DevSpecies &S = R.sp[sp];
+    B.kct[k] = C / S.mass[idx];
+    B.kpk[k].x = (Rl)S.charge[idx];
+    B.kpk[k].y = (Rl)(S.se[idx] * kSqrt24);
+    B.kpk[k].z = (Rl)(0.5 * S.sg[idx]);
+    B.kpk[k].w = (Rl)-1;
   }
   for (int a = threadIdx.x; a < L.n_in; a += blockDim.x) {
     int sp, idx, kk;
@@ -555,17 +599,23 @@ __device__ inline void batch_load(const DevReplica &R, const LView &L, const Bat
     }
     const DevSpecies &S = R.sp[sp];
     B.kidx[a] = (int16_t)kk;
-    const int m = sp == 2 ? (a - L.vbase) / L.na_v : -1;
-    B.pk[a].x = (Rl)S.charge[idx];
-    B.pk[a].y = (Rl)(S.se[idx] * kSqrt24);
-    B.pk[a].z = (Rl)(0.5 * S.sg[idx]);
-    B.pk[a].w = (Rl)m;
+    if (a < L.gbase) {
+      const int m = sp == 2 ? (a - L.vbase) / L.na_v : -1;
+      B.pk[a].x = (Rl)S.charge[idx];
+      B.pk[a].y = (Rl)(S.se[idx] * kSqrt24);
+      B.pk[a].z = (Rl)(0.5 * S.sg[idx]);
+      B.pk[a].w = (Rl)m;
+    }
     B.wr[a] = (Rl)R.atom_w[g];
   }
-  // finite padding to the end of the last 64-atom block (read by dense_n3)
-  for (int a = L.n_in + threadIdx.x; a < ((L.n_in + 63) & ~63); a += blockDim.x) {
-    for (int q = 0; q < 3; q++) { B.x[3 * a + q] = 0.0; B.f[3 * a + q] = 0.0; }
+  // finite values to the end of the ion + vapor range's last 64-atom block
+  // (read by dense_n3): parameters past the range, atom data past n_in
+  const int hend = (L.gbase + 63) & ~63;
+  for (int a = L.gbase + threadIdx.x; a < hend; a += blockDim.x) {
     B.pk[a].x = (Rl)0; B.pk[a].y = (Rl)0; B.pk[a].z = (Rl)0; B.pk[a].w = (Rl)-1;
+  }
+  for (int a = L.n_in + threadIdx.x; a < hend; a += blockDim.x) {
+    for (int q = 0; q < 3; q++) { B.x[3 * a + q] = 0.0; B.f[3 * a + q] = 0.0; }
     B.wr[a] = (Rl)0;
   }
 }
@@ -696,7 +746,7 @@ __global__ void __launch_bounds__(512) k_batched(const DevReplica *__restrict__ 
       __syncthreads();
       // forces of every registered potential (verlet.cpp:31-32)
       SStoreT<Rl> ss;
-      ss.L = L; ss.px = B.x; ss.pf = B.f; ss.pk = B.pk; ss.wr = B.wr;
+      ss.L = L; ss.px = B.x; ss.pf = B.f; ss.pk = B.pk; ss.kpk = B.kpk; ss.kidx = B.kidx; ss.wr = B.wr;
       ss.ptype = R.sp[0].type;
       double e[U_N] = {0, 0, 0, 0, 0, 0, 0, 0};
       batch_terms<Rl>(R, ss, B, tid, nt, eflag, e);
@@ -773,8 +823,8 @@ struct mdn_ctx {
   int batch_cap = 0, batch_threads = 256, batch_grid = 0;
   mdn::Dyn *d_dyn = nullptr;
   int last_cap = 0, last_lds = 0, batch_chunk = 1000, batch_split = 1, n_cu = 256;
-  hipStream_t stream2 = nullptr;
-  hipEvent_t evj = nullptr, evk = nullptr;
+  hipStream_t stream2 = nullptr, stream3 = nullptr;
+  hipEvent_t evj = nullptr, evk = nullptr, evl = nullptr;
   int *d_order = nullptr;
   double last_ms = 0;
   int last_heavy = 0;
@@ -1095,8 +1145,10 @@ int mdn_create(int device, mdn_ctx **out) {
   HIPCHK(hipEventCreate(&c->ev0));
   HIPCHK(hipEventCreate(&c->ev1));
   HIPCHK(hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking));
+  HIPCHK(hipStreamCreateWithFlags(&c->stream3, hipStreamNonBlocking));
   HIPCHK(hipEventCreateWithFlags(&c->evj, hipEventDisableTiming));
   HIPCHK(hipEventCreateWithFlags(&c->evk, hipEventDisableTiming));
+  HIPCHK(hipEventCreateWithFlags(&c->evl, hipEventDisableTiming));
   hipDeviceProp_t prop;
   HIPCHK(hipGetDeviceProperties(&prop, device));
   c->n_cu = prop.multiProcessorCount;
@@ -1114,7 +1166,9 @@ void mdn_destroy(mdn_ctx *c) {
   if (c->ev1) hipEventDestroy(c->ev1);
   if (c->evj) hipEventDestroy(c->evj);
   if (c->evk) hipEventDestroy(c->evk);
+  if (c->evl) hipEventDestroy(c->evl);
   if (c->stream2) { hipStreamSynchronize(c->stream2); hipStreamDestroy(c->stream2); }
+  if (c->stream3) { hipStreamSynchronize(c->stream3); hipStreamDestroy(c->stream3); }
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
 }
@@ -1347,25 +1401,29 @@ static BatchCaps batch_caps(const mdn_ctx *c, const std::vector<Dyn> &dyn, const
     kcap = std::max(kcap, d.sp[0].natoms + d.sp[1].natoms + d.sp[2].natoms);
     if (d.flags & MDN_ION_AMBER) acap = std::max(acap, d.sp[0].natoms * ((d.sp[0].natoms + 31) / 32));
   }
-  // the atom capacity is a whole number of 64-atom blocks (dense_n3 reads
-  // whole blocks)
+  // the ion + vapor capacity is a whole number of 64-atom blocks (dense_n3
+  // reads whole blocks of positions, weights, forces and parameters), the
+  // total a multiple of 32 and at least that
+  auto r32 = [](int v) { return (v + 31) / 32 * 32; };
   auto r64 = [](int v) { return (v + 63) / 64 * 64; };
-  most = r64(most);
+  most = r32(most);
+  most_h = r64(most_h);
   int lo_h = std::min(most_h, r64(need_h + 32)), hi_h = std::min(most_h, r64(need_h * 3 / 2 + 32));
-  int lo = std::min(most, r64(need + need / 8 + 64)), hi = std::min(most, r64(need * 3 / 2 + 64));
+  int lo = std::min(most, r32(need + need / 8 + 64)), hi = std::min(most, r32(need * 3 / 2 + 64));
   BatchCaps b;
   b.kcap = kcap; b.acap = acap;
   if (c->batch_cap > 0) {
-    b.cap = r64(c->batch_cap);
-    b.cap_h = std::min(b.cap, hi_h);
+    b.cap = r32(c->batch_cap);
+    b.cap_h = std::min(hi_h, b.cap & ~63);
   } else {
     b.cap = hi; b.cap_h = hi_h;
-    while (batch_lds_bytes(b.cap, b.cap_h, kcap, acap, rl) > budget && (b.cap > lo || b.cap_h > lo_h)) {
-      if (b.cap > lo) b.cap -= 64;
-      if (b.cap_h > lo_h) b.cap_h -= 32;
+    while (batch_lds_bytes(std::max(b.cap, b.cap_h), b.cap_h, kcap, acap, rl) > budget &&
+           (b.cap > lo || b.cap_h > lo_h)) {
+      if (b.cap > lo) b.cap -= 32;
+      if (b.cap_h > lo_h) b.cap_h -= 64;
     }
-    b.cap_h = std::min(b.cap_h, b.cap);
   }
+  b.cap = std::max(b.cap, b.cap_h);
   b.lds = batch_lds_bytes(b.cap, b.cap_h, kcap, acap, rl);
   return b;
 }
@@ -1377,7 +1435,8 @@ static int launch_batched(mdn_ctx *c, hipStream_t st, const int *order, int n, i
   if (b.lds > 160 * 1024)
     return set_error(MDN_E_CAPACITY, "mdn_run_batched: a replica's in-domain atoms exceed one workgroup's LDS; "
                                      "use mdn_verlet for it");
-  HIPCHK(hipFuncSetAttribute((const void *)k_batched<Rl>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)b.lds));
+  // launches of different sizes may run concurrently: allow the full LDS
+  HIPCHK(hipFuncSetAttribute((const void *)k_batched<Rl>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
   int grid = c->batch_grid > 0 ? std::min(c->batch_grid, n) : n;
   hipLaunchKernelGGL(k_batched<Rl>, dim3(grid), dim3(threads), b.lds, st, c->d_reps, order, n, ns, eflag_every,
                      step0, b.cap, b.cap_h, b.kcap, b.acap);
@@ -1407,8 +1466,10 @@ int mdn_run_batched(mdn_ctx *c, int32_t nsteps, int32_t eflag_every, int64_t ste
     HIPCHK(hipMemcpy(dyn.data(), c->d_dyn, sizeof(Dyn) * dyn.size(), hipMemcpyDeviceToHost));
     // Work per step grows with the square of the ion + vapor atom count.  The
     // heaviest replicas get a whole CU each (512-thread workgroups: at 256
-    // VGPRs one fills a CU) and the rest share CUs four at a time
-    // (128-thread workgroups), so that every replica runs in the first round.
+    // VGPRs one fills a CU); the rest share CUs in 128-thread workgroups, in
+    // two size classes whose LDS lets four (<= 40 KB) or three (<= 53 KB)
+    // share a CU.  The heavy count is the largest (among replicas well above
+    // the median cost) that lets every workgroup be resident at once.
     std::vector<double> cost(n);
     for (int r = 0; r < n; r++) {
       const SysDef &d = c->reps[r].def;
@@ -1418,35 +1479,66 @@ int mdn_run_batched(mdn_ctx *c, int32_t nsteps, int32_t eflag_every, int64_t ste
       order[r] = r;
     }
     std::stable_sort(order.begin(), order.end(), [&](int a, int b2) { return cost[a] > cost[b2]; });
-    int H = 0;
+    const size_t kA = 40 * 1024, kB = 53 * 1024;
+    std::vector<int> heavy, clsA, clsB;
     if (c->batch_split && c->batch_threads == 256 && n > c->n_cu) {
-      // heavy count: H + ceil((n - H) / 4) <= CUs, and only replicas well
-      // above the median cost
+      // each replica's own LDS need at the smallest headroom
+      std::vector<size_t> own(n);
+      for (int r = 0; r < n; r++) own[r] = batch_caps(c, dyn, std::vector<int>{r}, rl, 0).lds;
       double med = cost[order[n / 2]];
-      while (H < n && cost[order[H]] > 2.0 * med && H + 1 + (n - H - 1 + 3) / 4 <= c->n_cu) H++;
+      int Hmax = 0;
+      while (Hmax < n && Hmax < c->n_cu && cost[order[Hmax]] > 2.0 * med) Hmax++;
+      for (int H = Hmax; H >= 0; H--) {
+        heavy.assign(order.begin(), order.begin() + H);
+        clsA.clear();
+        clsB.clear();
+        for (int k = H; k < n; k++) {
+          const int r = order[k];
+          if (own[r] <= kA) clsA.push_back(r);
+          else if (own[r] <= kB) clsB.push_back(r);
+          else heavy.push_back(r);
+        }
+        const int cus = (int)heavy.size() + ((int)clsA.size() + 3) / 4 + ((int)clsB.size() + 2) / 3;
+        if (cus <= c->n_cu) break;
+      }
+      order.clear();
+      order.insert(order.end(), heavy.begin(), heavy.end());
+      order.insert(order.end(), clsB.begin(), clsB.end());
+      order.insert(order.end(), clsA.begin(), clsA.end());
     }
-    std::vector<int> heavy(order.begin(), order.begin() + H), light(order.begin() + H, order.end());
+    const int H = (int)heavy.size(), NB_ = (int)clsB.size(), NA = (int)clsA.size();
     HIPCHK(hipMemcpyAsync(c->d_order, order.data(), sizeof(int) * n, hipMemcpyHostToDevice, c->stream));
     const long long s0 = (long long)(step0 + done);
     HIPCHK(hipEventRecord(c->ev0, c->stream));
     int rc;
-    if (H > 0) {
-      BatchCaps bh = batch_caps(c, dyn, heavy, rl, 160 * 1024);
-      BatchCaps bl = batch_caps(c, dyn, light, rl, 40 * 1024);
+    if (H + NB_ + NA > 0) {
+      // heavy on stream, class B on stream2, class A on stream3, all after
+      // the order upload and joined back into stream
       HIPCHK(hipEventRecord(c->evj, c->stream));
       HIPCHK(hipStreamWaitEvent(c->stream2, c->evj, 0));
-      rc = precision == MDN_MIXED
-               ? launch_batched<float>(c, c->stream, c->d_order, H, 512, ns, eflag_every, s0, bh)
-               : launch_batched<double>(c, c->stream, c->d_order, H, 512, ns, eflag_every, s0, bh);
-      if (rc) return rc;
-      rc = precision == MDN_MIXED
-               ? launch_batched<float>(c, c->stream2, c->d_order + H, n - H, 128, ns, eflag_every, s0, bl)
-               : launch_batched<double>(c, c->stream2, c->d_order + H, n - H, 128, ns, eflag_every, s0, bl);
-      if (rc) return rc;
+      HIPCHK(hipStreamWaitEvent(c->stream3, c->evj, 0));
+      size_t lds = 0;
+      int cap = 0;
+      struct Part { hipStream_t st; int off, cnt, threads; size_t budget; const std::vector<int> *set; };
+      const Part parts[3] = {{c->stream, 0, H, 512, 160 * 1024, &heavy},
+                             {c->stream2, H, NB_, 128, kB, &clsB},
+                             {c->stream3, H + NB_, NA, 128, kA, &clsA}};
+      for (const Part &pt : parts) {
+        if (pt.cnt == 0) continue;
+        BatchCaps bc = batch_caps(c, dyn, *pt.set, rl, pt.budget);
+        rc = precision == MDN_MIXED
+                 ? launch_batched<float>(c, pt.st, c->d_order + pt.off, pt.cnt, pt.threads, ns, eflag_every, s0, bc)
+                 : launch_batched<double>(c, pt.st, c->d_order + pt.off, pt.cnt, pt.threads, ns, eflag_every, s0, bc);
+        if (rc) return rc;
+        lds = std::max(lds, bc.lds);
+        cap = std::max(cap, bc.cap);
+      }
       HIPCHK(hipEventRecord(c->evk, c->stream2));
       HIPCHK(hipStreamWaitEvent(c->stream, c->evk, 0));
-      c->last_cap = std::max(bh.cap, bl.cap);
-      c->last_lds = (int)std::max(bh.lds, bl.lds);
+      HIPCHK(hipEventRecord(c->evl, c->stream3));
+      HIPCHK(hipStreamWaitEvent(c->stream, c->evl, 0));
+      c->last_cap = cap;
+      c->last_lds = (int)lds;
     } else {
       BatchCaps b = batch_caps(c, dyn, order, rl, 80 * 1024);
       rc = precision == MDN_MIXED
