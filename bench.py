@@ -6,9 +6,11 @@ vapor molecules, 5,000 N2, 300 K, dt 1 fs), FP32 pair arithmetic with FP64
 accumulation and integration.  Beside it, on the same workload:
   * the plain FP64 per-call GPU path (one replica after another, one launch
     per reference call, no graph), timed on a sample of replicas;
-  * the CPU baseline: the FP64 oracle (a C restatement of the reference's
-    loops, one replica per OpenMP thread as the reference's main does),
-    timed on a bounded sample;
+  * the CPU baseline: the reference's own stepping loop, built from its
+    sources (oracle/_ref), one single-threaded process per replica as the
+    reference's main runs one replica per OpenMP thread, on a bounded sample
+    (the FP64 oracle, a C restatement of the same loops, is timed beside it
+    and stands in when the reference build is absent);
   * the relative RMS force error of the mixed path against the FP64 path on
     identical states.
 
@@ -120,19 +122,49 @@ def cpu_baseline(reps, seconds=12.0):
                       % (len(sample), len(sample), nsteps, el, threads)}
 
 
+REF_HARNESS = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+
+
+def cpu_baseline_reference(d, n_rep, steps=20000, procs=16):
+    """The reference's own stepping loop (MD::verlet with its own rebuilds,
+    built from the reference sources by oracle/Makefile) on the first
+    replicas of the workload: one single-threaded process per replica, as the
+    reference's main runs one replica per OpenMP thread.  Started before
+    anything touches the GPU.  None when the harness is not built."""
+    import subprocess
+    if not os.path.exists(REF_HARNESS):
+        return None
+    k = min(procs, n_rep)
+    env = dict(os.environ, OMP_NUM_THREADS="1")
+    ps = [subprocess.Popen([REF_HARNESS, os.path.join(d, "c4_%d.cond" % r), "-", "time", str(steps)], cwd=d,
+                           env=env, stdout=subprocess.PIPE, stderr=subprocess.DEVNULL, text=True) for r in range(k)]
+    acc, secs, wall = 0.0, 0.0, 0.0
+    for p in ps:
+        out, _ = p.communicate()
+        if p.returncode != 0:
+            return None
+        v = {ln.split()[0]: float(ln.split()[2]) for ln in out.strip().splitlines()}
+        acc += v["atom_steps"]
+        secs += v["seconds"]
+        wall = max(wall, v["seconds"])
+    # the whole mix on k cores kept busy: k x (atom-steps / process-seconds)
+    return {"value": k * acc / secs, "unit": "atom-steps/s", "cores": k, "kind": "reference",
+            "sample": "%d replicas (first %d of the workload) x %d steps each: the reference's own MD::verlet loop "
+                      "(its rebuilds; SHAKE is not called there), g++ -O3 -fopenmp build of its sources, one "
+                      "single-threaded process per replica at once, %.1f s longest loop; value = %d x total "
+                      "atom-steps / total process-seconds" % (k, k, steps, wall, k)}
+
+
 def copy_check(nbytes):
     """A plain device-to-device copy of known size (the reference point for
-    rocprofv3's FETCH_SIZE / WRITE_SIZE counters)."""
-    import ctypes as C
-    hip = C.CDLL("libamdhip64.so")
-    a, b = C.c_void_p(), C.c_void_p()
-    assert hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) == 0
-    assert hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)) == 0
-    assert hip.hipMemset(a, 1, C.c_size_t(nbytes)) == 0
-    assert hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3) == 0  # device to device
-    assert hip.hipDeviceSynchronize() == 0
-    hip.hipFree(a)
-    hip.hipFree(b)
+    rocprofv3's FETCH_SIZE / WRITE_SIZE counters), through the same HIP
+    runtime as libmdn."""
+    a = torch.ones(nbytes, dtype=torch.uint8, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    del a, b
     print("copy_check: %d bytes device-to-device" % nbytes, file=sys.stderr, flush=True)
 
 
@@ -244,6 +276,7 @@ def main():
     with tempfile.TemporaryDirectory() as d:
         t0 = time.perf_counter()
         reps = W.batched_replicas(d, n_rep=args.replicas)
+        cpu_ref = None if args.no_cpu else cpu_baseline_reference(d, len(reps))
         eng = W.build_engine(reps)
         print("workload ready: %d replicas in %.1f s" % (len(reps), time.perf_counter() - t0), file=sys.stderr,
               flush=True)
@@ -309,7 +342,8 @@ def main():
         plain = (total_atom_steps(ep, pr) - p0) / pel
         ep.close()
 
-        cpu = None if args.no_cpu else cpu_baseline(reps, args.cpu_seconds)
+        cpu_port = None if args.no_cpu else cpu_baseline(reps, args.cpu_seconds)
+        cpu = cpu_ref or cpu_port
         cl = None if args.no_cluster else cluster_bench(os.path.join(d, "cluster"), with_cpu=not args.no_cpu)
         adj = None if args.no_adjoint else adjoint_bench(os.path.join(d, "fm"))
         if args.copy_check:
@@ -334,6 +368,7 @@ def main():
                      "lds_cap_atoms": lds_cap,
                      "flops_per_step": flops},
         "cpu_baseline": cpu,
+        "cpu_oracle": cpu_port if cpu_ref else None,
         "autotune": tuned,
         "cluster": cl,
         "adjoint": adj,

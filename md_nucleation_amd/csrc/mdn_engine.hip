@@ -391,9 +391,9 @@ struct SStoreT {
 // pairs the reference evaluates: ion-ion by the AMBER exclusion bitmask,
 // ion-vapor by vi, vapor-vapor of distinct molecules by vv.
 //
-// Work items are runs of L pairs per lane of one block pair (L = 32, or 16 /
-// 8 when there are fewer items than waves), dealt round-robin to the waves,
-// so the waves finish together.  The LDS atom arrays are padded to whole
+// Work items are runs of L pairs per lane of one block pair (L = 32, 16 or
+// 8, whichever gives the waves the fewest rounds), dealt round-robin to the
+// waves, so the waves finish together.  The LDS atom arrays are padded to whole
 // 64-atom blocks with finite values (batch_load), so every lane loads and
 // adds unconditionally and the loop body has no branches: masked pairs carry
 // a zero force.
@@ -462,8 +462,13 @@ __device__ inline void dense_n3(const DevReplica &R, const BatchLds<Rl> &B, int 
   nh = __builtin_amdgcn_readfirstlane(nh);
   ni = __builtin_amdgcn_readfirstlane(ni);
   const int NB = (nh + 63) >> 6, n_off = NB * (NB - 1) / 2;
-  int L = 32;
-  while (L > 8 && NB * (32 / L) + n_off * (64 / L) < nw) L >>= 1;
+  // run length: the fewest rounds of items per wave, counting about six
+  // steps of set-up and flush per item
+  int L = 32, best = 1 << 30;
+  for (int l = 32; l >= 8; l >>= 1) {
+    const int items = NB * (32 / l) + n_off * (64 / l), c = (items + nw - 1) / nw * (l + 6);
+    if (c < best) { best = c; L = l; }
+  }
   const int W = __builtin_amdgcn_readfirstlane(R.amber_words);
   double *__restrict__ pf = B.f;
   double e_ion = 0, e_vi = 0, e_vv = 0;
