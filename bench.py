@@ -37,6 +37,10 @@ sys.path.insert(0, ROOT)
 METRIC = "atom-steps/s: batched mixed-prec vs plain FP64 GPU at ≤1e-5 rel force err"
 PEAK_FP32_TFLOPS = 157.3
 PEAK_FP64_TFLOPS = 78.6
+# HBM bytes per step of the heavy launch (155 replicas with a CU each), from
+# the FETCH_SIZE / WRITE_SIZE passes in profiles/k_batched_mixed_512replicas.md
+# (copy-calibrated: FETCH_SIZE x 2.0, WRITE_SIZE x 1.0)
+HEAVY_HBM_BYTES_PER_STEP = 0.561e6
 # algorithmic FLOPs per unique pair / bonded term, as written in the reference
 FLOP = {"ljcoul_w": 31, "ljcoul": 30, "lj": 24, "bond": 22, "angle": 70, "dihedral": 150}
 
@@ -319,6 +323,7 @@ def main():
         eng.sync()
         el = time.perf_counter() - t0
         kern_ms = eng.last_kernel_ms()
+        steps_per_launch = min(args.steps, tuned["shape"][3] if tuned and tuned["shape"][3] > 0 else args.steps)
         lds_cap, lds_bytes, n_heavy = eng.batch_info()
         a1 = total_atom_steps(eng, n)
         value = (a1 - a0) / el
@@ -363,7 +368,10 @@ def main():
         "batched_vs_plain": value / plain,
         "force_err": {"rel_rms": force_err, "max_abs": fmax, "vs": "FP64 path, identical states, %d replicas" % k_err},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                     "frac": achieved / PEAK_FP32_TFLOPS,
+                     "traffic": HEAVY_HBM_BYTES_PER_STEP * steps_per_launch,
+                     "traffic_note": "HBM bytes per heavy launch (%d steps), counter passes in "
+                                     "profiles/k_batched_mixed_512replicas.md" % steps_per_launch,
                      "kernel": "k_batched<float>", "kernel_ms": kern_ms, "lds_bytes_per_wg": lds_bytes, "replicas_with_own_cu": n_heavy,
                      "lds_cap_atoms": lds_cap,
                      "flops_per_step": flops},
