@@ -296,6 +296,14 @@ __device__ inline void dihedral_f(const double vb1[3], const double vb2[3], cons
 
 // ------------------------------------------------------------------------
 // Workgroup helpers (nthr = blockDim.x, multiple of 64, at most 1024)
+// 1 / d: the hardware reciprocal estimate refined by two Newton-Raphson
+// steps, r <- r + r (1 - d r) (each roughly doubles the correct bits)
+__device__ inline double rcp_nr(double d) {
+  double r = __builtin_amdgcn_rcp(d);
+  r = fma(r, fma(-d, r, 1.0), r);
+  r = fma(r, fma(-d, r, 1.0), r);
+  return r;
+}
 __device__ inline double wave_sum(double v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
   return v;

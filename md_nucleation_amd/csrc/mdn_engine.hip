@@ -305,11 +305,13 @@ struct BatchLds {
   uint32_t *allow;  // [acap] AMBER non-excluded ion-pair bitmask (rows of amber_words)
   double *red;      // [U_N + 6][16]
   double *sdbl;     // [16]
+  double *kms;      // [kcap] species-atom masses
+  ShakeLds shk;     // constraint tables of the ion and the vapor [2][scap]
   int *sint;        // [64]
 };
 
 template <typename Rl>
-__device__ inline BatchLds<Rl> carve(char *base, int cap, int cap_h, int kcap, int acap) {
+__device__ inline BatchLds<Rl> carve(char *base, int cap, int cap_h, int kcap, int acap, int scap) {
   BatchLds<Rl> B;
   double *d = reinterpret_cast<double *>(base);
   B.x = d; d += 3 * cap;
@@ -319,6 +321,10 @@ __device__ inline BatchLds<Rl> carve(char *base, int cap, int cap_h, int kcap, i
   B.kct = d; d += kcap;
   B.red = d; d += (U_N + 6) * 16;
   B.sdbl = d; d += 16;
+  B.kms = d; d += kcap;
+  B.shk.c = d; d += 8 * (size_t)scap;
+  B.shk.cons = reinterpret_cast<unsigned long long *>(d); d += 2;
+  B.shk.cap = scap;
   B.pk = reinterpret_cast<typename Pack4<Rl>::T *>(d);
   d += (4 * sizeof(Rl) * (size_t)cap_h) / sizeof(double);
   B.kpk = reinterpret_cast<typename Pack4<Rl>::T *>(d);
@@ -327,14 +333,18 @@ __device__ inline BatchLds<Rl> carve(char *base, int cap, int cap_h, int kcap, i
   B.wr = r; r += cap;
   int *ip = reinterpret_cast<int *>(r);
   B.sint = ip; ip += 64;
+  B.shk.n = ip; ip += 2;
+  B.shk.ij = ip; ip += 4 * (size_t)scap;
   B.allow = reinterpret_cast<uint32_t *>(ip); ip += acap;
   int16_t *hp = reinterpret_cast<int16_t *>(ip);
   B.kidx = hp; hp += cap;
   return B;
 }
 
-static size_t batch_lds_bytes(int cap, int cap_h, int kcap, int acap, size_t rl) {
-  return sizeof(double) * ((size_t)9 * cap + 3 * (size_t)cap_h + kcap + (U_N + 6) * 16 + 16) +
+static size_t batch_lds_bytes(int cap, int cap_h, int kcap, int acap, int scap, size_t rl) {
+  return sizeof(double) * ((size_t)9 * cap + 3 * (size_t)cap_h + 2 * (size_t)kcap + (U_N + 6) * 16 + 16 +
+                           8 * (size_t)scap + 2) +
+         sizeof(int) * (2 + 4 * (size_t)scap) +
          ((4 * rl * (size_t)kcap + sizeof(double) - 1) / sizeof(double)) * sizeof(double) +
          rl * (4 * (size_t)cap_h + (size_t)cap) + sizeof(int) * (64 + (size_t)acap) + sizeof(int16_t) * (size_t)cap;
 }
@@ -451,22 +461,35 @@ __device__ inline void dense_pair(const BatchLds<Rl> &B, const DenseI<Rl> &c, in
   }
 }
 
+// Item filter: every block pair, only those of two whole-ion blocks (ion-ion
+// pairs: no vapor position or weight involved), or all but those.
+enum { kDenseAll = 0, kDenseIonOnly = 1, kDenseNotIonOnly = 2 };
+
 template <typename Rl, bool EF>
 __device__ inline void dense_n3(const DevReplica &R, const BatchLds<Rl> &B, int nh, int ni, bool am, bool vi, bool vv,
-                                int t, int nt, double *e) {
+                                int t, int t0, int nt, int filt, double *e) {
   const Rl qq = (Rl)R.qqrd2e;
-  // the wave index and the counts are uniform within a wave: say so, so that
-  // the item loops and their trip counts stay scalar
-  const int lane = t & 63, wave = __builtin_amdgcn_readfirstlane(t >> 6);
+  // threads [t0, t0 + nt) take part (whole waves).  The wave index and the
+  // counts are uniform within a wave: say so, so that the item loops and
+  // their trip counts stay scalar
+  const int lane = t & 63, wave = __builtin_amdgcn_readfirstlane((t - t0) >> 6);
   const int nw = __builtin_amdgcn_readfirstlane(nt >> 6);
   nh = __builtin_amdgcn_readfirstlane(nh);
   ni = __builtin_amdgcn_readfirstlane(ni);
-  const int NB = (nh + 63) >> 6, n_off = NB * (NB - 1) / 2;
+  const int NB = (nh + 63) >> 6, NBi = ni >> 6;  // blocks; whole-ion blocks
+  auto keep = [&](int bi, int bj) {
+    const bool ion_only = bj < NBi;  // bi <= bj
+    return filt == kDenseAll || (filt == kDenseIonOnly) == ion_only;
+  };
+  int n_diag = 0, n_off = 0;
+  for (int bi = 0; bi < NB; bi++)
+    for (int bj = bi; bj < NB; bj++)
+      if (keep(bi, bj)) (bi == bj ? n_diag : n_off)++;
   // run length: the fewest rounds of items per wave, counting about six
   // steps of set-up and flush per item
   int L = 32, best = 1 << 30;
   for (int l = 32; l >= 8; l >>= 1) {
-    const int items = NB * (32 / l) + n_off * (64 / l), c = (items + nw - 1) / nw * (l + 6);
+    const int items = n_diag * (32 / l) + n_off * (64 / l), c = (items + nw - 1) / nw * (l + 6);
     if (c < best) { best = c; L = l; }
   }
   const int W = __builtin_amdgcn_readfirstlane(R.amber_words);
@@ -476,6 +499,7 @@ __device__ inline void dense_n3(const DevReplica &R, const BatchLds<Rl> &B, int 
   for (int bi = 0; bi < NB; bi++) {
     const int i = 64 * bi + lane;
     for (int bj = bi; bj < NB; bj++) {
+      if (!keep(bi, bj)) continue;
       const bool diag = bi == bj;
       const int np = (diag ? 32 : 64) / L;
       for (int part = 0; part < np; part++, item++) {
@@ -545,15 +569,15 @@ __device__ inline void dense_n3(const DevReplica &R, const BatchLds<Rl> &B, int 
 }
 
 template <typename Rl>
-__device__ void dense_heavy(const DevReplica &R, const BatchLds<Rl> &B, const LView &L, int t, int nt, bool eflag,
-                            double *e) {
+__device__ void dense_heavy(const DevReplica &R, const BatchLds<Rl> &B, const LView &L, int t, int t0, int nt,
+                            int filt, bool eflag, double *e) {
   const int nh = L.gbase, ni = L.n_ion;
   const bool am = (R.flags & B_AMBER) != 0, vi = (R.flags & B_VI) != 0, vv = (R.flags & B_VV) != 0;
   if (!(am || vi || vv) || nh == 0) return;
   if (eflag)
-    dense_n3<Rl, true>(R, B, nh, ni, am, vi, vv, t, nt, e);
+    dense_n3<Rl, true>(R, B, nh, ni, am, vi, vv, t, t0, nt, filt, e);
   else
-    dense_n3<Rl, false>(R, B, nh, ni, am, vi, vv, t, nt, e);
+    dense_n3<Rl, false>(R, B, nh, ni, am, vi, vv, t, t0, nt, filt, e);
 }
 
 // the batched path's force phase: registration order of setPotential.cpp
@@ -563,7 +587,7 @@ __device__ void batch_terms(const DevReplica &R, const SStoreT<Rl> &s, const Bat
                             bool eflag, double *e) {
   const uint32_t bits = R.flags;
   if (bits & B_GI) term_gi<Rl>(R, s, t, nt, eflag, e);
-  dense_heavy<Rl>(R, B, s.L, t, nt, eflag, e);
+  dense_heavy<Rl>(R, B, s.L, t, 0, nt, kDenseAll, eflag, e);
   if (bits & B_GG) term_gg<Rl>(R, s, t, nt, eflag, e);
   if (bits & B_GV) term_gv<Rl>(R, s, t, nt, eflag, e);
   if (bits & B_EFIELD) term_efield(R, s, t, nt);
@@ -575,17 +599,47 @@ __device__ void batch_terms(const DevReplica &R, const SStoreT<Rl> &s, const Bat
   if (bits & B_GINTRA) term_gintra(R, s, t, nt, eflag, e);
 }
 
+// The forces of a step whose vapor SHAKE runs beside them: phase A uses
+// only ion and gas positions (threads t of [t0, t0 + nt), whole waves);
+// phase B (all threads, after SHAKE and the vapor weights) the rest.
+template <typename Rl>
+__device__ void batch_terms_a(const DevReplica &R, const SStoreT<Rl> &s, const BatchLds<Rl> &B, int t, int t0, int nt,
+                              bool eflag, double *e) {
+  const uint32_t bits = R.flags;
+  const int tl = t - t0;
+  if (bits & B_GI) term_gi<Rl>(R, s, tl, nt, eflag, e);
+  dense_heavy<Rl>(R, B, s.L, t, t0, nt, kDenseIonOnly, eflag, e);
+  if (bits & B_EFIELD) term_efield(R, s, tl, nt);
+  if (bits & B_TERSOFF) term_tersoff(R, s, tl, nt, eflag, e);
+  if (bits & B_SW) term_sw(R, s, tl, nt, eflag, e);
+  if (bits & B_AMBER) term_amber_bonded(R, s, tl, nt, eflag, e);
+  if (bits & B_BMH) term_born(R, s, tl, nt, eflag, e);
+  if (bits & B_GINTRA) term_gintra(R, s, tl, nt, eflag, e);
+}
+template <typename Rl>
+__device__ void batch_terms_b(const DevReplica &R, const SStoreT<Rl> &s, const BatchLds<Rl> &B, int t, int nt,
+                              bool eflag, double *e) {
+  const uint32_t bits = R.flags;
+  dense_heavy<Rl>(R, B, s.L, t, 0, nt, kDenseNotIonOnly, eflag, e);
+  if (bits & B_GG) term_gg<Rl>(R, s, t, nt, eflag, e);
+  if (bits & B_GV) term_gv<Rl>(R, s, t, nt, eflag, e);
+  if (bits & B_VINTRA) term_vintra(R, s, t, nt, eflag, e);
+}
+
 template <typename Rl>
 __device__ inline void batch_load(const DevReplica &R, const LView &L, const BatchLds<Rl> &B) {
   const double C = 0.5 * R.dt * kKickUnit;
   if (R.flags & B_AMBER)
     for (int k = threadIdx.x; k < L.n_ion * R.amber_words; k += blockDim.x) B.allow[k] = R.amber_allow[k];
+  if (threadIdx.x == 0) shake_lds_fill(R, R.sp[0], B.shk, 0);
+  if (threadIdx.x == blockDim.x - 1) shake_lds_fill(R, R.sp[2], B.shk, 1);
   const int nkt = L.n_ion + L.na_v + L.na_g;
   for (int k = threadIdx.x; k < nkt; k += blockDim.x) {
     int sp = k < L.n_ion ? 0 : (k < L.n_ion + L.na_v ? 2 : 1);
     int idx = sp == 0 ? k : (sp == 2 ? k - L.n_ion : k - L.n_ion - L.na_v);
     const DevSpecies &S = R.sp[sp];
     B.kct[k] = C / S.mass[idx];
+    B.kms[k] = S.mass[idx];
     B.kpk[k].x = (Rl)S.charge[idx];
     B.kpk[k].y = (Rl)(S.se[idx] * kSqrt24);
     B.kpk[k].z = (Rl)(0.5 * S.sg[idx]);
@@ -642,9 +696,9 @@ template <typename Rl>
 __global__ void __launch_bounds__(512) k_batched(const DevReplica *__restrict__ reps,
                                                  const int *__restrict__ order, int n_rep, int nsteps,
                                                  int eflag_every, long long step0, int cap, int cap_h, int kcap,
-                                                 int acap) {
+                                                 int acap, int scap) {
   extern __shared__ __align__(16) char smem[];
-  BatchLds<Rl> B = carve<Rl>(smem, cap, cap_h, kcap, acap);
+  BatchLds<Rl> B = carve<Rl>(smem, cap, cap_h, kcap, acap, scap);
   const int tid = threadIdx.x, nt = blockDim.x;
   for (int rr = blockIdx.x; rr < n_rep; rr += gridDim.x) {
     const DevReplica &R = reps[order[rr]];
@@ -681,80 +735,128 @@ __global__ void __launch_bounds__(512) k_batched(const DevReplica *__restrict__ 
         }
       }
       __syncthreads();
-      if (shake) {
-        // constrained molecules: the ion and the vapor (gas has no rigid pairs)
-        for (int m = tid; m <= L.nv; m += nt) {
-          int sp = m == 0 ? 0 : 2, base = m == 0 ? 0 : L.vbase + (m - 1) * L.na_v;
-          const DevSpecies &S = R.sp[sp];
-          if (!S.nrigid) continue;
-          double *x = B.x + 3 * base, *v = B.v + 3 * base;
-          const double *xo = B.xo + 3 * base;
-          auto X = [&](int i, int q) -> double & { return x[3 * i + q]; };
-          auto XO = [&](int i, int q) -> double { return xo[3 * i + q]; };
-          auto V = [&](int i, int q) -> double & { return v[3 * i + q]; };
-          int it = shake_molecule(R, S, X, XO, R.shake_max_iter);
-          if (it < 0) {
-            atomicAdd(&B.sint[33], 1);
-            atomicMax(&B.sint[32], -it);
-            shake_undo(S, X, XO, V, R.dt);
-            continue;
-          }
-          atomicMax(&B.sint[32], it);
-          if (R.shake == 1) shake_velocity(S, X, XO, V, R.dt);
-        }
-        __syncthreads();
-      }
-      // ion COM (verlet.cpp:90-107)
-      {
-        double c[6] = {0, 0, 0, 0, 0, 0};
-        for (int i = tid; i < I.natoms; i += nt) {
-          double m = I.mass[i];
-          for (int q = 0; q < 3; q++) { c[q] += B.x[3 * i + q] * m; c[3 + q] += B.v[3 * i + q] * m; }
-        }
-        for (int k = 0; k < 6; k++) {
-          double sv = wave_sum(c[k]);
-          if ((tid & 63) == 0) B.red[k * 16 + (tid >> 6)] = sv;
-        }
-        __syncthreads();
-        if (tid < 6) {
-          double sv = 0;
-          for (int w = 0; w < nt / 64; w++) sv += B.red[tid * 16 + w];
-          B.sdbl[tid] = sv * (1 / I.molmass);
-        }
-        __syncthreads();
-      }
-      const double cx = B.sdbl[0], cy = B.sdbl[1], cz = B.sdbl[2];
-      // blend weights (verlet.cpp:109-134)
-      for (int a = L.n_ion + tid; a < L.n_in; a += nt) {
-        double dx = B.x[3 * a] - cx, dy = B.x[3 * a + 1] - cy, dz = B.x[3 * a + 2] - cz;
-        B.wr[a] = blend_weight_r<Rl>(dx * dx + dy * dy + dz * dz);
-      }
-      // check_pairlist (pairlist.cpp:58-63)
-      lp++;
-      if (lp > lu) {
-        __syncthreads();
-        batch_store(R, L, B);
-        if (tid < 3) { R.mol_x[tid] = B.sdbl[tid]; R.mol_v[tid] = B.sdbl[3 + tid]; }
-        if (tid == 0) D->loopPair = lp;
-        __syncthreads();
-        rebuild_wg(R, B.sint, B.red);
-        L = view_from_dyn(R);
-        lp = D->loopPair;
-        lu = D->loop_update;
-        if (L.n_in > cap || L.gbase > cap_h) {
-          if (tid == 0) atomicOr(&D->status, ST_LDS_CAP);
-          dead = true;
-          break;
-        }
-        batch_load(R, L, B);
-      }
-      __syncthreads();
-      // forces of every registered potential (verlet.cpp:31-32)
       SStoreT<Rl> ss;
       ss.L = L; ss.px = B.x; ss.pf = B.f; ss.pk = B.pk; ss.kpk = B.kpk; ss.kidx = B.kidx; ss.wr = B.wr;
       ss.ptype = R.sp[0].type;
       double e[U_N] = {0, 0, 0, 0, 0, 0, 0, 0};
-      batch_terms<Rl>(R, ss, B, tid, nt, eflag, e);
+      // SHAKE of one molecule whose first local atom is base
+      auto shake_one = [&](const DevSpecies &S, int base) {
+        double *x = B.x + 3 * base, *v = B.v + 3 * base;
+        const double *xo = B.xo + 3 * base;
+        auto X = [&](int i, int q) -> double & { return x[3 * i + q]; };
+        auto XO = [&](int i, int q) -> double { return xo[3 * i + q]; };
+        auto V = [&](int i, int q) -> double & { return v[3 * i + q]; };
+        const int sidx = &S == &R.sp[0] ? 0 : 1;
+        int it = shake_molecule_lds(R, B.shk, sidx, X, XO, R.shake_max_iter);
+        if (it < 0) {
+          atomicAdd(&B.sint[33], 1);
+          atomicMax(&B.sint[32], -it);
+          shake_undo(S, X, XO, V, R.dt);
+          return;
+        }
+        atomicMax(&B.sint[32], it);
+        if (R.shake == 1) shake_velocity_lds(B.shk, sidx, S.natoms, X, XO, V, R.dt);
+      };
+      const DevSpecies &Vs = R.sp[2];
+      const int nwv = nt >> 6;
+      // A step with constrained vapor, an unconstrained ion and no rebuild
+      // due: the vapor SHAKE (one thread per molecule, latency-bound) runs
+      // on the last wave(s) while the other waves compute the ion COM, the
+      // gas weights and the forces that involve no vapor position (phase A);
+      // then the vapor weights and the remaining forces (phase B).  The
+      // forces are the same terms as in the sequential order; only their
+      // accumulation order differs.
+      const bool overlap = shake && Vs.nrigid && !I.nrigid && L.nv > 0 && nwv >= 2 && lp + 1 <= lu;
+      if (overlap) {
+        const int nS = min(nwv - 1, (L.nv + 63) >> 6), nA = nwv - nS;
+        const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+        if (wave >= nA) {
+          for (int m = tid - 64 * nA; m < L.nv; m += 64 * nS) shake_one(Vs, L.vbase + m * L.na_v);
+        } else {
+          // ion COM (verlet.cpp:90-107), every phase-A wave for itself
+          double c[6] = {0, 0, 0, 0, 0, 0};
+          for (int i = tid & 63; i < I.natoms; i += 64) {
+            double m = B.kms[i];
+            for (int q = 0; q < 3; q++) { c[q] += B.x[3 * i + q] * m; c[3 + q] += B.v[3 * i + q] * m; }
+          }
+          for (int k = 0; k < 6; k++) c[k] = wave_sum(c[k]) * (1 / I.molmass);
+          if (tid < 6) B.sdbl[tid] = c[tid];
+          // blend weights of the gas atoms (verlet.cpp:109-134)
+          for (int a = L.gbase + tid; a < L.n_in; a += 64 * nA) {
+            double dx = B.x[3 * a] - c[0], dy = B.x[3 * a + 1] - c[1], dz = B.x[3 * a + 2] - c[2];
+            B.wr[a] = blend_weight_r<Rl>(dx * dx + dy * dy + dz * dz);
+          }
+          batch_terms_a<Rl>(R, ss, B, tid, 0, 64 * nA, eflag, e);
+        }
+        __syncthreads();
+        {
+          const double cx = B.sdbl[0], cy = B.sdbl[1], cz = B.sdbl[2];
+          for (int a = L.n_ion + tid; a < L.gbase; a += nt) {
+            double dx = B.x[3 * a] - cx, dy = B.x[3 * a + 1] - cy, dz = B.x[3 * a + 2] - cz;
+            B.wr[a] = blend_weight_r<Rl>(dx * dx + dy * dy + dz * dz);
+          }
+        }
+        lp++;  // check_pairlist: no rebuild due (pairlist.cpp:58-63)
+        __syncthreads();
+        batch_terms_b<Rl>(R, ss, B, tid, nt, eflag, e);
+      } else {
+        if (shake) {
+          // constrained molecules: the vapor and the ion (gas has no rigid
+          // pairs)
+          if (Vs.nrigid)
+            for (int m = tid; m < L.nv; m += nt) shake_one(Vs, L.vbase + m * L.na_v);
+          if (I.nrigid && tid == nt - 1) shake_one(I, 0);
+          __syncthreads();
+        }
+        // ion COM (verlet.cpp:90-107)
+        {
+          double c[6] = {0, 0, 0, 0, 0, 0};
+          for (int i = tid; i < I.natoms; i += nt) {
+            double m = B.kms[i];
+            for (int q = 0; q < 3; q++) { c[q] += B.x[3 * i + q] * m; c[3 + q] += B.v[3 * i + q] * m; }
+          }
+          for (int k = 0; k < 6; k++) {
+            double sv = wave_sum(c[k]);
+            if ((tid & 63) == 0) B.red[k * 16 + (tid >> 6)] = sv;
+          }
+          __syncthreads();
+          if (tid < 6) {
+            double sv = 0;
+            for (int w = 0; w < nt / 64; w++) sv += B.red[tid * 16 + w];
+            B.sdbl[tid] = sv * (1 / I.molmass);
+          }
+          __syncthreads();
+        }
+        const double cx = B.sdbl[0], cy = B.sdbl[1], cz = B.sdbl[2];
+        // blend weights (verlet.cpp:109-134)
+        for (int a = L.n_ion + tid; a < L.n_in; a += nt) {
+          double dx = B.x[3 * a] - cx, dy = B.x[3 * a + 1] - cy, dz = B.x[3 * a + 2] - cz;
+          B.wr[a] = blend_weight_r<Rl>(dx * dx + dy * dy + dz * dz);
+        }
+        // check_pairlist (pairlist.cpp:58-63)
+        lp++;
+        if (lp > lu) {
+          __syncthreads();
+          batch_store(R, L, B);
+          if (tid < 3) { R.mol_x[tid] = B.sdbl[tid]; R.mol_v[tid] = B.sdbl[3 + tid]; }
+          if (tid == 0) D->loopPair = lp;
+          __syncthreads();
+          rebuild_wg(R, B.sint, B.red);
+          L = view_from_dyn(R);
+          lp = D->loopPair;
+          lu = D->loop_update;
+          if (L.n_in > cap || L.gbase > cap_h) {
+            if (tid == 0) atomicOr(&D->status, ST_LDS_CAP);
+            dead = true;
+            break;
+          }
+          batch_load(R, L, B);
+          ss.L = L;
+        }
+        __syncthreads();
+        // forces of every registered potential (verlet.cpp:31-32)
+        batch_terms<Rl>(R, ss, B, tid, nt, eflag, e);
+      }
       __syncthreads();
       // second half-kick (verlet.cpp:34)
       for (int a = tid; a < L.n_in; a += nt) {
@@ -1392,12 +1494,13 @@ int mdn_verlet_graph(mdn_ctx *c, int32_t r, int32_t nsteps, int32_t eflag_every,
 namespace mdn {
 // LDS capacities for a set of replicas from their current in-domain counts,
 // with headroom for molecules that enter during a launch
-struct BatchCaps { int cap, cap_h, kcap, acap; size_t lds; };
+struct BatchCaps { int cap, cap_h, kcap, acap, scap; size_t lds; };
 static BatchCaps batch_caps(const mdn_ctx *c, const std::vector<Dyn> &dyn, const std::vector<int> &set, size_t rl,
                             size_t budget) {
-  int need = 16, need_h = 16, most = 16, most_h = 16, kcap = 8, acap = 1;
+  int need = 16, need_h = 16, most = 16, most_h = 16, kcap = 8, acap = 1, scap = 1;
   for (int r : set) {
     const SysDef &d = c->reps[r].def;
+    scap = std::max(scap, (int)std::max(d.sp[0].rigid.size(), d.sp[2].rigid.size()) / 3);
     int nh = d.sp[0].natoms + dyn[r].n_in_vap * d.sp[2].natoms;
     need_h = std::max(need_h, nh);
     need = std::max(need, nh + dyn[r].n_in_gas * d.sp[1].natoms);
@@ -1416,20 +1519,21 @@ static BatchCaps batch_caps(const mdn_ctx *c, const std::vector<Dyn> &dyn, const
   int lo_h = std::min(most_h, r64(need_h + 32)), hi_h = std::min(most_h, r64(need_h * 3 / 2 + 32));
   int lo = std::min(most, r32(need + need / 8 + 64)), hi = std::min(most, r32(need * 3 / 2 + 64));
   BatchCaps b;
-  b.kcap = kcap; b.acap = acap;
+  kcap += kcap & 1;  // keeps the LDS arrays after it 16-byte aligned
+  b.kcap = kcap; b.acap = acap; b.scap = scap;
   if (c->batch_cap > 0) {
     b.cap = r32(c->batch_cap);
     b.cap_h = std::min(hi_h, b.cap & ~63);
   } else {
     b.cap = hi; b.cap_h = hi_h;
-    while (batch_lds_bytes(std::max(b.cap, b.cap_h), b.cap_h, kcap, acap, rl) > budget &&
+    while (batch_lds_bytes(std::max(b.cap, b.cap_h), b.cap_h, kcap, acap, scap, rl) > budget &&
            (b.cap > lo || b.cap_h > lo_h)) {
       if (b.cap > lo) b.cap -= 32;
       if (b.cap_h > lo_h) b.cap_h -= 64;
     }
   }
   b.cap = std::max(b.cap, b.cap_h);
-  b.lds = batch_lds_bytes(b.cap, b.cap_h, kcap, acap, rl);
+  b.lds = batch_lds_bytes(b.cap, b.cap_h, kcap, acap, scap, rl);
   return b;
 }
 
@@ -1444,7 +1548,7 @@ static int launch_batched(mdn_ctx *c, hipStream_t st, const int *order, int n, i
   HIPCHK(hipFuncSetAttribute((const void *)k_batched<Rl>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
   int grid = c->batch_grid > 0 ? std::min(c->batch_grid, n) : n;
   hipLaunchKernelGGL(k_batched<Rl>, dim3(grid), dim3(threads), b.lds, st, c->d_reps, order, n, ns, eflag_every,
-                     step0, b.cap, b.cap_h, b.kcap, b.acap);
+                     step0, b.cap, b.cap_h, b.kcap, b.acap, b.scap);
   HIPCHK(hipGetLastError());
   return MDN_OK;
 }

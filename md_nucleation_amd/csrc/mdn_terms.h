@@ -620,6 +620,108 @@ __device__ int shake_molecule(const DevReplica &R, const DevSpecies &Sp, XA x, R
   return it;
 }
 
+// Constraint tables of the two constrained species (ion 0, vapor 1), staged
+// in LDS once per launch: [2][cap] endpoints, d^2, inverse masses and the
+// factor multiplying (d^2 - r'^2) (mode 2: 0.5 mred_inv, shake.cpp:67;
+// mode 1: 0.5 / mred_inv), and each species' constrained atoms (bit a of
+// cons, molecules of up to 64 atoms).
+struct ShakeLds {
+  int *n;        // [2]
+  int *ij;       // [2][cap][2]
+  double *c;     // [2][cap][4]: d^2, 1/m_i, 1/m_j, factor
+  unsigned long long *cons;  // [2]
+  int cap;
+};
+
+__device__ inline void shake_lds_fill(const DevReplica &R, const DevSpecies &Sp, const ShakeLds &T, int s) {
+  const int n = Sp.nrigid < T.cap ? Sp.nrigid : T.cap;
+  unsigned long long cons = 0;
+  for (int k = 0; k < n; k++) {
+    int i = Sp.rigid[3 * k], j = Sp.rigid[3 * k + 1];
+    double dk = R.btype[2 * Sp.rigid[3 * k + 2] + 1];
+    double mi = Sp.invmass[i], mj = Sp.invmass[j];
+    int *ij = T.ij + 2 * ((size_t)s * T.cap + k);
+    double *c = T.c + 4 * ((size_t)s * T.cap + k);
+    ij[0] = i; ij[1] = j;
+    c[0] = dk * dk; c[1] = mi; c[2] = mj;
+    c[3] = R.shake == 2 ? 0.5 * (mi + mj) : 0.5 / (mi + mj);
+    if (i < 64) cons |= 1ull << i;
+    if (j < 64) cons |= 1ull << j;
+  }
+  T.n[s] = Sp.nrigid;
+  T.cons[s] = cons;
+}
+
+// shake_molecule with the constraint data (and the old bond vectors) read
+// from LDS in every sweep: one code shape for any constraint count
+template <class XA, class RA>
+__device__ int shake_molecule_lds(const DevReplica &R, const ShakeLds &T, int s, XA x, RA r, int max_iter) {
+  const int nr = T.n[s];
+  const int *ij = T.ij + 2 * (size_t)s * T.cap;
+  const double *cc = T.c + 4 * (size_t)s * T.cap;
+  const double tolerance = 1e-20, rho_zero = 0.01;
+  int it = 0;
+  bool converge = false;
+  while (!converge) {
+    if (it >= max_iter) return -it;
+    it++;
+    converge = true;
+    for (int k = 0; k < nr; k++) {
+      const int i = ij[2 * k], j = ij[2 * k + 1];
+      const double dk2 = cc[4 * k], mi = cc[4 * k + 1], mj = cc[4 * k + 2], hm = cc[4 * k + 3];
+      double rk[3], rn[3];
+      for (int q = 0; q < 3; q++) { rk[q] = r(i, q) - r(j, q); rn[q] = x(i, q) - x(j, q); }
+      double rn2 = rn[0] * rn[0] + rn[1] * rn[1] + rn[2] * rn[2];
+      double dot = rk[0] * rn[0] + rk[1] * rn[1] + rk[2] * rn[2];
+      if (R.shake != 2) {
+        double d0 = rn2 - dk2;
+        if (d0 * d0 > tolerance) converge = false;
+      }
+      double ramk = (dk2 - rn2) * hm;
+      // 1 / dot by the hardware reciprocal estimate refined with two
+      // Newton-Raphson steps (r <- r + r (1 - dot r)): within an ulp of the
+      // quotient, without the dependent chain of a correctly rounded divide
+      if (dot > 1e-10) ramk *= rcp_nr(dot);
+      else {
+        double rk2 = rk[0] * rk[0] + rk[1] * rk[1] + rk[2] * rk[2];
+        ramk = 0.5 * rho_zero + ramk / (rk2 * rho_zero) * (mi + mj);
+      }
+      double cfi = ramk * mi, cfj = ramk * mj;
+      for (int q = 0; q < 3; q++) {
+        double ni = x(i, q) + rk[q] * cfi, nj = x(j, q) - rk[q] * cfj;
+        x(i, q) = ni;
+        x(j, q) = nj;
+        rn[q] = ni - nj;
+      }
+      rn2 = rn[0] * rn[0] + rn[1] * rn[1] + rn[2] * rn[2];
+      double dr = rn2 - dk2;
+      if (!(dr * dr <= 1e30)) return -it;
+      if (R.shake == 2 && dr * dr > tolerance) converge = false;
+    }
+  }
+  return it;
+}
+
+// shake_velocity over the atoms the table marks, (x - x_old) * (1 / dt)
+// (the same as the quotient when dt is a power of two)
+template <class XA, class RA, class VA>
+__device__ void shake_velocity_lds(const ShakeLds &T, int s, int natoms, XA x, RA r, VA v, double dt) {
+  const double inv_dt = 1.0 / dt;
+  const unsigned long long cons = T.cons[s];
+  const int nr = T.n[s];
+  const int *ij = T.ij + 2 * (size_t)s * T.cap;
+  for (int a = 0; a < natoms; a++) {
+    bool c;
+    if (a < 64) c = (cons >> a) & 1ull;
+    else {
+      c = false;
+      for (int k = 0; k < nr && !c; k++) c = ij[2 * k] == a || ij[2 * k + 1] == a;
+    }
+    if (!c) continue;
+    for (int q = 0; q < 3; q++) v(a, q) = (x(a, q) - r(a, q)) * inv_dt;
+  }
+}
+
 // a failed SHAKE leaves the molecule unconstrained for this step (the
 // reference, which never calls shake(), always does): x = x_old + v dt
 template <class XA, class RA, class VA>
