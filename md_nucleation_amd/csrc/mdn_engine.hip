@@ -420,20 +420,38 @@ struct DenseI {
 
 // one pair (i, j): g = minus the pair force vector (F_j += g, F_i -= g),
 // zero for masked pairs; energies by reference label
-template <typename Rl, bool EF>
+// Block-pair classes: both blocks whole ion, a whole-ion block with a
+// whole-vapor block, both blocks vapor, or a block that holds both (the
+// general selection).  The class is known per work item, so the specialised
+// forms drop the selection work that cannot change within the item.
+enum { kPairMixed = 0, kPairIonIon = 1, kPairIonVap = 2, kPairVapVap = 3 };
+
+template <typename Rl, bool EF, int CLS>
 __device__ inline void dense_pair(const BatchLds<Rl> &B, const DenseI<Rl> &c, int j, uint32_t valid_k, double &gx,
                                   double &gy, double &gz, double &e_ion, double &e_vi, double &e_vv) {
   const typename Pack4<Rl>::T pj = B.pk[j];
   const double *xq = B.x + __umul24((unsigned)j, 3u);
   const double dx = c.x0 - xq[0], dy = c.x1 - xq[1], dz = c.x2 - xq[2];
   const Rl wj = B.wr[j];
-  const uint32_t abit = (c.arow[min(j >> 5, c.wmax)] >> (j & 31)) & 1u;
   // pair selection as integer masks (no short-circuit control flow)
-  const uint32_t io_j = j < c.ni ? 1u : 0u;
-  const uint32_t both = c.ion & io_j, cross = c.ion ^ io_j;
-  const uint32_t diff_mol = pj.w != c.mi ? 1u : 0u;
-  const uint32_t pass = (both & c.am & abit) | (cross & c.vi) | ((1u ^ (both | cross)) & c.vv & diff_mol);
-  const uint32_t on = c.ok & (j < c.nh ? 1u : 0u) & valid_k & pass;
+  uint32_t both, cross, pass;
+  if (CLS == kPairIonIon) {
+    both = 1u; cross = 0u;
+    pass = c.am & (c.arow[min(j >> 5, c.wmax)] >> (j & 31)) & 1u;
+  } else if (CLS == kPairIonVap) {
+    both = 0u; cross = 1u;
+    pass = c.vi;
+  } else if (CLS == kPairVapVap) {
+    both = 0u; cross = 0u;
+    pass = c.vv & (pj.w != c.mi ? 1u : 0u);
+  } else {
+    const uint32_t abit = (c.arow[min(j >> 5, c.wmax)] >> (j & 31)) & 1u;
+    const uint32_t io_j = j < c.ni ? 1u : 0u;
+    both = c.ion & io_j; cross = c.ion ^ io_j;
+    const uint32_t diff_mol = pj.w != c.mi ? 1u : 0u;
+    pass = (both & c.am & abit) | (cross & c.vi) | ((1u ^ (both | cross)) & c.vv & diff_mol);
+  }
+  const uint32_t on = (CLS == kPairIonIon ? 1u : c.ok & (j < c.nh ? 1u : 0u)) & valid_k & pass;
   const Rl m = on ? (Rl)1 : (Rl)0;
   const Rl rx = (Rl)dx, ry = (Rl)dy, rz = (Rl)dz;
   const Rl r2 = rx * rx + ry * ry + rz * rz;
@@ -443,7 +461,7 @@ __device__ inline void dense_pair(const BatchLds<Rl> &B, const DenseI<Rl> &c, in
   // c1) is 24 eps u3 (2 u3 - 1)
   const Rl e24 = c.sei * pj.y;
   const Rl sg = c.sgi + pj.z;
-  const Rl w = both ? (Rl)1 : c.wi * wj;
+  const Rl w = CLS == kPairIonIon ? (Rl)1 : (CLS == kPairMixed && both ? (Rl)1 : c.wi * wj);
   Rl r2inv, rinv;
   inv_r<Rl>(rsq, r2inv, rinv);
   const Rl u = sg * sg * r2inv, u3 = u * u * u;
@@ -458,6 +476,27 @@ __device__ inline void dense_pair(const BatchLds<Rl> &B, const DenseI<Rl> &c, in
     e_ion += both ? ep : 0.0;
     e_vi += cross ? ep : 0.0;
     e_vv += (both | cross) ? 0.0 : ep;
+  }
+}
+
+// the off-diagonal run of one work item: partners k and k + 32 together
+// (two independent chains per lane), each with its travelling F_j
+// accumulator; returns through the references
+template <typename Rl, bool EF, int CLS>
+__device__ inline void dense_run_off(const BatchLds<Rl> &B, const DenseI<Rl> &c, int lane, int bj, int kb, int hL,
+                                     double &fx, double &fy, double &fz, double *a0, double *a1, double &e_ion,
+                                     double &e_vi, double &e_vv) {
+  for (int k = kb; k < kb + hL; k++) {
+    double gx, gy, gz, hx, hy, hz;
+    dense_pair<Rl, EF, CLS>(B, c, 64 * bj + ((lane + k) & 63), 1u, gx, gy, gz, e_ion, e_vi, e_vv);
+    dense_pair<Rl, EF, CLS>(B, c, 64 * bj + ((lane + k + 32) & 63), 1u, hx, hy, hz, e_ion, e_vi, e_vv);
+    fx -= gx + hx; fy -= gy + hy; fz -= gz + hz;
+    a0[0] = wave_rol1(a0[0] + gx);
+    a0[1] = wave_rol1(a0[1] + gy);
+    a0[2] = wave_rol1(a0[2] + gz);
+    a1[0] = wave_rol1(a1[0] + hx);
+    a1[1] = wave_rol1(a1[1] + hy);
+    a1[2] = wave_rol1(a1[2] + hz);
   }
 }
 
@@ -521,8 +560,8 @@ __device__ inline void dense_n3(const DevReplica &R, const BatchLds<Rl> &B, int 
           const uint32_t half_ok = lane < 32 ? 1u : 0u;
           for (int k = kb; k < kb + L; k++) {
             double gx, gy, gz;
-            dense_pair<Rl, EF>(B, c, 64 * bj + ((lane + k) & 63), k != 32 ? 1u : half_ok, gx, gy, gz, e_ion, e_vi,
-                               e_vv);
+            dense_pair<Rl, EF, kPairMixed>(B, c, 64 * bj + ((lane + k) & 63), k != 32 ? 1u : half_ok, gx, gy, gz,
+                                           e_ion, e_vi, e_vv);
             fx -= gx; fy -= gy; fz -= gz;
             a0x = wave_rol1(a0x + gx);
             a0y = wave_rol1(a0y + gy);
@@ -533,18 +572,22 @@ __device__ inline void dense_n3(const DevReplica &R, const BatchLds<Rl> &B, int 
           // two independent partners per step, k and k + 32, each with its
           // own travelling F_j accumulator
           const int hL = L >> 1, kb = part * hL;
-          for (int k = kb; k < kb + hL; k++) {
-            double gx, gy, gz, hx, hy, hz;
-            dense_pair<Rl, EF>(B, c, 64 * bj + ((lane + k) & 63), 1u, gx, gy, gz, e_ion, e_vi, e_vv);
-            dense_pair<Rl, EF>(B, c, 64 * bj + ((lane + k + 32) & 63), 1u, hx, hy, hz, e_ion, e_vi, e_vv);
-            fx -= gx + hx; fy -= gy + hy; fz -= gz + hz;
-            a0x = wave_rol1(a0x + gx);
-            a0y = wave_rol1(a0y + gy);
-            a0z = wave_rol1(a0z + gz);
-            a1x = wave_rol1(a1x + hx);
-            a1y = wave_rol1(a1y + hy);
-            a1z = wave_rol1(a1z + hz);
-          }
+          double a0[3] = {0, 0, 0}, a1[3] = {0, 0, 0};
+          // the item's class (bi < bj): whole-ion blocks end at NBi; a
+          // block at or past the ion's end holds vapor only
+          const int cls = bj < NBi ? kPairIonIon
+                                   : (bi < NBi && 64 * bj >= ni ? kPairIonVap
+                                                               : (64 * bi >= ni ? kPairVapVap : kPairMixed));
+          if (cls == kPairIonIon)
+            dense_run_off<Rl, EF, kPairIonIon>(B, c, lane, bj, kb, hL, fx, fy, fz, a0, a1, e_ion, e_vi, e_vv);
+          else if (cls == kPairIonVap)
+            dense_run_off<Rl, EF, kPairIonVap>(B, c, lane, bj, kb, hL, fx, fy, fz, a0, a1, e_ion, e_vi, e_vv);
+          else if (cls == kPairVapVap)
+            dense_run_off<Rl, EF, kPairVapVap>(B, c, lane, bj, kb, hL, fx, fy, fz, a0, a1, e_ion, e_vi, e_vv);
+          else
+            dense_run_off<Rl, EF, kPairMixed>(B, c, lane, bj, kb, hL, fx, fy, fz, a0, a1, e_ion, e_vi, e_vv);
+          a0x = a0[0]; a0y = a0[1]; a0z = a0[2];
+          a1x = a1[0]; a1y = a1[1]; a1z = a1[2];
           jend0 = 64 * bj + ((lane + kb + hL) & 63);
           jend1 = 64 * bj + ((lane + kb + hL + 32) & 63);
           atomicAdd(&pf[3 * jend1], a1x);
