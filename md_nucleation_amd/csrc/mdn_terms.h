@@ -215,9 +215,14 @@ __device__ inline void dihedral_term(const DevReplica &R, const S &s, int base, 
 template <class S>
 __device__ void term_amber_bonded(const DevReplica &R, const S &s, int t, int nt, bool eflag, double *e) {
   const DevSpecies &I = R.sp[0];
-  for (int p = t; p < I.nbond; p += nt) bond_term(R, s, 0, I.bond + 3 * p, eflag, e[U_ION]);
-  for (int p = t; p < I.nangle; p += nt) angle_term(R, s, 0, I.angle + 4 * p, eflag, e[U_ION]);
-  for (int p = t; p < I.ndih; p += nt) dihedral_term(R, s, 0, I.dih + 5 * p, eflag, e[U_ION]);
+  // one index space over bonds, angles and dihedrals, so that each thread
+  // takes about one term
+  const int nb = I.nbond, nba = nb + I.nangle, n = nba + I.ndih;
+  for (int p = t; p < n; p += nt) {
+    if (p < nb) bond_term(R, s, 0, I.bond + 3 * p, eflag, e[U_ION]);
+    else if (p < nba) angle_term(R, s, 0, I.angle + 4 * (p - nb), eflag, e[U_ION]);
+    else dihedral_term(R, s, 0, I.dih + 5 * (p - nba), eflag, e[U_ION]);
+  }
 }
 template <typename Rl, class S>
 __device__ void term_amber(const DevReplica &R, const S &s, int t, int nt, bool eflag, double *e) {
