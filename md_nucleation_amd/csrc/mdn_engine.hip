@@ -387,26 +387,31 @@ struct SStoreT {
 // AMBER non-excluded ion pairs (computeLong, compute_AMBER.cpp:21-49), vi
 // (VaporIon.cpp:5-43) and vv (VaporVapor.cpp:5-45) as one pass over the ion
 // + in-domain vapor atoms, each unordered pair evaluated once (Newton's third
-// law).  The atoms are cut into 64-atom blocks; a wave takes a block pair
-// (bi <= bj) with lane l owning atom i = 64 bi + l and, at step k, partner
-// j = 64 bj + (l + k) mod 64, so the 64 lanes always touch 64 different j.
+// law).  The ion range [0, ni) and the vapor range [ni, nh) are each cut into
+// 64-atom blocks (the last one of each range partial), and the work units are
+// block pairs of three kinds: ion-ion (bi <= bj), ion-vapor (every ion block
+// with every vapor block) and vapor-vapor (bi <= bj).  A wave takes a unit
+// with lane l owning atom i = start_i + l and, at step k, partner
+// j = start_j + (l + k) mod 64, so the 64 lanes always touch 64 different j.
 // F_i accumulates in FP64 registers.  F_j accumulates in FP64 registers that
 // travel with their atom: the partner of lane l at step k + 1 is the partner
 // of lane l + 1 at step k, so after each step every lane takes the
 // accumulator of lane l + 1 (a DPP wave rotate), and each accumulator is
-// added to LDS once at the end of a work item.  A diagonal block pair runs
+// added to LDS once at the end of a work item.  A diagonal unit runs
 // k = 1..32 (k = 32 on half the lanes), which visits each of its pairs once;
 // an off-diagonal one runs k and k + 32 together (two independent chains per
-// lane), k = 0..31.  A per-lane mask selects the
-// pairs the reference evaluates: ion-ion by the AMBER exclusion bitmask,
-// ion-vapor by vi, vapor-vapor of distinct molecules by vv.
+// lane), k = 0..31.  Ion-ion pairs are selected by the AMBER exclusion
+// bitmask, ion-vapor pairs by vi, vapor-vapor pairs of distinct molecules by
+// vv.  For an ion of a few atoms the ion-vapor units are instead one per
+// vapor block: each lane holds a vapor atom and loops over the ion atoms,
+// whose forces are wave sums.
 //
-// Work items are runs of L pairs per lane of one block pair (L = 32, 16 or
-// 8, whichever gives the waves the fewest rounds), dealt round-robin to the
-// waves, so the waves finish together.  The LDS atom arrays are padded to whole
-// 64-atom blocks with finite values (batch_load), so every lane loads and
-// adds unconditionally and the loop body has no branches: masked pairs carry
-// a zero force.
+// Work items are runs of L pairs per lane of one unit (L = 32, 16 or 8,
+// whichever gives the waves the fewest rounds), dealt round-robin to the
+// waves, so the waves finish together.  The LDS atom arrays hold finite
+// values one block past the ion + vapor range (batch_load), so every lane
+// loads and adds unconditionally and the loop body has no branches: masked
+// pairs carry a zero force.
 
 // the work item's atom i (lane) and pair-selection flags
 template <typename Rl>
@@ -414,17 +419,15 @@ struct DenseI {
   double x0, x1, x2;
   Rl qi, sei, sgi, mi, wi;
   const uint32_t *arow;
-  uint32_t ok, ion, am, vi, vv;
-  int ni, nh, wmax;
+  uint32_t ok, am, vi, vv;
+  int jlim, wmax;  // partners j < jlim are real
 };
 
 // one pair (i, j): g = minus the pair force vector (F_j += g, F_i -= g),
 // zero for masked pairs; energies by reference label
-// Block-pair classes: both blocks whole ion, a whole-ion block with a
-// whole-vapor block, both blocks vapor, or a block that holds both (the
-// general selection).  The class is known per work item, so the specialised
-// forms drop the selection work that cannot change within the item.
-enum { kPairMixed = 0, kPairIonIon = 1, kPairIonVap = 2, kPairVapVap = 3 };
+// Unit kinds: the pair selection that cannot change within a unit is
+// compiled out.
+enum { kPairIonIon = 1, kPairIonVap = 2, kPairVapVap = 3 };
 
 template <typename Rl, bool EF, int CLS>
 __device__ inline void dense_pair(const BatchLds<Rl> &B, const DenseI<Rl> &c, int j, uint32_t valid_k, double &gx,
@@ -434,24 +437,11 @@ __device__ inline void dense_pair(const BatchLds<Rl> &B, const DenseI<Rl> &c, in
   const double dx = c.x0 - xq[0], dy = c.x1 - xq[1], dz = c.x2 - xq[2];
   const Rl wj = B.wr[j];
   // pair selection as integer masks (no short-circuit control flow)
-  uint32_t both, cross, pass;
-  if (CLS == kPairIonIon) {
-    both = 1u; cross = 0u;
-    pass = c.am & (c.arow[min(j >> 5, c.wmax)] >> (j & 31)) & 1u;
-  } else if (CLS == kPairIonVap) {
-    both = 0u; cross = 1u;
-    pass = c.vi;
-  } else if (CLS == kPairVapVap) {
-    both = 0u; cross = 0u;
-    pass = c.vv & (pj.w != c.mi ? 1u : 0u);
-  } else {
-    const uint32_t abit = (c.arow[min(j >> 5, c.wmax)] >> (j & 31)) & 1u;
-    const uint32_t io_j = j < c.ni ? 1u : 0u;
-    both = c.ion & io_j; cross = c.ion ^ io_j;
-    const uint32_t diff_mol = pj.w != c.mi ? 1u : 0u;
-    pass = (both & c.am & abit) | (cross & c.vi) | ((1u ^ (both | cross)) & c.vv & diff_mol);
-  }
-  const uint32_t on = (CLS == kPairIonIon ? 1u : c.ok & (j < c.nh ? 1u : 0u)) & valid_k & pass;
+  uint32_t pass;
+  if (CLS == kPairIonIon) pass = c.am & (c.arow[min(j >> 5, c.wmax)] >> (j & 31)) & 1u;
+  else if (CLS == kPairIonVap) pass = c.vi;
+  else pass = c.vv & (pj.w != c.mi ? 1u : 0u);
+  const uint32_t on = c.ok & (j < c.jlim ? 1u : 0u) & valid_k & pass;
   const Rl m = on ? (Rl)1 : (Rl)0;
   const Rl rx = (Rl)dx, ry = (Rl)dy, rz = (Rl)dz;
   const Rl r2 = rx * rx + ry * ry + rz * rz;
@@ -461,7 +451,7 @@ __device__ inline void dense_pair(const BatchLds<Rl> &B, const DenseI<Rl> &c, in
   // c1) is 24 eps u3 (2 u3 - 1)
   const Rl e24 = c.sei * pj.y;
   const Rl sg = c.sgi + pj.z;
-  const Rl w = CLS == kPairIonIon ? (Rl)1 : (CLS == kPairMixed && both ? (Rl)1 : c.wi * wj);
+  const Rl w = CLS == kPairIonIon ? (Rl)1 : c.wi * wj;
   Rl r2inv, rinv;
   inv_r<Rl>(rsq, r2inv, rinv);
   const Rl u = sg * sg * r2inv, u3 = u * u * u;
@@ -473,9 +463,7 @@ __device__ inline void dense_pair(const BatchLds<Rl> &B, const DenseI<Rl> &c, in
   gz = (double)(fn * rz);
   if (EF) {
     const double ep = (double)((e24 * (Rl)(1.0 / 6.0) * u3 * (u3 - (Rl)1) + fc) * m);
-    e_ion += both ? ep : 0.0;
-    e_vi += cross ? ep : 0.0;
-    e_vv += (both | cross) ? 0.0 : ep;
+    (CLS == kPairIonIon ? e_ion : CLS == kPairIonVap ? e_vi : e_vv) += ep;
   }
 }
 
@@ -483,13 +471,13 @@ __device__ inline void dense_pair(const BatchLds<Rl> &B, const DenseI<Rl> &c, in
 // (two independent chains per lane), each with its travelling F_j
 // accumulator; returns through the references
 template <typename Rl, bool EF, int CLS>
-__device__ inline void dense_run_off(const BatchLds<Rl> &B, const DenseI<Rl> &c, int lane, int bj, int kb, int hL,
+__device__ inline void dense_run_off(const BatchLds<Rl> &B, const DenseI<Rl> &c, int lane, int jb, int kb, int hL,
                                      double &fx, double &fy, double &fz, double *a0, double *a1, double &e_ion,
                                      double &e_vi, double &e_vv) {
   for (int k = kb; k < kb + hL; k++) {
     double gx, gy, gz, hx, hy, hz;
-    dense_pair<Rl, EF, CLS>(B, c, 64 * bj + ((lane + k) & 63), 1u, gx, gy, gz, e_ion, e_vi, e_vv);
-    dense_pair<Rl, EF, CLS>(B, c, 64 * bj + ((lane + k + 32) & 63), 1u, hx, hy, hz, e_ion, e_vi, e_vv);
+    dense_pair<Rl, EF, CLS>(B, c, jb + ((lane + k) & 63), 1u, gx, gy, gz, e_ion, e_vi, e_vv);
+    dense_pair<Rl, EF, CLS>(B, c, jb + ((lane + k + 32) & 63), 1u, hx, hy, hz, e_ion, e_vi, e_vv);
     fx -= gx + hx; fy -= gy + hy; fz -= gz + hz;
     a0[0] = wave_rol1(a0[0] + gx);
     a0[1] = wave_rol1(a0[1] + gy);
@@ -504,9 +492,13 @@ __device__ inline void dense_run_off(const BatchLds<Rl> &B, const DenseI<Rl> &c,
 // pairs: no vapor position or weight involved), or all but those.
 enum { kDenseAll = 0, kDenseIonOnly = 1, kDenseNotIonOnly = 2 };
 
+// ions of at most this many atoms take the one-unit-per-vapor-block form
+constexpr int kIonBroadcastMax = 16;
+
+// returns the number of work items (dealt round-robin from wave 0)
 template <typename Rl, bool EF>
-__device__ inline void dense_n3(const DevReplica &R, const BatchLds<Rl> &B, int nh, int ni, bool am, bool vi, bool vv,
-                                int t, int t0, int nt, int filt, double *e) {
+__device__ inline int dense_n3(const DevReplica &R, const BatchLds<Rl> &B, int nh, int ni, bool am, bool vi, bool vv,
+                               int t, int t0, int nt, int filt, double *e) {
   const Rl qq = (Rl)R.qqrd2e;
   // threads [t0, t0 + nt) take part (whole waves).  The wave index and the
   // counts are uniform within a wave: say so, so that the item loops and
@@ -515,112 +507,136 @@ __device__ inline void dense_n3(const DevReplica &R, const BatchLds<Rl> &B, int 
   const int nw = __builtin_amdgcn_readfirstlane(nt >> 6);
   nh = __builtin_amdgcn_readfirstlane(nh);
   ni = __builtin_amdgcn_readfirstlane(ni);
-  const int NB = (nh + 63) >> 6, NBi = ni >> 6;  // blocks; whole-ion blocks
-  auto keep = [&](int bi, int bj) {
-    const bool ion_only = bj < NBi;  // bi <= bj
-    return filt == kDenseAll || (filt == kDenseIonOnly) == ion_only;
-  };
-  int n_diag = 0, n_off = 0;
-  for (int bi = 0; bi < NB; bi++)
-    for (int bj = bi; bj < NB; bj++)
-      if (keep(bi, bj)) (bi == bj ? n_diag : n_off)++;
+  const int NBI = (ni + 63) >> 6, NBV = (nh - ni + 63) >> 6;
+  const bool doII = am && ni >= 2 && filt != kDenseNotIonOnly;
+  const bool doIV = vi && ni > 0 && NBV > 0 && filt != kDenseIonOnly;
+  const bool doVV = vv && NBV > 0 && filt != kDenseIonOnly;
+  const bool bcast = ni <= kIonBroadcastMax;
+  const int n_diag = (doII ? NBI : 0) + (doVV ? NBV : 0);
+  const int n_off = (doII ? NBI * (NBI - 1) / 2 : 0) + (doIV && !bcast ? NBI * NBV : 0) +
+                    (doVV ? NBV * (NBV - 1) / 2 : 0);
+  const int n_bc = doIV && bcast ? NBV : 0;
   // run length: the fewest rounds of items per wave, counting about six
   // steps of set-up and flush per item
   int L = 32, best = 1 << 30;
   for (int l = 32; l >= 8; l >>= 1) {
-    const int items = n_diag * (32 / l) + n_off * (64 / l), c = (items + nw - 1) / nw * (l + 6);
+    const int items = n_diag * (32 / l) + n_off * (64 / l) + n_bc, c = (items + nw - 1) / nw * (l + 6);
     if (c < best) { best = c; L = l; }
   }
   const int W = __builtin_amdgcn_readfirstlane(R.amber_words);
   double *__restrict__ pf = B.f;
   double e_ion = 0, e_vi = 0, e_vv = 0;
-  int item = 0;
-  for (int bi = 0; bi < NB; bi++) {
-    const int i = 64 * bi + lane;
-    for (int bj = bi; bj < NB; bj++) {
-      if (!keep(bi, bj)) continue;
-      const bool diag = bi == bj;
-      const int np = (diag ? 32 : 64) / L;
-      for (int part = 0; part < np; part++, item++) {
-        if (item % nw != wave) continue;
-        DenseI<Rl> c;
-        c.x0 = B.x[3 * i]; c.x1 = B.x[3 * i + 1]; c.x2 = B.x[3 * i + 2];
-        const typename Pack4<Rl>::T pi = B.pk[i];
-        c.qi = qq * pi.x; c.sei = pi.y; c.sgi = pi.z; c.mi = pi.w; c.wi = B.wr[i];
-        c.arow = B.allow + (size_t)(i < ni ? i : 0) * W;
-        c.ok = i < nh ? 1u : 0u; c.ion = i < ni ? 1u : 0u;
-        c.am = am ? 1u : 0u; c.vi = vi ? 1u : 0u; c.vv = vv ? 1u : 0u;
-        c.ni = ni; c.nh = nh; c.wmax = W > 0 ? W - 1 : 0;
-        double fx = 0, fy = 0, fz = 0;
-        int jend0, jend1 = 0;
-        double a0x = 0, a0y = 0, a0z = 0, a1x = 0, a1y = 0, a1z = 0;
-        if (diag) {
-          // k = 1..32 in runs of L; k = 32 only on lanes 0..31
-          const int kb = 1 + part * L;
-          const uint32_t half_ok = lane < 32 ? 1u : 0u;
-          for (int k = kb; k < kb + L; k++) {
-            double gx, gy, gz;
-            dense_pair<Rl, EF, kPairMixed>(B, c, 64 * bj + ((lane + k) & 63), k != 32 ? 1u : half_ok, gx, gy, gz,
-                                           e_ion, e_vi, e_vv);
-            fx -= gx; fy -= gy; fz -= gz;
-            a0x = wave_rol1(a0x + gx);
-            a0y = wave_rol1(a0y + gy);
-            a0z = wave_rol1(a0z + gz);
-          }
-          jend0 = 64 * bj + ((lane + kb + L) & 63);
-        } else {
-          // two independent partners per step, k and k + 32, each with its
-          // own travelling F_j accumulator
-          const int hL = L >> 1, kb = part * hL;
-          double a0[3] = {0, 0, 0}, a1[3] = {0, 0, 0};
-          // the item's class (bi < bj): whole-ion blocks end at NBi; a
-          // block at or past the ion's end holds vapor only
-          const int cls = bj < NBi ? kPairIonIon
-                                   : (bi < NBi && 64 * bj >= ni ? kPairIonVap
-                                                               : (64 * bi >= ni ? kPairVapVap : kPairMixed));
-          if (cls == kPairIonIon)
-            dense_run_off<Rl, EF, kPairIonIon>(B, c, lane, bj, kb, hL, fx, fy, fz, a0, a1, e_ion, e_vi, e_vv);
-          else if (cls == kPairIonVap)
-            dense_run_off<Rl, EF, kPairIonVap>(B, c, lane, bj, kb, hL, fx, fy, fz, a0, a1, e_ion, e_vi, e_vv);
-          else if (cls == kPairVapVap)
-            dense_run_off<Rl, EF, kPairVapVap>(B, c, lane, bj, kb, hL, fx, fy, fz, a0, a1, e_ion, e_vi, e_vv);
-          else
-            dense_run_off<Rl, EF, kPairMixed>(B, c, lane, bj, kb, hL, fx, fy, fz, a0, a1, e_ion, e_vi, e_vv);
-          a0x = a0[0]; a0y = a0[1]; a0z = a0[2];
-          a1x = a1[0]; a1y = a1[1]; a1z = a1[2];
-          jend0 = 64 * bj + ((lane + kb + hL) & 63);
-          jend1 = 64 * bj + ((lane + kb + hL + 32) & 63);
-          atomicAdd(&pf[3 * jend1], a1x);
-          atomicAdd(&pf[3 * jend1 + 1], a1y);
-          atomicAdd(&pf[3 * jend1 + 2], a1z);
-        }
-        // each accumulator now belongs to the partner one step past the run
-        // (64 distinct atoms of block bj per accumulator set; zero sums for
-        // padding and masked partners)
-        atomicAdd(&pf[3 * jend0], a0x);
-        atomicAdd(&pf[3 * jend0 + 1], a0y);
-        atomicAdd(&pf[3 * jend0 + 2], a0z);
-        if (i < nh) {
-          atomicAdd(&pf[3 * i], fx);
-          atomicAdd(&pf[3 * i + 1], fy);
-          atomicAdd(&pf[3 * i + 2], fz);
-        }
+  const uint32_t amu = am ? 1u : 0u, viu = vi ? 1u : 0u, vvu = vv ? 1u : 0u;
+  // the lane's atom of a unit whose i block starts at ib
+  auto setup = [&](int i, int ilim, int jlim, DenseI<Rl> &c) {
+    c.x0 = B.x[3 * i]; c.x1 = B.x[3 * i + 1]; c.x2 = B.x[3 * i + 2];
+    const typename Pack4<Rl>::T pi = B.pk[i];
+    c.qi = qq * pi.x; c.sei = pi.y; c.sgi = pi.z; c.mi = pi.w; c.wi = B.wr[i];
+    c.arow = B.allow + (size_t)(i < ni ? i : 0) * W;
+    c.ok = i < ilim ? 1u : 0u;
+    c.am = amu; c.vi = viu; c.vv = vvu;
+    c.jlim = jlim; c.wmax = W > 0 ? W - 1 : 0;
+  };
+  auto flush = [&](int idx, double ax, double ay, double az) {
+    atomicAdd(&pf[3 * idx], ax);
+    atomicAdd(&pf[3 * idx + 1], ay);
+    atomicAdd(&pf[3 * idx + 2], az);
+  };
+  // one block-pair unit's run: i block at ib (atoms < ilim real), j block at
+  // jb (atoms < jlim real)
+  auto run = [&](int cls, bool diag, int ib, int ilim, int jb, int jlim, int part) {
+    const int i = ib + lane;
+    DenseI<Rl> c;
+    setup(i, ilim, jlim, c);
+    double fx = 0, fy = 0, fz = 0;
+    if (diag) {
+      // k = 1..32 in runs of L; k = 32 only on lanes 0..31
+      const int kb = 1 + part * L;
+      const uint32_t half_ok = lane < 32 ? 1u : 0u;
+      double a0x = 0, a0y = 0, a0z = 0;
+      for (int k = kb; k < kb + L; k++) {
+        double gx, gy, gz;
+        const int j = jb + ((lane + k) & 63);
+        const uint32_t vk = k != 32 ? 1u : half_ok;
+        if (cls == kPairIonIon) dense_pair<Rl, EF, kPairIonIon>(B, c, j, vk, gx, gy, gz, e_ion, e_vi, e_vv);
+        else dense_pair<Rl, EF, kPairVapVap>(B, c, j, vk, gx, gy, gz, e_ion, e_vi, e_vv);
+        fx -= gx; fy -= gy; fz -= gz;
+        a0x = wave_rol1(a0x + gx);
+        a0y = wave_rol1(a0y + gy);
+        a0z = wave_rol1(a0z + gz);
       }
+      // each accumulator now belongs to the partner one step past the run
+      // (64 distinct atoms of the j block; zero sums for masked partners)
+      flush(jb + ((lane + kb + L) & 63), a0x, a0y, a0z);
+    } else {
+      // two independent partners per step, k and k + 32, each with its own
+      // travelling F_j accumulator
+      const int hL = L >> 1, kb = part * hL;
+      double a0[3] = {0, 0, 0}, a1[3] = {0, 0, 0};
+      if (cls == kPairIonIon)
+        dense_run_off<Rl, EF, kPairIonIon>(B, c, lane, jb, kb, hL, fx, fy, fz, a0, a1, e_ion, e_vi, e_vv);
+      else if (cls == kPairIonVap)
+        dense_run_off<Rl, EF, kPairIonVap>(B, c, lane, jb, kb, hL, fx, fy, fz, a0, a1, e_ion, e_vi, e_vv);
+      else
+        dense_run_off<Rl, EF, kPairVapVap>(B, c, lane, jb, kb, hL, fx, fy, fz, a0, a1, e_ion, e_vi, e_vv);
+      flush(jb + ((lane + kb + hL) & 63), a0[0], a0[1], a0[2]);
+      flush(jb + ((lane + kb + hL + 32) & 63), a1[0], a1[1], a1[2]);
     }
-  }
+    if (i < ilim) flush(i, fx, fy, fz);
+  };
+  int item = 0;
+  // ion-ion units
+  if (doII)
+    for (int bi = 0; bi < NBI; bi++)
+      for (int bj = bi; bj < NBI; bj++) {
+        const int np = (bi == bj ? 32 : 64) / L;
+        for (int part = 0; part < np; part++, item++)
+          if (item % nw == wave) run(kPairIonIon, bi == bj, 64 * bi, ni, 64 * bj, ni, part);
+      }
+  // ion-vapor units
+  if (doIV && !bcast)
+    for (int bi = 0; bi < NBI; bi++)
+      for (int bj = 0; bj < NBV; bj++)
+        for (int part = 0; part < 64 / L; part++, item++)
+          if (item % nw == wave) run(kPairIonVap, false, 64 * bi, ni, ni + 64 * bj, nh, part);
+  if (doIV && bcast)
+    for (int bj = 0; bj < NBV; bj++, item++) {
+      if (item % nw != wave) continue;
+      // the lane's vapor atom against every ion atom (uniform partner):
+      // F_vapor in registers, F_ion a wave sum per ion atom
+      const int v = ni + 64 * bj + lane;
+      DenseI<Rl> c;
+      setup(v, nh, ni, c);
+      double fx = 0, fy = 0, fz = 0;
+      for (int i = 0; i < ni; i++) {
+        double gx, gy, gz;
+        dense_pair<Rl, EF, kPairIonVap>(B, c, i, 1u, gx, gy, gz, e_ion, e_vi, e_vv);
+        fx -= gx; fy -= gy; fz -= gz;
+        const double sx = wave_sum(gx), sy = wave_sum(gy), sz = wave_sum(gz);
+        if (lane == 0) flush(i, sx, sy, sz);
+      }
+      if (v < nh) flush(v, fx, fy, fz);
+    }
+  // vapor-vapor units
+  if (doVV)
+    for (int bi = 0; bi < NBV; bi++)
+      for (int bj = bi; bj < NBV; bj++) {
+        const int np = (bi == bj ? 32 : 64) / L;
+        for (int part = 0; part < np; part++, item++)
+          if (item % nw == wave) run(kPairVapVap, bi == bj, ni + 64 * bi, nh, ni + 64 * bj, nh, part);
+      }
   if (EF) { e[U_ION] += e_ion; e[U_VI] += e_vi; e[U_VV] += e_vv; }
+  return item;
 }
 
 template <typename Rl>
-__device__ void dense_heavy(const DevReplica &R, const BatchLds<Rl> &B, const LView &L, int t, int t0, int nt,
-                            int filt, bool eflag, double *e) {
+__device__ int dense_heavy(const DevReplica &R, const BatchLds<Rl> &B, const LView &L, int t, int t0, int nt,
+                           int filt, bool eflag, double *e) {
   const int nh = L.gbase, ni = L.n_ion;
   const bool am = (R.flags & B_AMBER) != 0, vi = (R.flags & B_VI) != 0, vv = (R.flags & B_VV) != 0;
-  if (!(am || vi || vv) || nh == 0) return;
-  if (eflag)
-    dense_n3<Rl, true>(R, B, nh, ni, am, vi, vv, t, t0, nt, filt, e);
-  else
-    dense_n3<Rl, false>(R, B, nh, ni, am, vi, vv, t, t0, nt, filt, e);
+  if (!(am || vi || vv) || nh == 0) return 0;
+  if (eflag) return dense_n3<Rl, true>(R, B, nh, ni, am, vi, vv, t, t0, nt, filt, e);
+  return dense_n3<Rl, false>(R, B, nh, ni, am, vi, vv, t, t0, nt, filt, e);
 }
 
 // the batched path's force phase: registration order of setPotential.cpp
@@ -663,10 +679,12 @@ template <typename Rl>
 __device__ void batch_terms_b(const DevReplica &R, const SStoreT<Rl> &s, const BatchLds<Rl> &B, int t, int nt,
                               bool eflag, double *e) {
   const uint32_t bits = R.flags;
-  dense_heavy<Rl>(R, B, s.L, t, 0, nt, kDenseNotIonOnly, eflag, e);
-  if (bits & B_GG) term_gg<Rl>(R, s, t, nt, eflag, e);
-  if (bits & B_GV) term_gv<Rl>(R, s, t, nt, eflag, e);
-  if (bits & B_VINTRA) term_vintra(R, s, t, nt, eflag, e);
+  const int items = dense_heavy<Rl>(R, B, s.L, t, 0, nt, kDenseNotIonOnly, eflag, e);
+  // the remaining terms start on the first wave that got one item fewer
+  const int tr = (t + nt - 64 * (items % (nt >> 6))) % nt;
+  if (bits & B_GG) term_gg<Rl>(R, s, tr, nt, eflag, e);
+  if (bits & B_GV) term_gv<Rl>(R, s, tr, nt, eflag, e);
+  if (bits & B_VINTRA) term_vintra(R, s, tr, nt, eflag, e);
 }
 
 template <typename Rl>
@@ -710,9 +728,9 @@ __device__ inline void batch_load(const DevReplica &R, const LView &L, const Bat
     }
     B.wr[a] = (Rl)R.atom_w[g];
   }
-  // finite values to the end of the ion + vapor range's last 64-atom block
-  // (read by dense_n3): parameters past the range, atom data past n_in
-  const int hend = (L.gbase + 63) & ~63;
+  // finite values for one block past the ion + vapor range (read by
+  // dense_n3): parameters past the range, atom data past n_in
+  const int hend = L.gbase + 64;
   for (int a = L.gbase + threadIdx.x; a < hend; a += blockDim.x) {
     B.pk[a].x = (Rl)0; B.pk[a].y = (Rl)0; B.pk[a].z = (Rl)0; B.pk[a].w = (Rl)-1;
   }
@@ -748,7 +766,7 @@ __global__ void __launch_bounds__(512) k_batched(const DevReplica *__restrict__ 
     Dyn *D = R.dyn;
     if (D->status & ST_LDS_CAP) continue;
     LView L = view_from_dyn(R);
-    if (L.n_in > cap || L.gbase > cap_h || L.n_ion + L.na_v + L.na_g > kcap ||
+    if (L.n_in > cap || L.gbase + 64 > cap_h || L.n_ion + L.na_v + L.na_g > kcap ||
         ((R.flags & B_AMBER) && L.n_ion * R.amber_words > acap)) {
       if (tid == 0) atomicOr(&D->status, ST_LDS_CAP);
       continue;
@@ -888,7 +906,7 @@ __global__ void __launch_bounds__(512) k_batched(const DevReplica *__restrict__ 
           L = view_from_dyn(R);
           lp = D->loopPair;
           lu = D->loop_update;
-          if (L.n_in > cap || L.gbase > cap_h) {
+          if (L.n_in > cap || L.gbase + 64 > cap_h) {
             if (tid == 0) atomicOr(&D->status, ST_LDS_CAP);
             dead = true;
             break;
@@ -1552,14 +1570,14 @@ static BatchCaps batch_caps(const mdn_ctx *c, const std::vector<Dyn> &dyn, const
     kcap = std::max(kcap, d.sp[0].natoms + d.sp[1].natoms + d.sp[2].natoms);
     if (d.flags & MDN_ION_AMBER) acap = std::max(acap, d.sp[0].natoms * ((d.sp[0].natoms + 31) / 32));
   }
-  // the ion + vapor capacity is a whole number of 64-atom blocks (dense_n3
-  // reads whole blocks of positions, weights, forces and parameters), the
-  // total a multiple of 32 and at least that
+  // the ion + vapor capacity has a block (64 atoms) of slack past the range
+  // (dense_n3 reads whole blocks of positions, weights, forces and
+  // parameters), the total a multiple of 32 and at least that
   auto r32 = [](int v) { return (v + 31) / 32 * 32; };
   auto r64 = [](int v) { return (v + 63) / 64 * 64; };
   most = r32(most);
-  most_h = r64(most_h);
-  int lo_h = std::min(most_h, r64(need_h + 32)), hi_h = std::min(most_h, r64(need_h * 3 / 2 + 32));
+  most_h = r64(most_h) + 64;
+  int lo_h = std::min(most_h, r64(need_h + 32) + 64), hi_h = std::min(most_h, r64(need_h * 3 / 2 + 32) + 64);
   int lo = std::min(most, r32(need + need / 8 + 64)), hi = std::min(most, r32(need * 3 / 2 + 64));
   BatchCaps b;
   kcap += kcap & 1;  // keeps the LDS arrays after it 16-byte aligned
