@@ -116,9 +116,18 @@ def _cond(tmp_path, margin=1.0, n_gas=400, n_vap=12, pressure=6e5, shake=True, i
     from md_nucleation_amd import configs
     d = str(tmp_path)
     m = configs.molecule_files(d)
-    ionf = m[ion] if ion in m else configs.write_peptide(d + "/pep30.txt", seed=5, n=30)
+    # ions: Na+ (one atom), a 30- and a 70-atom AMBER chain (the latter ends
+    # in a partial 64-atom block), a 27-atom NaCl cluster (BMH, no AMBER pairs)
+    pot = None
+    if ion in m:
+        ionf = m[ion]
+    elif ion == "nacl":
+        ionf, pot = configs.write_nacl_cluster(d + "/nacl27.txt", n_side=3), "Born-Mayer-Huggins-NaCl"
+    else:
+        n = 70 if ion == "pep70" else 30
+        ionf, pot = configs.write_peptide(d + "/pep%d.txt" % n, seed=5, n=n), "AMBER"
     return configs.Condition(ionf, m["methanol"], n_vap, "N2", n_gas, pressure=pressure, margin=margin,
-                             shake_h=shake, ion_potential="AMBER" if ion == "pep" else None).write(d + "/t.cond")
+                             shake_h=shake, ion_potential=pot).write(d + "/t.cond")
 
 
 def _trajectory_case(tmp_path, path, nsteps, precision, ion="na", shake=True, seeds=(3,)):
@@ -148,8 +157,8 @@ def _trajectory_case(tmp_path, path, nsteps, precision, ion="na", shake=True, se
     return eng, sds, orcs, Ss
 
 
-@pytest.mark.parametrize("path", ["percall", "batched"])
-@pytest.mark.parametrize("ion", ["na", "pep"])
+@pytest.mark.parametrize("path,ion", [("percall", "na"), ("batched", "na"), ("percall", "pep"), ("batched", "pep"),
+                                      ("batched", "nacl"), ("batched", "pep70")])
 def test_trajectory_fp64_vs_oracle(tmp_path, path, ion):
     """200 FP64 steps with SHAKE and several pair-list rebuilds (Margin 1 A)."""
     eng, sds, orcs, Ss = _trajectory_case(tmp_path, path, 200, M.FP64, ion=ion, seeds=(3, 4))
@@ -169,7 +178,7 @@ def test_trajectory_fp64_vs_oracle(tmp_path, path, ion):
         assert rb == S.c.rebuild_count
 
 
-@pytest.mark.parametrize("ion", ["na", "pep"])
+@pytest.mark.parametrize("ion", ["na", "pep", "nacl", "pep70"])
 def test_batched_mixed_force_error_one_step(tmp_path, ion):
     """After one step both sides hold F(x1) for identical x1 (x1 depends only on
     the state's forces): the mixed-precision gate, relative RMS <= 1e-5."""
