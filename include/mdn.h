@@ -64,6 +64,8 @@ enum { MDN_U_ION = 0, MDN_U_GAS, MDN_U_VAP, MDN_U_GI, MDN_U_GG, MDN_U_VG,
        MDN_U_VI, MDN_U_VV, MDN_U_N };
 /* precision of the pair and bonded arithmetic */
 enum { MDN_FP64 = 0, MDN_MIXED = 1 };
+/* ion thermostat (flags velocity_scaling / nose_hoover_ion, flags.hpp:5-8) */
+enum { MDN_NVE = 0, MDN_VSCALE = 1, MDN_NOSE_HOOVER = 2 };
 
 typedef struct mdn_sysdef mdn_sysdef;
 typedef struct mdn_ctx mdn_ctx;
@@ -99,6 +101,11 @@ typedef struct mdn_system {
   int64_t total_steps, relax_steps;
   int32_t observe;
   double cluster_radius;     /* >0: vapor pre-packed around the ion */
+  /* ion thermostat of the relaxation phase ("NVTion", setCondition.cpp:
+   * 158-172): 0 off, 1 velocity scaling to 300 K (nvt.cpp:5-16), 2
+   * Nose-Hoover at Tnh_ion (nvt.cpp:19-35) */
+  int32_t nvt_ion;
+  double Tnh_ion;
 } mdn_system;
 
 /* Caller-owned host state of one replica (sizes from mdn_sysdef_sizes). */
@@ -113,6 +120,7 @@ typedef struct mdn_state {
   int32_t n_gi, n_gv, n_gg;
   int32_t *gi, *gv, *gg;      /* [cap*2] pair lists (molecule, ion atom) etc. */
   double U[MDN_U_N];
+  double zeta_ion;            /* Nose-Hoover friction of the ion (vars->zeta_ion) */
 } mdn_state;
 
 const char *mdn_last_error(void);
@@ -142,8 +150,15 @@ int mdn_add_replica(mdn_ctx *c, const mdn_sysdef *d, const mdn_state *st,
                     uint64_t seed, uint32_t replica_id, int32_t *handle);
 /* allocates device memory for all replicas and uploads them */
 int mdn_commit(mdn_ctx *c);
+/* replaces one replica's state; the other replicas keep theirs */
 int mdn_set_state(mdn_ctx *c, int32_t replica, const mdn_state *st);
 int mdn_get_state(mdn_ctx *c, int32_t replica, mdn_state *st);
+/* ion thermostat of MD::verlet (verlet.cpp:22, 27, 36; nvt.cpp:5-35) for one
+ * replica or all (replica = -1).  MDN_NVE is MD::setNVE (nvt.cpp:38-43, called
+ * by run.cpp:56 after the relaxation); MDN_VSCALE scales the ion to 300 K
+ * after the first half-kick; MDN_NOSE_HOOVER at Tnh (K) resets zeta_ion to 0.
+ * A replica starts in the mode of its condition file's NVTion line. */
+int mdn_set_thermostat(mdn_ctx *c, int32_t replica, int32_t mode, double Tnh);
 
 /* drop-in path: one replica, FP64, one launch per reference call */
 int mdn_make_pair(mdn_ctx *c, int32_t replica);

@@ -17,6 +17,7 @@ MDN_GI, MDN_VI, MDN_GG, MDN_GV, MDN_VV = 1, 2, 4, 8, 16
 MDN_ION_AMBER, MDN_ION_BMH, MDN_ION_TERSOFF, MDN_ION_SW = 32, 64, 128, 256
 MDN_EFIELD, MDN_VAPOR_INTRA, MDN_GAS_INTRA = 512, 1024, 2048
 FP64, MIXED = 0, 1
+NVE, VSCALE, NOSE_HOOVER = 0, 1, 2  # ion thermostat modes (NVTion OFF / scale / <T>)
 U_LABELS = ("Uion", "Ugas", "Uvap", "Ugi", "Ugg", "Uvg", "Uvi", "Uvv")
 
 _dp = C.POINTER(C.c_double)
@@ -37,7 +38,7 @@ class MdnSystem(C.Structure):
         ("m_gas_kg", C.c_double), ("m_vap_kg", C.c_double), ("gastype", C.c_int32), ("gas_spin", C.c_int32),
         ("flags", C.c_uint32), ("efield", C.c_double * 3), ("shake", C.c_int32), ("shake_max_iter", C.c_int32),
         ("total_steps", C.c_int64), ("relax_steps", C.c_int64), ("observe", C.c_int32),
-        ("cluster_radius", C.c_double),
+        ("cluster_radius", C.c_double), ("nvt_ion", C.c_int32), ("Tnh_ion", C.c_double),
     ]
 
 
@@ -48,7 +49,7 @@ class MdnState(C.Structure):
         ("loopPair", C.c_int32), ("loop_update", C.c_int32), ("rebuild_count", C.c_int64),
         ("cap_gi", C.c_int32), ("cap_gv", C.c_int32), ("cap_gg", C.c_int32),
         ("n_gi", C.c_int32), ("n_gv", C.c_int32), ("n_gg", C.c_int32),
-        ("gi", _ip), ("gv", _ip), ("gg", _ip), ("U", C.c_double * 8),
+        ("gi", _ip), ("gv", _ip), ("gg", _ip), ("U", C.c_double * 8), ("zeta_ion", C.c_double),
     ]
 
 
@@ -68,6 +69,7 @@ _SIGS = {
     "mdn_commit": (C.c_int, [C.c_void_p]),
     "mdn_set_state": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(MdnState)]),
     "mdn_get_state": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(MdnState)]),
+    "mdn_set_thermostat": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_double]),
     "mdn_make_pair": (C.c_int, [C.c_void_p, C.c_int32]),
     "mdn_zero_forces": (C.c_int, [C.c_void_p, C.c_int32]),
     "mdn_compute": (C.c_int, [C.c_void_p, C.c_int32, C.c_uint32, C.c_int32, C.c_int32]),
@@ -159,7 +161,7 @@ class SysDef:
             })
         for k in ("n_gas", "n_vap", "ion_type_base", "dt", "T", "pressure", "d_size", "cutoff", "margin", "ML2",
                   "m_gas_kg", "m_vap_kg", "gastype", "gas_spin", "flags", "shake", "shake_max_iter",
-                  "total_steps", "relax_steps", "observe", "cluster_radius"):
+                  "total_steps", "relax_steps", "observe", "cluster_radius", "nvt_ion", "Tnh_ion"):
             d[k] = getattr(v, k)
         d["efield"] = tuple(v.efield)
         return d
@@ -209,6 +211,7 @@ class State:
         c.loopPair = int(a.get("loopPair", 0))
         c.loop_update = int(a.get("loop_update", 0))
         c.rebuild_count = int(a.get("rebuild_count", 0))
+        c.zeta_ion = float(a.get("zeta_ion", 0.0))
         for k in ("gi", "gv", "gg"):
             arr = np.ascontiguousarray(np.asarray(a.get(k, np.zeros(0)), np.int32).reshape(-1))
             capk = max(cap, arr.size // 2)
@@ -238,7 +241,8 @@ class State:
 
     def as_dict(self):
         d = {k: getattr(self, k).copy() for k in ("mol_x", "mol_v", "region", "atom_x", "atom_v", "atom_f", "atom_w")}
-        d.update(loopPair=self.c.loopPair, loop_update=self.c.loop_update, rebuild_count=self.c.rebuild_count)
+        d.update(loopPair=self.c.loopPair, loop_update=self.c.loop_update, rebuild_count=self.c.rebuild_count,
+                 zeta_ion=self.c.zeta_ion)
         for k in ("gi", "gv", "gg"):
             d[k] = self.list(k).reshape(-1)
         return d
@@ -265,6 +269,10 @@ class Engine:
 
     def set_state(self, r, st: State):
         check(lib().mdn_set_state(self.h, int(r), C.byref(st.c)), "mdn_set_state")
+
+    def set_thermostat(self, r, mode, Tnh=0.0):
+        """MD::setNVE (mode NVE) or an NVTion mode for replica r (-1: all)."""
+        check(lib().mdn_set_thermostat(self.h, int(r), int(mode), float(Tnh)), "mdn_set_thermostat")
 
     def get_state(self, r, cap=None) -> State:
         sd = self.sys[r]

@@ -221,6 +221,8 @@ class Condition:
     ion_potential: str | None = None
     efield: tuple | None = None
     cluster: tuple | None = None  # (radius A, density g/cm3)
+    nvt_ion: str = "OFF"  # OFF, scale, or a Nose-Hoover temperature
+    observe: int = 100000000  # Output ... OBSERVE cadence
 
     def write(self, path):
         lines = ["Input " + self.ion_file]
@@ -234,8 +236,8 @@ class Condition:
             lines.append("CutOff %g" % self.cutoff)
         if self.margin is not None:
             lines.append("Margin %g" % self.margin)
-        lines.append("Output out 100000000")
-        lines.append("NVTion OFF")
+        lines.append("Output out %d" % self.observe)
+        lines.append("NVTion %s" % self.nvt_ion)
         if self.shake_h:
             lines.append("SHAKE H")
         if self.cluster:

@@ -278,6 +278,16 @@ void load_sysdef(SysDef &d, const std::string &cond) {
   d.flags = flags;
   d.shake = shakeH ? 1 : 0;
   d.shake_max_iter = 1000;
+  // NVTion (setCondition.cpp:158-172): OFF, "scale" (velocity scaling), or
+  // a Nose-Hoover temperature
+  if (d.nvt_ion.empty() || d.nvt_ion == "OFF") d.nvt_ion_mode = 0;
+  else if (d.nvt_ion == "scale") d.nvt_ion_mode = 1;
+  else {
+    char *end = nullptr;
+    d.Tnh_ion = strtod(d.nvt_ion.c_str(), &end);
+    if (end == d.nvt_ion.c_str()) throw std::runtime_error("NVTion: expected OFF, scale or a temperature");
+    d.nvt_ion_mode = 2;
+  }
   // setCondition.cpp:247-250
   d.d_size = pow(d.n_gas * kKb * d.T / d.pressure, 1 / 3.0) * 1e10;
   d.ML2 = (d.cutoff + d.margin) * (d.cutoff + d.margin);
@@ -526,6 +536,7 @@ int mdn_sysdef_view(const mdn_sysdef *h, mdn_system *o) {
   o->shake = d.shake; o->shake_max_iter = d.shake_max_iter;
   o->total_steps = d.total_steps; o->relax_steps = d.relax_steps; o->observe = d.observe;
   o->cluster_radius = d.cluster_radius;
+  o->nvt_ion = d.nvt_ion_mode; o->Tnh_ion = d.Tnh_ion;
   return MDN_OK;
 }
 

@@ -23,6 +23,12 @@ constexpr double kRI2 = (kRinter + kOverlap) * (kRinter + kOverlap);
 constexpr double kPI = 3.14159265358979323846;
 constexpr double kSqrt24 = 4.89897948556635619639;  // sqrt(24)
 constexpr double kKickUnit = 4.184e-4;
+// constants.hpp:30, 49-50 and observer.hpp:6: Boltzmann constant in kcal/mol/K,
+// g/mol A^2/fs^2 -> kcal/mol, and Tion = K / N * (1 / kB) / 1.5
+constexpr double kKbReal = 1.38e-23 * 6.02e23 / 4184.0;
+constexpr double kRealToKcal = 10000 / 4.184;
+// thermostat modes of the ion (NVTion line, setCondition.cpp:158-172)
+enum { THERMO_NVE = 0, THERMO_SCALE = 1, THERMO_NH = 2 };
 
 // interaction bits (same values as mdn.h)
 enum : uint32_t {
@@ -51,8 +57,9 @@ struct Dyn {
   int n_gi, n_gv, n_gg;
   int n_in_gas, n_in_vap;
   int status, shake_iters_max;
-  int pad[2];
+  int thermo, pad;           // THERMO_* of the ion
   unsigned long long shake_fail;
+  double zeta_ion, Tnh_ion;  // Nose-Hoover friction and target (nvt.cpp:19-35)
 };
 
 struct DevReplica {
@@ -308,6 +315,17 @@ __device__ inline double wave_sum(double v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
   return v;
 }
+// Tion of Observer::computeProps (observer.cpp:27-36) from sum m v^2 over
+// the N ion atoms
+__host__ __device__ inline double ion_temperature(double mv2, int n) {
+  return mv2 * (0.5 * kRealToKcal) / (double)n * ((1 / kKbReal) / 1.5);
+}
+// MD::nosehoover_zeta, nvt.cpp:18-24 (Q^-1 = 1e-4, g = 3N)
+__host__ __device__ inline double nh_zeta_step(double zeta, double Tion, double Tnh, int n, double dt) {
+  return zeta + (Tion - Tnh) * (n * 3) * kKbReal * 0.0001 * dt;
+}
+// MD::nosehoover_ion, nvt.cpp:27-35
+__host__ __device__ inline double nh_factor(double zeta, double dt) { return exp(-zeta * 0.5 * dt); }
 // exclusive scan of one int per thread over the workgroup; s_tmp holds
 // >= nthr/64 ints.  Returns the prefix, *total = the workgroup sum.
 __device__ inline int block_scan(int v, int *s_tmp, int *total) {

@@ -55,8 +55,10 @@ __device__ inline double gmass(const DevReplica &R, const LView &L, int a) {
 
 // MD::velocity_calculation, verlet.cpp:47-65 (region atoms only).  At the
 // start of an energy step it also clears the energy buckets (Uzero,
-// run.cpp:68-71); at the end of a step it counts in-domain atom-steps.
-__global__ void k_kick(const DevReplica *__restrict__ Rp, int zeroU, int count) {
+// run.cpp:68-71); at the end of a step it counts in-domain atom-steps and,
+// under the Nose-Hoover thermostat, scales the ion's velocities by
+// exp(-zeta dt / 2) after the kick (MD::nosehoover_ion, verlet.cpp:36).
+__global__ void k_kick(const DevReplica *__restrict__ Rp, int zeroU, int count, int nh) {
   const DevReplica &R = *Rp;
   LView L = view_from_dyn(R);
   int t = blockIdx.x * blockDim.x + threadIdx.x, nt = gridDim.x * blockDim.x;
@@ -64,6 +66,7 @@ __global__ void k_kick(const DevReplica *__restrict__ Rp, int zeroU, int count) 
     for (int k = 0; k < U_N; k++) R.dyn->U[k] = 0;
   if (t == 0 && count) R.dyn->atom_steps += L.n_in;
   const double C = 0.5 * R.dt * kKickUnit;
+  const double fac = nh ? nh_factor(R.dyn->zeta_ion, R.dt) : 1.0;
   for (int a = t; a < L.n_in; a += nt) {
     int g = gatom(R, L, a);
     double c2 = C / gmass(R, L, a);
@@ -71,7 +74,38 @@ __global__ void k_kick(const DevReplica *__restrict__ Rp, int zeroU, int count) 
     v[0] += f[0] * c2;
     v[1] += f[1] * c2;
     v[2] += f[2] * c2;
+    if (nh && a < L.n_ion) { v[0] *= fac; v[1] *= fac; v[2] *= fac; }
   }
+}
+
+// sum of m v^2 over the ion atoms, returned to every thread of the (single)
+// workgroup; s_red holds >= 16 doubles
+__device__ inline double ion_mv2(const DevReplica &R, double *s_red) {
+  const DevSpecies &I = R.sp[0];
+  int tid = threadIdx.x, nt = blockDim.x;
+  double k = 0;
+  for (int i = tid; i < I.natoms; i += nt) {
+    const double *v = R.atom_v + 3 * (size_t)i;
+    double m = I.mass[i];
+    k += v[0] * v[0] * m + v[1] * v[1] * m + v[2] * v[2] * m;
+  }
+  k = wave_sum(k);
+  if ((tid & 63) == 0) s_red[tid >> 6] = k;
+  __syncthreads();
+  double s = 0;
+  for (int w = 0; w < (nt + 63) / 64; w++) s += s_red[w];
+  return s;
+}
+
+// MD::velocity_scaling, nvt.cpp:5-16: the ion's velocities scaled to 300 K
+// right after the first half-kick (verlet.cpp:22); one workgroup
+__global__ void __launch_bounds__(1024) k_ion_scale(const DevReplica *__restrict__ Rp) {
+  const DevReplica &R = *Rp;
+  __shared__ double s_red[16];
+  const DevSpecies &I = R.sp[0];
+  double Tion = ion_temperature(ion_mv2(R, s_red), I.natoms);
+  double ratio = sqrt(300.0 / Tion);
+  for (int i = threadIdx.x; i < 3 * I.natoms; i += blockDim.x) R.atom_v[i] *= ratio;
 }
 
 // drift + force clear, verlet.cpp:76-88 / shake.cpp:6-24
@@ -120,32 +154,40 @@ __global__ void k_shake(const DevReplica *__restrict__ Rp) {
   }
 }
 
-// ion COM + blend weights, verlet.cpp:90-134 (one workgroup)
-__global__ void __launch_bounds__(1024) k_com_weights(const DevReplica *__restrict__ Rp) {
+// ion COM + blend weights, verlet.cpp:90-134 (one workgroup); with nh set,
+// also the Nose-Hoover friction update that follows update_position
+// (MD::nosehoover_zeta, verlet.cpp:27)
+__global__ void __launch_bounds__(1024) k_com_weights(const DevReplica *__restrict__ Rp, int nh) {
   const DevReplica &R = *Rp;
-  __shared__ double s_red[6][16];
+  __shared__ double s_red[7][16];
   __shared__ double s_com[6];
   LView L = view_from_dyn(R);
   int tid = threadIdx.x, nt = blockDim.x;
-  double acc[6] = {0, 0, 0, 0, 0, 0};
+  double acc[7] = {0, 0, 0, 0, 0, 0, 0};
   const DevSpecies &I = R.sp[0];
   for (int i = tid; i < I.natoms; i += nt) {
     double m = I.mass[i];
     const double *x = R.atom_x + 3 * (size_t)i, *v = R.atom_v + 3 * (size_t)i;
     acc[0] += x[0] * m; acc[1] += x[1] * m; acc[2] += x[2] * m;
     acc[3] += v[0] * m; acc[4] += v[1] * m; acc[5] += v[2] * m;
+    acc[6] += v[0] * v[0] * m + v[1] * v[1] * m + v[2] * v[2] * m;
   }
-  for (int k = 0; k < 6; k++) {
+  for (int k = 0; k < 7; k++) {
     double s = wave_sum(acc[k]);
     if ((tid & 63) == 0) s_red[k][tid >> 6] = s;
   }
   __syncthreads();
-  if (tid < 6) {
+  if (tid < 7) {
     double s = 0;
     for (int w = 0; w < (nt + 63) / 64; w++) s += s_red[tid][w];
-    double c = s * (1 / I.molmass);
-    s_com[tid] = c;
-    if (tid < 3) R.mol_x[tid] = c; else R.mol_v[tid - 3] = c;
+    if (tid == 6) {
+      if (nh) R.dyn->zeta_ion = nh_zeta_step(R.dyn->zeta_ion, ion_temperature(s, I.natoms), R.dyn->Tnh_ion,
+                                             I.natoms, R.dt);
+    } else {
+      double c = s * (1 / I.molmass);
+      s_com[tid] = c;
+      if (tid < 3) R.mol_x[tid] = c; else R.mol_v[tid - 3] = c;
+    }
   }
   __syncthreads();
   for (int a = L.n_ion + tid; a < L.n_in; a += nt) {
@@ -776,6 +818,11 @@ __global__ void __launch_bounds__(512) k_batched(const DevReplica *__restrict__ 
     long long acc_steps = 0;
     const DevSpecies &I = R.sp[0];
     const bool shake = R.shake != 0;
+    // ion thermostat (nvt.cpp): the mode is uniform over the launch, zeta
+    // lives in a register and reaches the replica with the state
+    const int thermo = D->thermo;
+    const double Tnh = D->Tnh_ion;
+    double zeta = D->zeta_ion;
     // SHAKE diagnostics accumulate in LDS (sint[32]: max iterations,
     // sint[33]: failures) and reach the replica's counters once per launch
     if (tid == 0) { B.sint[32] = 0; B.sint[33] = 0; }
@@ -783,9 +830,32 @@ __global__ void __launch_bounds__(512) k_batched(const DevReplica *__restrict__ 
     bool dead = false;
     for (int s = 0; s < nsteps && !dead; s++) {
       const bool eflag = eflag_every > 0 && ((step0 + s + 1) % eflag_every) == 0;
+      // MD::velocity_scaling (verlet.cpp:22): the ion is kicked first, its
+      // temperature reduced and its velocities scaled to 300 K; the fused
+      // loop below then only drifts it (kick coefficient 0)
+      const bool vscale = thermo == THERMO_SCALE;
+      if (vscale) {
+        double k2 = 0;
+        for (int a = tid; a < L.n_ion; a += nt) {
+          double kc = B.kct[B.kidx[a]], m = B.kms[a];
+          for (int q = 0; q < 3; q++) {
+            double v = B.v[3 * a + q] + B.f[3 * a + q] * kc;
+            B.v[3 * a + q] = v;
+            k2 += v * v * m;
+          }
+        }
+        k2 = wave_sum(k2);
+        if ((tid & 63) == 0) B.red[tid >> 6] = k2;
+        __syncthreads();
+        double sv = 0;
+        for (int w = 0; w < nt / 64; w++) sv += B.red[w];
+        const double ratio = sqrt(300.0 / ion_temperature(sv, I.natoms));
+        for (int a = tid; a < L.n_ion; a += nt)
+          for (int q = 0; q < 3; q++) B.v[3 * a + q] *= ratio;
+      }
       // first half-kick + drift + force clear (verlet.cpp:21-25)
       for (int a = tid; a < L.n_in; a += nt) {
-        double kc = B.kct[B.kidx[a]];
+        double kc = vscale && a < L.n_ion ? 0.0 : B.kct[B.kidx[a]];
         bool keep = shake && a < L.gbase;
         for (int q = 0; q < 3; q++) {
           double v = B.v[3 * a + q] + B.f[3 * a + q] * kc;
@@ -827,7 +897,8 @@ __global__ void __launch_bounds__(512) k_batched(const DevReplica *__restrict__ 
       // then the vapor weights and the remaining forces (phase B).  The
       // forces are the same terms as in the sequential order; only their
       // accumulation order differs.
-      const bool overlap = shake && Vs.nrigid && !I.nrigid && L.nv > 0 && nwv >= 2 && lp + 1 <= lu;
+      const bool overlap =
+          shake && Vs.nrigid && !I.nrigid && L.nv > 0 && nwv >= 2 && lp + 1 <= lu && thermo == THERMO_NVE;
       if (overlap) {
         const int nS = min(nwv - 1, (L.nv + 63) >> 6), nA = nwv - nS;
         const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -869,24 +940,29 @@ __global__ void __launch_bounds__(512) k_batched(const DevReplica *__restrict__ 
           if (I.nrigid && tid == nt - 1) shake_one(I, 0);
           __syncthreads();
         }
-        // ion COM (verlet.cpp:90-107)
+        // ion COM (verlet.cpp:90-107); under Nose-Hoover also sum m v^2 of
+        // the ion for MD::nosehoover_zeta (verlet.cpp:27)
         {
-          double c[6] = {0, 0, 0, 0, 0, 0};
+          const int nk = thermo == THERMO_NH ? 7 : 6;
+          double c[7] = {0, 0, 0, 0, 0, 0, 0};
           for (int i = tid; i < I.natoms; i += nt) {
             double m = B.kms[i];
             for (int q = 0; q < 3; q++) { c[q] += B.x[3 * i + q] * m; c[3 + q] += B.v[3 * i + q] * m; }
+            c[6] += B.v[3 * i] * B.v[3 * i] * m + B.v[3 * i + 1] * B.v[3 * i + 1] * m +
+                    B.v[3 * i + 2] * B.v[3 * i + 2] * m;
           }
-          for (int k = 0; k < 6; k++) {
+          for (int k = 0; k < nk; k++) {
             double sv = wave_sum(c[k]);
             if ((tid & 63) == 0) B.red[k * 16 + (tid >> 6)] = sv;
           }
           __syncthreads();
-          if (tid < 6) {
+          if (tid < nk) {
             double sv = 0;
             for (int w = 0; w < nt / 64; w++) sv += B.red[tid * 16 + w];
-            B.sdbl[tid] = sv * (1 / I.molmass);
+            B.sdbl[tid] = tid < 6 ? sv * (1 / I.molmass) : sv;
           }
           __syncthreads();
+          if (thermo == THERMO_NH) zeta = nh_zeta_step(zeta, ion_temperature(B.sdbl[6], I.natoms), Tnh, I.natoms, R.dt);
         }
         const double cx = B.sdbl[0], cy = B.sdbl[1], cz = B.sdbl[2];
         // blend weights (verlet.cpp:109-134)
@@ -900,7 +976,7 @@ __global__ void __launch_bounds__(512) k_batched(const DevReplica *__restrict__ 
           __syncthreads();
           batch_store(R, L, B);
           if (tid < 3) { R.mol_x[tid] = B.sdbl[tid]; R.mol_v[tid] = B.sdbl[3 + tid]; }
-          if (tid == 0) D->loopPair = lp;
+          if (tid == 0) { D->loopPair = lp; D->zeta_ion = zeta; }
           __syncthreads();
           rebuild_wg(R, B.sint, B.red);
           L = view_from_dyn(R);
@@ -919,10 +995,17 @@ __global__ void __launch_bounds__(512) k_batched(const DevReplica *__restrict__ 
         batch_terms<Rl>(R, ss, B, tid, nt, eflag, e);
       }
       __syncthreads();
-      // second half-kick (verlet.cpp:34)
-      for (int a = tid; a < L.n_in; a += nt) {
-        double kc = B.kct[B.kidx[a]];
-        for (int q = 0; q < 3; q++) B.v[3 * a + q] += B.f[3 * a + q] * kc;
+      // second half-kick (verlet.cpp:34), then MD::nosehoover_ion on the ion
+      // (verlet.cpp:36)
+      {
+        const double fac = thermo == THERMO_NH ? nh_factor(zeta, R.dt) : 1.0;
+        for (int a = tid; a < L.n_in; a += nt) {
+          double kc = B.kct[B.kidx[a]];
+          if (thermo == THERMO_NH && a < L.n_ion)
+            for (int q = 0; q < 3; q++) B.v[3 * a + q] = (B.v[3 * a + q] + B.f[3 * a + q] * kc) * fac;
+          else
+            for (int q = 0; q < 3; q++) B.v[3 * a + q] += B.f[3 * a + q] * kc;
+        }
       }
       acc_steps += L.n_in;
       if (eflag) {
@@ -942,7 +1025,7 @@ __global__ void __launch_bounds__(512) k_batched(const DevReplica *__restrict__ 
     if (!dead) {
       batch_store(R, L, B);
       if (tid < 3) { R.mol_x[tid] = B.sdbl[tid]; R.mol_v[tid] = B.sdbl[3 + tid]; }
-      if (tid == 0) D->loopPair = lp;
+      if (tid == 0) { D->loopPair = lp; D->zeta_ion = zeta; }
     }
     if (tid == 0) {
       D->atom_steps += acc_steps;
@@ -975,6 +1058,12 @@ struct HostReplica {
   DevReplica dev;
   size_t off_dyn = 0;
   hipGraphExec_t graph[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};  // [precision][eflag]
+  int thermo = THERMO_NVE;  // ion thermostat (host copy of Dyn::thermo)
+  double Tnh = 0, zeta_ion = 0;
+  // diagnostics carried over a re-upload (mdn_set_state of any replica)
+  long long atom_steps = 0;
+  int status = 0, shake_iters_max = 0;
+  unsigned long long shake_fail = 0;
 };
 
 }  // namespace mdn
@@ -1161,6 +1250,10 @@ static int upload_all(mdn_ctx *c) {
     o.gg = lay.vec(lgg, false);
     Dyn dyn;
     memset(&dyn, 0, sizeof(dyn));
+    dyn.atom_steps = h.atom_steps;
+    dyn.status = h.status;
+    dyn.shake_iters_max = h.shake_iters_max;
+    dyn.shake_fail = h.shake_fail;
     for (int k = 0; k < U_N; k++) dyn.U[k] = h.U[k];
     dyn.rebuild_count = h.rebuild_count;
     dyn.loopPair = h.loopPair;
@@ -1172,6 +1265,9 @@ static int upload_all(mdn_ctx *c) {
     in_lists(h, ig2, iv2);
     dyn.n_in_gas = (int)ig2.size();
     dyn.n_in_vap = (int)iv2.size();
+    dyn.thermo = h.thermo;
+    dyn.zeta_ion = h.zeta_ion;
+    dyn.Tnh_ion = h.Tnh;
     o.dyn = dyn_block + sizeof(Dyn) * r;
     memcpy(lay.host.data() + o.dyn, &dyn, sizeof(dyn));
     h.off_dyn = o.dyn;
@@ -1280,15 +1376,18 @@ template <typename Rl>
 static int launch_step(mdn_ctx *c, const HostReplica &h, const DevReplica *R, bool eflag) {
   const SysDef &d = h.def;
   int g = grid_for(d.n_atoms);
-  hipLaunchKernelGGL(k_kick, dim3(g), dim3(kBlk), 0, c->stream, R, eflag ? 1 : 0, 0);
+  const int nh = h.thermo == THERMO_NH ? 1 : 0;
+  hipLaunchKernelGGL(k_kick, dim3(g), dim3(kBlk), 0, c->stream, R, eflag ? 1 : 0, 0, 0);
+  // thermostat hooks where MD::verlet calls them (verlet.cpp:22, 27, 36)
+  if (h.thermo == THERMO_SCALE) hipLaunchKernelGGL(k_ion_scale, dim3(1), dim3(1024), 0, c->stream, R);
   bool sh = d.shake && (d.sp[0].rigid.size() || d.sp[1].rigid.size() || d.sp[2].rigid.size());
   hipLaunchKernelGGL(k_drift, dim3(g), dim3(kBlk), 0, c->stream, R, sh ? 1 : 0);
   if (sh) hipLaunchKernelGGL(k_shake, dim3(grid_for(d.n_mol, 64)), dim3(64), 0, c->stream, R);
-  hipLaunchKernelGGL(k_com_weights, dim3(1), dim3(1024), 0, c->stream, R);
+  hipLaunchKernelGGL(k_com_weights, dim3(1), dim3(1024), 0, c->stream, R, nh);
   hipLaunchKernelGGL(k_rebuild, dim3(1), dim3(1024), 0, c->stream, R, 0);
   int rc = launch_terms<Rl>(c, h, R, d.flags, eflag);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_kick, dim3(g), dim3(kBlk), 0, c->stream, R, 0, 1);
+  hipLaunchKernelGGL(k_kick, dim3(g), dim3(kBlk), 0, c->stream, R, 0, 1, nh);
   HIPCHK(hipGetLastError());
   return MDN_OK;
 }
@@ -1296,6 +1395,44 @@ static int launch_step(mdn_ctx *c, const HostReplica &h, const DevReplica *R, bo
 }  // namespace mdn
 
 using namespace mdn;
+
+extern "C" int mdn_get_state(mdn_ctx *c, int32_t r, mdn_state *st);
+
+namespace mdn {
+// the device's state and counters of replica r into its host copy (the input
+// of the next upload); with state == false only the counters
+static int pull_replica(mdn_ctx *c, int32_t r, bool state) {
+  HostReplica &h = c->reps[r];
+  const DevReplica &R = h.dev;
+  Dyn dyn;
+  HIPCHK(hipMemcpy(&dyn, R.dyn, sizeof(Dyn), hipMemcpyDeviceToHost));
+  h.atom_steps = dyn.atom_steps;
+  h.status = dyn.status;
+  h.shake_iters_max = dyn.shake_iters_max;
+  h.shake_fail = dyn.shake_fail;
+  if (!state) return MDN_OK;
+  std::vector<int> gi(2 * (size_t)std::max(h.cap_gi, 1)), gv(2 * (size_t)std::max(h.cap_gv, 1)),
+      gg(2 * (size_t)std::max(h.cap_gg, 1));
+  mdn_state st;
+  memset(&st, 0, sizeof(st));
+  st.mol_x = h.mol_x.data(); st.mol_v = h.mol_v.data(); st.region = h.region.data();
+  st.atom_x = h.atom_x.data(); st.atom_v = h.atom_v.data(); st.atom_f = h.atom_f.data();
+  st.atom_w = h.atom_w.data();
+  st.cap_gi = h.cap_gi; st.cap_gv = h.cap_gv; st.cap_gg = h.cap_gg;
+  st.gi = gi.data(); st.gv = gv.data(); st.gg = gg.data();
+  int rc = mdn_get_state(c, r, &st);
+  if (rc && !(rc == MDN_E_CAPACITY && h.status)) return rc;  // a flagged replica keeps its flags
+  h.loopPair = st.loopPair;
+  h.loop_update = st.loop_update;
+  h.rebuild_count = st.rebuild_count;
+  h.zeta_ion = st.zeta_ion;
+  for (int k = 0; k < U_N; k++) h.U[k] = st.U[k];
+  h.gi.assign(gi.begin(), gi.begin() + 2 * (size_t)st.n_gi);
+  h.gv.assign(gv.begin(), gv.begin() + 2 * (size_t)st.n_gv);
+  h.gg.assign(gg.begin(), gg.begin() + 2 * (size_t)st.n_gg);
+  return MDN_OK;
+}
+}  // namespace mdn
 
 extern "C" {
 
@@ -1357,6 +1494,7 @@ static int copy_state_in(HostReplica &h, const mdn_state *st) {
   h.loopPair = st->loopPair;
   h.loop_update = st->loop_update;
   h.rebuild_count = st->rebuild_count;
+  h.zeta_ion = st->zeta_ion;
   for (int k = 0; k < U_N; k++) h.U[k] = st->U[k];
   h.gi.assign(st->gi ? st->gi : nullptr, st->gi ? st->gi + 2 * (size_t)st->n_gi : nullptr);
   h.gv.assign(st->gv ? st->gv : nullptr, st->gv ? st->gv + 2 * (size_t)st->n_gv : nullptr);
@@ -1384,6 +1522,9 @@ int mdn_add_replica(mdn_ctx *c, const mdn_sysdef *sd, const mdn_state *st, uint6
   h.def = *reinterpret_cast<const SysDef *>(sd);
   h.seed = seed;
   h.replica_id = replica_id;
+  // the condition file's NVTion line (until mdn_set_thermostat changes it)
+  h.thermo = h.def.nvt_ion_mode;
+  h.Tnh = h.def.Tnh_ion;
   int rc = copy_state_in(h, st);
   if (rc) { c->reps.pop_back(); return rc; }
   c->committed = false;
@@ -1403,10 +1544,46 @@ int mdn_commit(mdn_ctx *c) {
 int mdn_set_state(mdn_ctx *c, int32_t r, const mdn_state *st) {
   int rc = check_replica(c, r);
   if (rc) return rc;
+  HIPCHK(hipSetDevice(c->device));
   HIPCHK(hipStreamSynchronize(c->stream));
+  // the device holds every replica's current state: bring the others' (and
+  // every replica's counters) back to the host copies the re-upload reads
+  if (c->committed)
+    for (int32_t q = 0; q < (int32_t)c->reps.size(); q++)
+      if ((rc = pull_replica(c, q, q != r))) return rc;
   rc = copy_state_in(c->reps[r], st);
   if (rc) return rc;
   return mdn_commit(c);
+}
+
+// MD::setNVE (nvt.cpp:38-43) and the NVTion modes (setCondition.cpp:158-172)
+// for one replica (r >= 0) or all (r = -1); Nose-Hoover also resets zeta
+int mdn_set_thermostat(mdn_ctx *c, int32_t r, int32_t mode, double Tnh) {
+  if (!c) return set_error(MDN_E_ARG, "mdn_set_thermostat: null context");
+  if (mode != MDN_NVE && mode != MDN_VSCALE && mode != MDN_NOSE_HOOVER)
+    return set_error(MDN_E_ARG, "mdn_set_thermostat: mode must be MDN_NVE, MDN_VSCALE or MDN_NOSE_HOOVER");
+  if (mode == MDN_NOSE_HOOVER && !(Tnh >= 0)) return set_error(MDN_E_ARG, "mdn_set_thermostat: Tnh must be >= 0");
+  int r0 = r, r1 = r + 1;
+  if (r == -1) { r0 = 0; r1 = (int)c->reps.size(); }
+  else if (int rc = check_replica(c, r)) return rc;
+  HIPCHK(hipSetDevice(c->device));
+  drop_graphs(c);
+  for (int q = r0; q < r1; q++) {
+    HostReplica &h = c->reps[q];
+    h.thermo = mode;
+    if (mode == MDN_NOSE_HOOVER) { h.Tnh = Tnh; h.zeta_ion = 0; }
+    if (c->committed) {
+      // read-modify-write of the replica's Dyn (every stream has joined
+      // c->stream, so no kernel holds it)
+      Dyn dyn;
+      HIPCHK(hipStreamSynchronize(c->stream));
+      HIPCHK(hipMemcpy(&dyn, h.dev.dyn, sizeof(Dyn), hipMemcpyDeviceToHost));
+      dyn.thermo = mode;
+      if (mode == MDN_NOSE_HOOVER) { dyn.zeta_ion = 0; dyn.Tnh_ion = Tnh; }
+      HIPCHK(hipMemcpy(h.dev.dyn, &dyn, sizeof(Dyn), hipMemcpyHostToDevice));
+    }
+  }
+  return MDN_OK;
 }
 
 int mdn_get_state(mdn_ctx *c, int32_t r, mdn_state *st) {
@@ -1430,6 +1607,7 @@ int mdn_get_state(mdn_ctx *c, int32_t r, mdn_state *st) {
   st->loopPair = dyn.loopPair;
   st->loop_update = dyn.loop_update;
   st->rebuild_count = dyn.rebuild_count;
+  st->zeta_ion = dyn.zeta_ion;
   for (int k = 0; k < U_N; k++) st->U[k] = dyn.U[k];
   std::vector<int> ig(std::max(dyn.n_in_gas, 1)), iv(std::max(dyn.n_in_vap, 1));
   if (dyn.n_in_gas) HIPCHK(hipMemcpy(ig.data(), R.in_gas, sizeof(int) * dyn.n_in_gas, hipMemcpyDeviceToHost));
@@ -1487,7 +1665,8 @@ int mdn_compute(mdn_ctx *c, int32_t r, uint32_t bits, int32_t eflag, int32_t pre
 int mdn_velocity_calculation(mdn_ctx *c, int32_t r) {
   int rc = check_replica(c, r);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_kick, dim3(grid_for(c->reps[r].def.n_atoms)), dim3(kBlk), 0, c->stream, c->d_reps + r, 0, 0);
+  hipLaunchKernelGGL(k_kick, dim3(grid_for(c->reps[r].def.n_atoms)), dim3(kBlk), 0, c->stream, c->d_reps + r, 0, 0,
+                     0);
   HIPCHK(hipGetLastError());
   return MDN_OK;
 }
@@ -1500,7 +1679,7 @@ int mdn_update_position(mdn_ctx *c, int32_t r) {
   const DevReplica *R = c->d_reps + r;
   hipLaunchKernelGGL(k_drift, dim3(grid_for(h.def.n_atoms)), dim3(kBlk), 0, c->stream, R, sh ? 1 : 0);
   if (sh) hipLaunchKernelGGL(k_shake, dim3(grid_for(h.def.n_mol, 64)), dim3(64), 0, c->stream, R);
-  hipLaunchKernelGGL(k_com_weights, dim3(1), dim3(1024), 0, c->stream, R);
+  hipLaunchKernelGGL(k_com_weights, dim3(1), dim3(1024), 0, c->stream, R, 0);
   HIPCHK(hipGetLastError());
   return MDN_OK;
 }

@@ -46,6 +46,8 @@ struct SysDef {
   int observe = 10000000;
   std::string output_name, nvt_ion, nvt_gas;
   double cluster_radius = 0, cluster_density = 0;
+  int nvt_ion_mode = 0;  // 0 off, 1 velocity scaling, 2 Nose-Hoover
+  double Tnh_ion = 0;
 
   void finalize_views();
   int mol_offset(int i) const;

@@ -28,6 +28,11 @@ static const double Overlap = 10;
 #define Ri2 (Rinter * Rinter)
 #define RI2 ((Rinter + Overlap) * (Rinter + Overlap))
 
+/* constants.hpp:30, 49-50; observer.hpp:6 */
+static const double kb_real = 1.38e-23 * 6.02e23 / 4184.0;
+static const double real_to_kcalmol = 10000 / 4.184;
+#define kb_real_inv (1 / kb_real)
+
 static double qqrd2e(void) {
   return echarge * echarge / 4.0 / M_PI / e0 * Nw * 1e10 / 4184.0;
 }
@@ -1138,13 +1143,54 @@ int orc_force_term(orc_ctx *c, orc_state *st, uint32_t bit, int eflag) {
 static const uint32_t ORDER[] = {ORC_GI, ORC_VI, ORC_GG, ORC_GV, ORC_VV, ORC_EFIELD, ORC_ION_TERSOFF,
                                  ORC_ION_SW, ORC_ION_AMBER, ORC_ION_BMH, ORC_VAPOR_INTRA, ORC_GAS_INTRA};
 
+/* ---------------- ion thermostat (relaxation phase) --------------------- */
+/* Tion of Observer::computeProps, observer.cpp:27-36 */
+double orc_ion_temperature(const orc_ctx *c, const orc_state *st) {
+  const orc_system *s = &c->s;
+  int n = s->natoms[0];
+  double K = 0;
+  for (int i = 0; i < n; i++) {
+    const double *v = st->atom_v + 3 * i;
+    double m = s->mass[0][i];
+    K += v[0] * v[0] * m;
+    K += v[1] * v[1] * m;
+    K += v[2] * v[2] * m;
+  }
+  K *= 0.5 * real_to_kcalmol;
+  return K / (double)n * (kb_real_inv / 1.5);
+}
+
+static void scale_ion_velocities(const orc_ctx *c, orc_state *st, double r) {
+  for (int i = 0; i < 3 * c->s.natoms[0]; i++) st->atom_v[i] *= r;
+}
+
+/* MD::velocity_scaling, nvt.cpp:5-16 (target 300 K) */
+void orc_velocity_scaling(orc_ctx *c, orc_state *st) {
+  double Tp = 300;
+  scale_ion_velocities(c, st, sqrt(Tp / orc_ion_temperature(c, st)));
+}
+
+/* MD::nosehoover_zeta, nvt.cpp:18-24 */
+void orc_nosehoover_zeta(orc_ctx *c, orc_state *st) {
+  double Tion = orc_ion_temperature(c, st);
+  int g = c->s.natoms[0] * 3;
+  double Q_inv = 0.0001;
+  st->zeta_ion += (Tion - c->s.Tnh_ion) * g * kb_real * Q_inv * c->s.dt;
+}
+
+/* MD::nosehoover_ion, nvt.cpp:27-35 */
+void orc_nosehoover_ion(orc_ctx *c, orc_state *st) {
+  scale_ion_velocities(c, st, exp(-st->zeta_ion * 0.5 * c->s.dt));
+}
+
 int orc_forces(orc_ctx *c, orc_state *st, int eflag) {
   for (unsigned k = 0; k < sizeof(ORDER) / sizeof(ORDER[0]); k++)
     if (c->s.flags & ORDER[k]) orc_force_term(c, st, ORDER[k], eflag);
   return 0;
 }
 
-/* MD::verlet, verlet.cpp:20-39 (+ energy cadence of run.cpp:68-71) */
+/* MD::verlet, verlet.cpp:20-39 (+ energy cadence of run.cpp:68-71), with
+ * the ion thermostat where the reference calls it (verlet.cpp:22, 27, 36) */
 int orc_verlet(orc_ctx *c, orc_state *st, int nsteps, int eflag_every, int64_t step0, int64_t *atom_steps) {
   const orc_system *s = &c->s;
   int64_t acc = 0;
@@ -1153,8 +1199,10 @@ int orc_verlet(orc_ctx *c, orc_state *st, int nsteps, int eflag_every, int64_t s
     int eflag = eflag_every > 0 && ((step0 + it + 1) % eflag_every) == 0;
     if (eflag) for (int k = 0; k < U_N; k++) st->U[k] = 0;
     orc_velocity_calculation(c, st);
+    if (s->nvt_ion == 1) orc_velocity_scaling(c, st);
     if (s->shake) orc_update_position_shake(c, st);
     else orc_update_position(c, st);
+    if (s->nvt_ion == 2) orc_nosehoover_zeta(c, st);
     int rb = orc_check_pairlist(c, st);
     if (rb < 0) return -2;
     if (rb || n_in < 0) {
@@ -1164,6 +1212,7 @@ int orc_verlet(orc_ctx *c, orc_state *st, int nsteps, int eflag_every, int64_t s
     }
     orc_forces(c, st, eflag);
     orc_velocity_calculation(c, st);
+    if (s->nvt_ion == 2) orc_nosehoover_ion(c, st);
     acc += n_in;
     if (st->status) return st->status;
   }

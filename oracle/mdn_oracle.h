@@ -84,6 +84,10 @@ typedef struct orc_system {
   int shake_max_iter;
   uint64_t seed;
   uint32_t replica;
+  int nvt_ion;              /* relaxation thermostat of the ion (NVTion,
+                               setCondition.cpp:158-172): 0 off, 1 velocity
+                               scaling to 300 K, 2 Nose-Hoover at Tnh_ion */
+  double Tnh_ion;
 } orc_system;
 
 typedef struct orc_state {
@@ -100,6 +104,7 @@ typedef struct orc_state {
   int shake_iters_max;      /* diagnostics: max SHAKE iterations seen */
   int64_t shake_failures;   /* SHAKE calls that hit the cap or diverged */
   int status;               /* 0 ok, <0 error */
+  double zeta_ion;          /* Nose-Hoover friction of the ion (vars->zeta_ion) */
 } orc_state;
 
 typedef struct orc_ctx orc_ctx;
@@ -131,6 +136,12 @@ void orc_velocity_calculation(orc_ctx *c, orc_state *st);
 void orc_update_position(orc_ctx *c, orc_state *st);
 void orc_update_position_shake(orc_ctx *c, orc_state *st);
 void orc_ion_com_weights(orc_ctx *c, orc_state *st);
+/* ion thermostat (nvt.cpp): Tion of Observer::computeProps (observer.cpp:
+ * 27-36), MD::velocity_scaling, MD::nosehoover_zeta, MD::nosehoover_ion */
+double orc_ion_temperature(const orc_ctx *c, const orc_state *st);
+void orc_velocity_scaling(orc_ctx *c, orc_state *st);
+void orc_nosehoover_zeta(orc_ctx *c, orc_state *st);
+void orc_nosehoover_ion(orc_ctx *c, orc_state *st);
 int orc_make_pair(orc_ctx *c, orc_state *st);
 int orc_check_pairlist(orc_ctx *c, orc_state *st);
 /* every registered Potential::compute in registration order; adds to atom_f */

@@ -43,7 +43,7 @@ class OrcSystem(C.Structure):
         ("ML2", C.c_double), ("m_gas_kg", C.c_double), ("m_vap_kg", C.c_double),
         ("gas_spin", C.c_int), ("flags", C.c_uint32), ("efield", C.c_double * 3),
         ("shake", C.c_int), ("shake_max_iter", C.c_int),
-        ("seed", C.c_uint64), ("replica", C.c_uint32),
+        ("seed", C.c_uint64), ("replica", C.c_uint32), ("nvt_ion", C.c_int), ("Tnh_ion", C.c_double),
     ]
 
 
@@ -56,6 +56,7 @@ class OrcState(C.Structure):
         ("n_gi", C.c_int), ("n_gv", C.c_int), ("n_gg", C.c_int),
         ("gi", _ip), ("gv", _ip), ("gg", _ip),
         ("U", C.c_double * 8), ("shake_iters_max", C.c_int), ("shake_failures", C.c_int64), ("status", C.c_int),
+        ("zeta_ion", C.c_double),
     ]
 
 
@@ -75,7 +76,7 @@ def lib():
             getattr(L, fn).argtypes = [C.c_void_p, C.POINTER(OrcState)]
             getattr(L, fn).restype = C.c_int
         for fn in ("orc_velocity_calculation", "orc_update_position", "orc_update_position_shake",
-                   "orc_ion_com_weights"):
+                   "orc_ion_com_weights", "orc_velocity_scaling", "orc_nosehoover_zeta", "orc_nosehoover_ion"):
             getattr(L, fn).argtypes = [C.c_void_p, C.POINTER(OrcState)]
             getattr(L, fn).restype = None
         L.orc_forces.argtypes = [C.c_void_p, C.POINTER(OrcState), C.c_int]
@@ -91,6 +92,8 @@ def lib():
         L.orc_rotation.argtypes = [_dp, _dp, _dp] + [C.c_double] * 6
         L.orc_weight.argtypes = [C.c_double]
         L.orc_weight.restype = C.c_double
+        L.orc_ion_temperature.argtypes = [C.c_void_p, C.POINTER(OrcState)]
+        L.orc_ion_temperature.restype = C.c_double
         L.orc_n_long.argtypes = [C.c_void_p]
         L.orc_n_long.restype = C.c_int
         L.orc_rng_normal2.argtypes = [C.POINTER(OrcSystem), C.c_uint32, C.c_int64, C.c_uint32, _dp]
@@ -154,6 +157,8 @@ class Oracle:
         s.shake_max_iter = int(sysd.get("shake_max_iter", 1000))
         s.seed = int(sysd.get("seed", 1))
         s.replica = int(sysd.get("replica", 0))
+        s.nvt_ion = int(sysd.get("nvt_ion", 0))
+        s.Tnh_ion = float(sysd.get("Tnh_ion", 0.0))
         self.s = s
         self.h = lib().orc_create(C.byref(s))
         self.n_mol = 1 + s.n_gas + s.n_vap
@@ -198,6 +203,9 @@ class Oracle:
         S.sync()
         return rc, a.value
 
+    def ion_temperature(self, S):
+        return lib().orc_ion_temperature(self.h, C.byref(S.c))
+
     def velocity_calculation(self, S):
         lib().orc_velocity_calculation(self.h, C.byref(S.c))
 
@@ -234,6 +242,7 @@ class OracleState:
         c.loopPair = int(st.get("loopPair", 0))
         c.loop_update = int(st.get("loop_update", 0))
         c.rebuild_count = int(st.get("rebuild_count", 0))
+        c.zeta_ion = float(st.get("zeta_ion", 0.0))
         c.cap_gi = c.cap_gv = c.cap_gg = cap
         c.gi, c.gv, c.gg = (a.ctypes.data_as(_ip) for a in (self.gi, self.gv, self.gg))
         for k, nmk in (("gi", "n_gi"), ("gv", "n_gv"), ("gg", "n_gg")):
@@ -257,7 +266,8 @@ class OracleState:
 
     def as_dict(self):
         d = {k: getattr(self, k).copy() for k in ("mol_x", "mol_v", "region", "atom_x", "atom_v", "atom_f", "atom_w")}
-        d.update(loopPair=self.c.loopPair, loop_update=self.c.loop_update, rebuild_count=self.c.rebuild_count)
+        d.update(loopPair=self.c.loopPair, loop_update=self.c.loop_update, rebuild_count=self.c.rebuild_count,
+                 zeta_ion=self.c.zeta_ion)
         d.update({k: v.reshape(-1) for k, v in self.lists().items()})
         return d
 
@@ -313,6 +323,8 @@ def sysd_from_ref_dump(a, has_gas=True, seed=1, replica=0, shake=0):
         "ML2": sc[6], "m_gas_kg": sc[7], "m_vap_kg": sc[8], "gas_spin": 1 if int(sc[9]) == 2 else 0,
         "flags": flags, "efield": tuple(a.get("efield", np.zeros(3))), "shake": shake, "shake_max_iter": 1000, "seed": seed,
         "replica": replica, "n_ion": n_ion,
+        # (mode, Tnh) of the NVTion line; absent from dumps that predate it
+        "nvt_ion": int(a["nvt"][0]) if "nvt" in a else 0, "Tnh_ion": float(a["nvt"][1]) if "nvt" in a else 0.0,
     }
 
 

@@ -130,6 +130,8 @@ static void load_state(MD *md, Arrays &a) {
   md->loopPair = (int)need(a, "loopPair")[0];
   md->loop_update = (int)need(a, "loop_update")[0];
   md->margin_length = need(a, "margin_length")[0];
+  // Variables never initialises zeta_ion; the state sets it (default 0)
+  v->zeta_ion = a.count("zeta_ion") ? a["zeta_ion"][0] : 0.0;
   v->Uzero();
 }
 
@@ -155,6 +157,7 @@ static void dump_state(FILE *f, MD *md) {
   put(f, "loop_update", {(double)md->loop_update});
   const Potentials &U = v->Utotal;
   put(f, "U", {U.Uion, U.Ugas, U.Uvap, U.Ugi, U.Ugg, U.Uvg, U.Uvi, U.Uvv});
+  put(f, "zeta_ion", {v->zeta_ion});
   for (auto *p : md->InterInter) {
     std::vector<double> l;
     for (auto &q : p->pairList) l.push_back(q.i), l.push_back(q.j);
@@ -217,6 +220,8 @@ static void dump_sys(FILE *f, MD *md) {
                    (double)md->flags->inter_vg, (double)md->flags->inter_vv, (double)md->flags->intra_AMBER,
                    (double)md->flags->force_born, (double)md->flags->force_ters, (double)md->flags->force_sw,
                    (double)md->flags->efield, (double)md->flags->vapor_intra, (double)md->flags->gas_intra});
+  // NVTion (setCondition.cpp:158-172): 0 off, 1 velocity scaling, 2 Nose-Hoover
+  put(f, "nvt", {(double)(md->flags->nose_hoover_ion ? 2 : md->flags->velocity_scaling ? 1 : 0), md->pp->Tnh_ion});
 }
 
 static double now() {
@@ -236,6 +241,7 @@ int main(int argc, char **argv) {
   FILE *out = fdopen(dup(1), "w");
   if (!freopen("/dev/null", "w", stdout)) return 4;
   MD *md = new MD((char *)cond, 0);
+  md->vars->zeta_ion = 0;
   md->margin_length = md->MARGIN;
   std::string o(op);
   if (o == "sys") {

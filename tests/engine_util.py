@@ -14,6 +14,7 @@ def oracle_sysd(v, seed=1, replica=0):
         "margin": v["margin"], "ML2": v["ML2"], "m_gas_kg": v["m_gas_kg"], "m_vap_kg": v["m_vap_kg"],
         "gas_spin": v["gas_spin"], "flags": v["flags"], "efield": v["efield"], "shake": v["shake"],
         "shake_max_iter": v["shake_max_iter"], "seed": seed, "replica": replica,
+        "nvt_ion": v.get("nvt_ion", 0), "Tnh_ion": v.get("Tnh_ion", 0.0),
     }
 
 

@@ -16,6 +16,12 @@ def write_case(name, d):
         pep = configs.write_peptide(os.path.join(d, "pep40.txt"), seed=3, n=40)
         return configs.Condition(pep, m["water"], 20, "N2", 200, pressure=1e6, ion_potential="AMBER").write(
             os.path.join(d, "pep.cond"))
+    if name in ("nvt_scale", "nvt_nh"):
+        # the pep system under the relaxation thermostat of the ion
+        pep = configs.write_peptide(os.path.join(d, "pep40.txt"), seed=3, n=40)
+        nvt = "scale" if name == "nvt_scale" else "250"
+        return configs.Condition(pep, m["water"], 20, "N2", 200, pressure=1e6, ion_potential="AMBER",
+                                 nvt_ion=nvt).write(os.path.join(d, name + ".cond"))
     if name == "bmh":
         nacl = configs.write_nacl_cluster(os.path.join(d, "nacl.txt"))
         return configs.Condition(nacl, m["water"], 1, "He", 1, ion_potential="Born-Mayer-Huggins-NaCl",

@@ -14,7 +14,11 @@ selected term, a no-rebuild trajectory with energies, one MD::make_pair):
   pep     40-atom AMBER chain + 20 water (SHAKE H) + 200 N2, ion AMBER
   bmh     8-ion NaCl cluster, PotentialBorn::compute alone (c1 carries Efield)
   si      64-atom Si cluster: Tersoff and Stillinger-Weber (own lists)
+  nvt     the pep system with "NVTion scale" and "NVTion 250": parsed mode,
+          initial state, a no-rebuild trajectory with the ion thermostat
 plus SHAKE on perturbed methanol / water molecules and ROTATION samples.
+
+    python tests/golden/make_golden.py nvt    # rewrites only nvt.npz
 """
 import os
 import subprocess
@@ -82,10 +86,36 @@ def case(name, cond, d, nsteps=20, every=5, loop_pair=37, shake_mols=()):
     print(name, {k: v.size for k, v in res.items() if k.startswith(("f_list", "s_U", "f_U"))})
 
 
+def nvt_cases(d, nsteps=30, every=10):
+    """MD::verlet with velocity scaling (nvt.cpp:5-16) and with Nose-Hoover
+    (nvt.cpp:18-35) from the same kind of state; zeta_ion starts at 0."""
+    res = {}
+    for name in ("nvt_scale", "nvt_nh"):
+        cond = write_case(name, d)
+        sysa = harness(cond, None, "sys", cwd=d)
+        init = harness(cond, None, "init", cwd=d)
+        st = {k: init[k] for k in STATE_KEYS}
+        sp = os.path.join(d, "state.txt")
+        O.write_arrays(sp, dict(st, loopPair=[0], loop_update=[10 ** 8], margin_length=[10.0], eflag_every=[every],
+                                zeta_ion=[0.0]))
+        s = harness(cond, sp, "steps", nsteps, cwd=d)
+        res.update(pack(name + "_sys_", sysa))
+        res.update(pack(name + "_init_", st))
+        res.update(pack(name + "_s_", s, STATE_KEYS + ("U", "zeta_ion", "list_gi", "list_gv")))
+        res[name + "_s_nsteps"] = np.array([nsteps, every], np.float64)
+    np.savez_compressed(os.path.join(OUT, "nvt.npz"), **res)
+    print("nvt", {k: v.size for k, v in res.items() if "zeta" in k or k.endswith("nvt")})
+
+
 def main():
     if not os.path.exists(O.REF_HARNESS):
         subprocess.check_call(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "ref"])
+    if sys.argv[1:] == ["nvt"]:
+        with tempfile.TemporaryDirectory() as d:
+            nvt_cases(d)
+        return
     with tempfile.TemporaryDirectory() as d:
+        nvt_cases(d)
         case("c1", write_case("c1", d), d, shake_mols=(601, 605))
         case("pep", write_case("pep", d), d, shake_mols=(201, 210))
         # the reference needs a gas line; its BMH table is indexed by ion-local

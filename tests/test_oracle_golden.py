@@ -136,3 +136,69 @@ def test_rotation():
         O.lib().orc_rotation(O.C.byref(X), O.C.byref(Y), O.C.byref(Z), *[float(v) for v in r])
         out += [X.value, Y.value, Z.value]
     np.testing.assert_array_equal(np.array(out), fx["rot_out"])
+
+
+# ---- ion thermostat of the relaxation phase (nvt.cpp) ----------------------
+NVT = ["nvt_scale", "nvt_nh"]
+
+
+@pytest.mark.parametrize("name", NVT)
+def test_thermostat_trajectory(name):
+    """MD::verlet with NVTion scale / NVTion 250: the reference's own 30-step
+    trajectory (positions, velocities, forces, energies, zeta_ion)."""
+    fx = G.sub(G.load("nvt"), name + "_")
+    orc, sysd = G.oracle_for(fx)
+    assert sysd["nvt_ion"] == (1 if name == "nvt_scale" else 2)
+    S = orc.new_state(G.state(fx, loopPair=0, loop_update=10 ** 8))
+    G.set_lists(S, fx, prefix="s_")
+    nsteps, every = (int(v) for v in fx["s_nsteps"])
+    rc, _ = orc.verlet(S, nsteps, eflag_every=every)
+    assert rc == 0
+    for k in ("atom_x", "atom_v", "atom_f", "atom_w", "mol_x", "mol_v"):
+        assert rel(getattr(S, k), fx["s_" + k]) < 1e-12, k
+    np.testing.assert_allclose(S.U, fx["s_U"], rtol=1e-10, atol=1e-10)
+    assert abs(S.c.zeta_ion - fx["s_zeta_ion"][0]) <= 1e-12 * max(abs(fx["s_zeta_ion"][0]), 1e-6)
+    # the thermostat changed the trajectory: NVE from the same state differs
+    orc0, _ = G.oracle_for(fx, nvt_ion=0)
+    S0 = orc0.new_state(G.state(fx, loopPair=0, loop_update=10 ** 8))
+    G.set_lists(S0, fx, prefix="s_")
+    orc0.verlet(S0, nsteps, eflag_every=every)
+    assert rel(S0.atom_v, fx["s_atom_v"]) > 1e-6
+
+
+def _kinetic_T(sysd, v):
+    """Equipartition temperature of the ion from its atom velocities:
+    K = 1/2 sum m v^2 (g/mol A^2/fs^2 -> kcal/mol x 1e4/4.184),
+    T = 2 K / (3 N kB) with kB = 1.38e-23 * 6.02e23 / 4184 kcal/mol/K."""
+    m = np.asarray(sysd["species"][0]["mass"], np.float64)
+    n = m.size
+    K = 0.5 * np.sum(m[:, None] * v[: 3 * n].reshape(-1, 3) ** 2) * 1e4 / 4.184
+    return 2 * K / (3 * n * (1.38e-23 * 6.02e23 / 4184.0))
+
+
+def test_thermostat_pieces_closed_forms():
+    """Tion is the equipartition temperature; velocity scaling lands on
+    exactly 300 K; the Nose-Hoover zeta step is (Tion - Tnh) * 3N * kB *
+    1e-4 * dt and the velocity factor exp(-zeta dt / 2)."""
+    fx = G.sub(G.load("nvt"), "nvt_nh_")
+    orc, sysd = G.oracle_for(fx)
+    S = orc.new_state(G.state(fx))
+    v0 = S.atom_v.copy()
+    T0 = orc.ion_temperature(S)
+    assert abs(T0 - _kinetic_T(sysd, v0)) < 1e-10 * T0
+    assert 50 < T0 < 2000
+    n = sysd["species"][0]["mass"].size
+    O.lib().orc_velocity_scaling(orc.h, O.C.byref(S.c))
+    assert abs(orc.ion_temperature(S) - 300.0) < 1e-9
+    ratio = S.atom_v[: 3 * n] / v0[: 3 * n]
+    np.testing.assert_allclose(ratio, np.sqrt(300.0 / T0), rtol=1e-13)
+    np.testing.assert_array_equal(S.atom_v[3 * n:], v0[3 * n:])  # only the ion
+    S.atom_v[:] = v0
+    S.c.zeta_ion = 0.01
+    O.lib().orc_nosehoover_zeta(orc.h, O.C.byref(S.c))
+    kB = 1.38e-23 * 6.02e23 / 4184.0
+    want = 0.01 + (T0 - sysd["Tnh_ion"]) * 3 * n * kB * 1e-4 * sysd["dt"]
+    assert abs(S.c.zeta_ion - want) < 1e-15
+    O.lib().orc_nosehoover_ion(orc.h, O.C.byref(S.c))
+    np.testing.assert_allclose(S.atom_v[: 3 * n], v0[: 3 * n] * np.exp(-want * 0.5 * sysd["dt"]), rtol=1e-14)
+    np.testing.assert_array_equal(S.atom_v[3 * n:], v0[3 * n:])
