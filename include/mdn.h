@@ -160,6 +160,53 @@ int mdn_get_state(mdn_ctx *c, int32_t replica, mdn_state *st);
  * A replica starts in the mode of its condition file's NVTion line. */
 int mdn_set_thermostat(mdn_ctx *c, int32_t replica, int32_t mode, double Tnh);
 
+/* ---- observables and outputs (output.cpp, observer.cpp, run.cpp) ------- */
+/* Observer::computeProps (observer.cpp:5-36) for one replica or all (-1):
+ * out[6 per replica] = Kion, K_g, K_v (kcal/mol), Tion, T_g, T_v (K).  Gas
+ * and vapor use every molecule's COM velocity (updated at rebuilds, as in the
+ * reference) and declared molecule mass. */
+int mdn_observe(mdn_ctx *c, int32_t replica, double *out);
+
+/* One MD::Ovin / MD::Ovout record (output.cpp:43-64, written at each rebuild
+ * by updateVaporIn.cpp:33, 39) */
+typedef struct mdn_vapor_event {
+  int32_t mol;     /* molecule id (0 = ion, gas 1..n_gas, vapor after) */
+  int32_t kind;    /* 0: was in and stays in (vapor_in file), 1: leaves (vapor_out) */
+  int64_t step;    /* the step counter (MD::itime) at the rebuild; time = step * dt */
+  double dx[3];    /* position[mol] - position[0] */
+  double dv[3];    /* velocity[mol] - velocity[0] */
+} mdn_vapor_event;
+/* records up to cap events per replica between drains (0: off, the
+ * default); re-uploads the context when called after mdn_commit.  Events not
+ * drained are lost by mdn_set_state and mdn_set_vapor_log. */
+int mdn_set_vapor_log(mdn_ctx *c, int32_t cap);
+/* copies the replica's events recorded since the last drain (in rebuild
+ * order, unordered within one rebuild) and clears them; MDN_E_CAPACITY if
+ * out holds fewer than recorded (nothing drained) or if events were dropped
+ * (the kept ones are returned) */
+int mdn_drain_vapor_log(mdn_ctx *c, int32_t replica, mdn_vapor_event *out, int32_t cap, int32_t *n);
+/* sets the step counter (MD::itime, run.cpp:22, 41, 61) that stamps vapor
+ * events; every step advances it by one; replica -1: all */
+int mdn_set_step(mdn_ctx *c, int32_t replica, int64_t step);
+/* the re-centre of run.cpp:42-53 as written (only position[0] becomes 0) */
+int mdn_recentre(mdn_ctx *c, int32_t replica);
+
+/* MD::run (run.cpp:18-75) for every replica of the context as calculation
+ * number calc0 + r (main.cpp:25-29): RelaxSteps under the NVTion thermostat
+ * with display/dump every Output steps, the re-centre, MD::make_pair,
+ * MD::setNVE, then TotalSteps with the ion file every 10000 steps and
+ * display/dump every Output steps, energies on the steps before.  Writes the
+ * reference's files under outdir (output.cpp): K_<n>.dat, U_<n>.dat,
+ * ion_<T>_<n>.dat, <Output name>_<n>.dump, vapor_in_<n>.dat,
+ * vapor_out_<n>.dat (and an empty vapor_collision_<n>.dat).  Steps through
+ * mdn_run_batched in the given precision.  Call it where MD::MD ends: after
+ * mdn_commit and mdn_make_pair, with the vapor log enabled first if the
+ * constructor's rebuild is to be logged (mdn_run enables it otherwise).
+ * Replicas must agree on RelaxSteps, TotalSteps and the Output cadence.
+ * opts: MDN_RUN_VERBOSE prints each display() block to stdout. */
+enum { MDN_RUN_VERBOSE = 1 };
+int mdn_run(mdn_ctx *c, const char *outdir, int32_t calc0, int32_t precision, uint32_t opts);
+
 /* drop-in path: one replica, FP64, one launch per reference call */
 int mdn_make_pair(mdn_ctx *c, int32_t replica);
 int mdn_zero_forces(mdn_ctx *c, int32_t replica);

@@ -54,6 +54,11 @@ class MdnState(C.Structure):
 
 
 # every entry point of include/mdn.h with its argument types
+class MdnVaporEvent(C.Structure):
+    _fields_ = [("mol", C.c_int32), ("kind", C.c_int32), ("step", C.c_int64), ("dx", C.c_double * 3),
+                ("dv", C.c_double * 3)]
+
+
 _SIGS = {
     "mdn_last_error": (C.c_char_p, []),
     "mdn_sysdef_load": (C.c_int, [C.c_char_p, C.POINTER(C.c_void_p)]),
@@ -70,6 +75,12 @@ _SIGS = {
     "mdn_set_state": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(MdnState)]),
     "mdn_get_state": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(MdnState)]),
     "mdn_set_thermostat": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_double]),
+    "mdn_observe": (C.c_int, [C.c_void_p, C.c_int32, _dp]),
+    "mdn_set_vapor_log": (C.c_int, [C.c_void_p, C.c_int32]),
+    "mdn_drain_vapor_log": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(MdnVaporEvent), C.c_int32, _ip]),
+    "mdn_set_step": (C.c_int, [C.c_void_p, C.c_int32, C.c_int64]),
+    "mdn_recentre": (C.c_int, [C.c_void_p, C.c_int32]),
+    "mdn_run": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int32, C.c_int32, C.c_uint32]),
     "mdn_make_pair": (C.c_int, [C.c_void_p, C.c_int32]),
     "mdn_zero_forces": (C.c_int, [C.c_void_p, C.c_int32]),
     "mdn_compute": (C.c_int, [C.c_void_p, C.c_int32, C.c_uint32, C.c_int32, C.c_int32]),
@@ -273,6 +284,38 @@ class Engine:
     def set_thermostat(self, r, mode, Tnh=0.0):
         """MD::setNVE (mode NVE) or an NVTion mode for replica r (-1: all)."""
         check(lib().mdn_set_thermostat(self.h, int(r), int(mode), float(Tnh)), "mdn_set_thermostat")
+
+    def observe(self, r=-1):
+        """Observer::computeProps: rows of (Kion, K_g, K_v, Tion, T_g, T_v)."""
+        n = len(self.sys) if r == -1 else 1
+        out = np.zeros(6 * n)
+        check(lib().mdn_observe(self.h, int(r), out.ctypes.data_as(_dp)), "mdn_observe")
+        return out.reshape(n, 6)
+
+    def set_vapor_log(self, cap):
+        check(lib().mdn_set_vapor_log(self.h, int(cap)), "mdn_set_vapor_log")
+
+    def drain_vapor_log(self, r, cap=1 << 16):
+        """The replica's Ovin/Ovout events since the last drain: a structured
+        array (mol, kind, step, dx[3], dv[3])."""
+        buf = (MdnVaporEvent * cap)()
+        n = C.c_int32(0)
+        check(lib().mdn_drain_vapor_log(self.h, int(r), buf, cap, C.byref(n)), "mdn_drain_vapor_log")
+        dt = np.dtype([("mol", np.int32), ("kind", np.int32), ("step", np.int64), ("dx", np.float64, 3),
+                       ("dv", np.float64, 3)])
+        return np.frombuffer(bytes(buf)[: n.value * C.sizeof(MdnVaporEvent)], dtype=dt).copy()
+
+    def set_step(self, r, step):
+        check(lib().mdn_set_step(self.h, int(r), int(step)), "mdn_set_step")
+
+    def recentre(self, r=-1):
+        check(lib().mdn_recentre(self.h, int(r)), "mdn_recentre")
+
+    def run(self, outdir, calc0=0, precision=FP64, verbose=False):
+        """MD::run for every replica (relaxation, re-centre, make_pair, setNVE,
+        main loop) writing the reference's output files under outdir."""
+        check(lib().mdn_run(self.h, str(outdir).encode(), int(calc0), int(precision), 1 if verbose else 0),
+              "mdn_run")
 
     def get_state(self, r, cap=None) -> State:
         sd = self.sys[r]

@@ -11,7 +11,7 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(PKG, "csrc")
 ROOT = os.path.dirname(PKG)
 LIB = os.path.join(PKG, "libmdn.so")
-SOURCES = ["mdn_engine.hip", "mdn_adjoint.hip", "host_sysdef.cpp"]
+SOURCES = ["mdn_engine.hip", "mdn_adjoint.hip", "host_sysdef.cpp", "host_run.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["-O3", "-std=c++17", "-fPIC", "-shared", "--offload-arch=gfx950",
          # packed FP32 is kept off for every kernel of this library

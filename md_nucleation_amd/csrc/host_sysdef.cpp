@@ -71,14 +71,28 @@ static std::string resolve(const std::string &base, const std::string &f) {
   return base + "/" + f;
 }
 
-// GAFF hydrogen names become "H" (ion) / "H(v)" (vapor) for SHAKE H
-// detection (readIonFile.cpp:52-53, readVaporFile.cpp:36-37)
-static bool is_gaff_h(const std::string &n) {
-  static const char *hs[] = {"#h1", "#h2", "#h3", "#h4", "#h5", "#ha", "#hc",
-                             "#hn", "#ho", "#hp", "#hs", "#hw", "#hx"};
-  for (const char *h : hs)
-    if (n == h) return true;
-  return n == "H";
+// Element name of a GAFF atom type, as readIonFile.cpp:48-67 and
+// readVaporFile.cpp:32-51 assign it (vapor names carry "(v)", except P);
+// types outside the table keep their own name.  SHAKE H treats an atom as
+// hydrogen by this name ("H" in the ion, "H(v)" in the vapor).
+static std::string element_name(const std::string &t, bool vapor) {
+  struct Group { const char *el; const char *types; };
+  static const Group groups[] = {
+      {"C", "#c #cs #c1 #c2 #c3 #ca #cp #cq #cc #cd #ce #cf #cg #ch #cx #cy #cu #cv #cz"},
+      {"H", "#h1 #h2 #h3 #h4 #h5 #ha #hc #hn #ho #hp #hs #hw #hx"},
+      {"F", "f"}, {"Cl", "cl"}, {"Br", "br"}, {"I", "i"},
+      {"N", "#n #n1 #n2 #n3 #n4 #na #nb #nc #nd #ne #nf #nh #no #ns #nt #nx #ny #nz #n+ #nu #nv #n7 #n8 #n9"},
+      {"O", "#o #oh #os #ow"},
+      {"P", "#p2 #p3 #p4 #p5 #pb #pc #pd #pe #pf #px #py"},
+      {"S", "#s #s2 #s4 #s6 #sh #ss #sx #sy"},
+  };
+  for (const Group &g : groups) {
+    std::istringstream ss(g.types);
+    std::string w;
+    while (ss >> w)
+      if (w == t) return std::string(g.el) + (vapor && std::string(g.el) != "P" ? "(v)" : "");
+  }
+  return t;
 }
 
 static double num(const std::vector<std::string> &r, size_t k) {
@@ -113,8 +127,8 @@ static void read_molecule_file(SysDef &d, const std::string &path, int sp, bool 
     if (r.empty()) continue;
     if (sec == 1) {
       AtomType at;
-      at.name = r.size() > 1 ? r[1] : "";
-      at.is_h = is_gaff_h(at.name);
+      at.name = element_name(r.size() > 1 ? r[1] : "", sp == 2);
+      at.is_h = at.name == (sp == 2 ? "H(v)" : "H");
       at.mass = num(r, 2);
       at.eps = num(r, 3);
       at.sigma = num(r, 4);

@@ -60,6 +60,17 @@ struct Dyn {
   int thermo, pad;           // THERMO_* of the ion
   unsigned long long shake_fail;
   double zeta_ion, Tnh_ion;  // Nose-Hoover friction and target (nvt.cpp:19-35)
+  long long step;            // MD::itime: steps since the caller last set it
+  int n_log, pad2;           // vapor events recorded since the last drain
+};
+
+// one Ovin / Ovout record (output.cpp:43-64) of a rebuild: a vapor molecule
+// that was in the domain and stays in (kind 0) or leaves (kind 1), relative
+// to the ion COM; same layout as mdn_vapor_event
+struct VapEvent {
+  int mol, kind;
+  long long step;
+  double dx[3], dv[3];
 };
 
 struct DevReplica {
@@ -84,6 +95,8 @@ struct DevReplica {
   int *gi, *gv, *gg;  // local-index lists (in-list positions), 2 ints per entry
   int cap_gi, cap_gv, cap_gg;
   Dyn *dyn;
+  VapEvent *vlog;  // [log_cap] or null (vapor events not recorded)
+  int log_cap;
 };
 
 // ------------------------------------------------------------------------

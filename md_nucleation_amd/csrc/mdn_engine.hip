@@ -64,7 +64,7 @@ __global__ void k_kick(const DevReplica *__restrict__ Rp, int zeroU, int count, 
   int t = blockIdx.x * blockDim.x + threadIdx.x, nt = gridDim.x * blockDim.x;
   if (t == 0 && zeroU)
     for (int k = 0; k < U_N; k++) R.dyn->U[k] = 0;
-  if (t == 0 && count) R.dyn->atom_steps += L.n_in;
+  if (t == 0 && count) { R.dyn->atom_steps += L.n_in; R.dyn->step++; }
   const double C = 0.5 * R.dt * kKickUnit;
   const double fac = nh ? nh_factor(R.dyn->zeta_ion, R.dt) : 1.0;
   for (int a = t; a < L.n_in; a += nt) {
@@ -151,6 +151,43 @@ __global__ void k_shake(const DevReplica *__restrict__ Rp) {
     }
     atomicMax(&R.dyn->shake_iters_max, it);
     if (R.shake == 1) shake_velocity(S, X, XO, V, R.dt);
+  }
+}
+
+// Observer::computeProps (observer.cpp:5-36), one workgroup per replica:
+// out[8 r ..] = Kion, K_g, K_v, Tion, T_g, T_v.  Gas and vapor use every
+// molecule's COM velocity (as of its last rebuild, or its draw when out of
+// the domain) and declared molecule mass; the ion its atom velocities.
+__global__ void __launch_bounds__(256) k_observe(const DevReplica *__restrict__ reps, int r0, double *out) {
+  const DevReplica &R = reps[r0 + blockIdx.x];
+  __shared__ double s_red[3][4];
+  const int tid = threadIdx.x, nt = blockDim.x;
+  double k[3] = {0, 0, 0};
+  const DevSpecies &I = R.sp[0];
+  for (int i = tid; i < I.natoms; i += nt) {
+    const double *v = R.atom_v + 3 * (size_t)i;
+    double m = I.mass[i];
+    k[0] += v[0] * v[0] * m + v[1] * v[1] * m + v[2] * v[2] * m;
+  }
+  for (int i = 1 + tid; i < R.n_mol; i += nt) {
+    const double *v = R.mol_v + 3 * (size_t)i;
+    const int sp = i <= R.n_gas ? 1 : 2;
+    double m = R.sp[sp].molmass;
+    k[sp] += v[0] * v[0] * m + v[1] * v[1] * m + v[2] * v[2] * m;
+  }
+  for (int q = 0; q < 3; q++) {
+    double sv = wave_sum(k[q]);
+    if ((tid & 63) == 0) s_red[q][tid >> 6] = sv;
+  }
+  __syncthreads();
+  if (tid < 3) {
+    double sv = 0;
+    for (int w = 0; w < nt / 64; w++) sv += s_red[tid][w];
+    const double K = sv * (0.5 * kRealToKcal);
+    const int n = tid == 0 ? I.natoms : (tid == 1 ? R.n_gas : R.n_vap);
+    double *o = out + 8 * (size_t)blockIdx.x;
+    o[tid] = K;
+    o[3 + tid] = K / (double)n * ((1 / kKbReal) / 1.5);
   }
 }
 
@@ -823,6 +860,7 @@ __global__ void __launch_bounds__(512) k_batched(const DevReplica *__restrict__ 
     const int thermo = D->thermo;
     const double Tnh = D->Tnh_ion;
     double zeta = D->zeta_ion;
+    const long long step_base = D->step;
     // SHAKE diagnostics accumulate in LDS (sint[32]: max iterations,
     // sint[33]: failures) and reach the replica's counters once per launch
     if (tid == 0) { B.sint[32] = 0; B.sint[33] = 0; }
@@ -976,7 +1014,7 @@ __global__ void __launch_bounds__(512) k_batched(const DevReplica *__restrict__ 
           __syncthreads();
           batch_store(R, L, B);
           if (tid < 3) { R.mol_x[tid] = B.sdbl[tid]; R.mol_v[tid] = B.sdbl[3 + tid]; }
-          if (tid == 0) { D->loopPair = lp; D->zeta_ion = zeta; }
+          if (tid == 0) { D->loopPair = lp; D->zeta_ion = zeta; D->step = step_base + s; }
           __syncthreads();
           rebuild_wg(R, B.sint, B.red);
           L = view_from_dyn(R);
@@ -1025,7 +1063,7 @@ __global__ void __launch_bounds__(512) k_batched(const DevReplica *__restrict__ 
     if (!dead) {
       batch_store(R, L, B);
       if (tid < 3) { R.mol_x[tid] = B.sdbl[tid]; R.mol_v[tid] = B.sdbl[3 + tid]; }
-      if (tid == 0) { D->loopPair = lp; D->zeta_ion = zeta; }
+      if (tid == 0) { D->loopPair = lp; D->zeta_ion = zeta; D->step = step_base + nsteps; }
     }
     if (tid == 0) {
       D->atom_steps += acc_steps;
@@ -1061,7 +1099,7 @@ struct HostReplica {
   int thermo = THERMO_NVE;  // ion thermostat (host copy of Dyn::thermo)
   double Tnh = 0, zeta_ion = 0;
   // diagnostics carried over a re-upload (mdn_set_state of any replica)
-  long long atom_steps = 0;
+  long long atom_steps = 0, step = 0;
   int status = 0, shake_iters_max = 0;
   unsigned long long shake_fail = 0;
 };
@@ -1086,6 +1124,8 @@ struct mdn_ctx {
   double last_ms = 0;
   int last_heavy = 0;
   int any_shake = 0;
+  int log_cap = 0;          // vapor events per replica between drains (0: off)
+  double *d_obs = nullptr;  // [8 * n_rep] observables
 };
 
 namespace mdn {
@@ -1178,7 +1218,7 @@ static int upload_all(mdn_ctx *c) {
   struct Offs {
     size_t sp[3][12];
     size_t pc, bt, ct, dc, dm, lp, am;
-    size_t mol_x, mol_v, region, ax, av, af, aw, axo, ing, inv, gi, gv, gg, dyn;
+    size_t mol_x, mol_v, region, ax, av, af, aw, axo, ing, inv, gi, gv, gg, dyn, vlog;
   };
   std::vector<Offs> offs(c->reps.size());
   c->any_shake = 0;
@@ -1248,9 +1288,11 @@ static int upload_all(mdn_ctx *c) {
     o.gi = lay.vec(lgi, false);
     o.gv = lay.vec(lgv, false);
     o.gg = lay.vec(lgg, false);
+    o.vlog = c->log_cap > 0 ? lay.zeros(sizeof(VapEvent) * (size_t)c->log_cap) : 0;
     Dyn dyn;
     memset(&dyn, 0, sizeof(dyn));
     dyn.atom_steps = h.atom_steps;
+    dyn.step = h.step;
     dyn.status = h.status;
     dyn.shake_iters_max = h.shake_iters_max;
     dyn.shake_fail = h.shake_fail;
@@ -1274,6 +1316,7 @@ static int upload_all(mdn_ctx *c) {
   }
   size_t reps_off = lay.zeros(sizeof(DevReplica) * c->reps.size());
   size_t order_off = lay.zeros(sizeof(int) * c->reps.size());
+  size_t obs_off = lay.zeros(sizeof(double) * 8 * c->reps.size());
   if (c->dbuf) { hipFree(c->dbuf); c->dbuf = nullptr; }
   HIPCHK(hipSetDevice(c->device));
   HIPCHK(hipMalloc(&c->dbuf, lay.host.size()));
@@ -1324,12 +1367,15 @@ static int upload_all(mdn_ctx *c) {
     R.gi = at<int>(B, o.gi); R.gv = at<int>(B, o.gv); R.gg = at<int>(B, o.gg);
     R.cap_gi = h.cap_gi; R.cap_gv = h.cap_gv; R.cap_gg = h.cap_gg;
     R.dyn = at<Dyn>(B, o.dyn);
+    R.vlog = c->log_cap > 0 ? at<VapEvent>(B, o.vlog) : nullptr;
+    R.log_cap = c->log_cap;
     h.dev = R;
   }
   memcpy(lay.host.data() + reps_off, drs.data(), sizeof(DevReplica) * drs.size());
   c->d_reps = at<DevReplica>(B, reps_off);
   c->d_dyn = at<Dyn>(B, dyn_block);
   c->d_order = at<int>(B, order_off);
+  c->d_obs = at<double>(B, obs_off);
   HIPCHK(hipMemcpy(c->dbuf, lay.host.data(), lay.host.size(), hipMemcpyHostToDevice));
   return MDN_OK;
 }
@@ -1407,6 +1453,7 @@ static int pull_replica(mdn_ctx *c, int32_t r, bool state) {
   Dyn dyn;
   HIPCHK(hipMemcpy(&dyn, R.dyn, sizeof(Dyn), hipMemcpyDeviceToHost));
   h.atom_steps = dyn.atom_steps;
+  h.step = dyn.step;
   h.status = dyn.status;
   h.shake_iters_max = dyn.shake_iters_max;
   h.shake_fail = dyn.shake_fail;
@@ -1583,6 +1630,108 @@ int mdn_set_thermostat(mdn_ctx *c, int32_t r, int32_t mode, double Tnh) {
       HIPCHK(hipMemcpy(h.dev.dyn, &dyn, sizeof(Dyn), hipMemcpyHostToDevice));
     }
   }
+  return MDN_OK;
+}
+
+}  // extern "C"
+
+namespace mdn {
+int engine_replicas(const mdn_ctx *c) { return c && c->committed ? (int)c->reps.size() : -1; }
+const SysDef &engine_sysdef(const mdn_ctx *c, int r) { return c->reps[r].def; }
+int engine_log_cap(const mdn_ctx *c) { return c->log_cap; }
+}  // namespace mdn
+
+extern "C" {
+
+static_assert(sizeof(VapEvent) == sizeof(mdn_vapor_event) && offsetof(VapEvent, dv) == offsetof(mdn_vapor_event, dv),
+              "VapEvent mirrors mdn_vapor_event");
+
+// the replica range of an entry point that takes one replica or all (-1)
+static int replica_range(mdn_ctx *c, int32_t r, int *r0, int *r1) {
+  if (!c) return set_error(MDN_E_ARG, "null context");
+  if (!c->committed) return set_error(MDN_E_STATE, "call mdn_commit first");
+  if (r == -1) { *r0 = 0; *r1 = (int)c->reps.size(); return MDN_OK; }
+  if (int rc = check_replica(c, r)) return rc;
+  *r0 = r; *r1 = r + 1;
+  return MDN_OK;
+}
+
+int mdn_set_vapor_log(mdn_ctx *c, int32_t cap) {
+  if (!c) return set_error(MDN_E_ARG, "mdn_set_vapor_log: null context");
+  if (cap < 0) return set_error(MDN_E_ARG, "mdn_set_vapor_log: negative capacity");
+  if (cap == c->log_cap) return MDN_OK;
+  int rc;
+  if (c->committed) {
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int32_t q = 0; q < (int32_t)c->reps.size(); q++)
+      if ((rc = pull_replica(c, q, true))) return rc;
+  }
+  c->log_cap = cap;
+  return c->committed ? mdn_commit(c) : MDN_OK;
+}
+
+int mdn_drain_vapor_log(mdn_ctx *c, int32_t r, mdn_vapor_event *out, int32_t cap, int32_t *n) {
+  int rc = check_replica(c, r);
+  if (rc) return rc;
+  if (!n || (cap > 0 && !out)) return set_error(MDN_E_ARG, "mdn_drain_vapor_log: null argument");
+  *n = 0;
+  if (c->log_cap == 0) return MDN_OK;
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  const DevReplica &R = c->reps[r].dev;
+  char *nlog = reinterpret_cast<char *>(R.dyn) + offsetof(Dyn, n_log);
+  int recorded = 0;
+  HIPCHK(hipMemcpy(&recorded, nlog, sizeof(int), hipMemcpyDeviceToHost));
+  const int kept = std::min(recorded, c->log_cap);
+  if (kept > cap) return set_error(MDN_E_CAPACITY, "mdn_drain_vapor_log: output buffer holds fewer events than recorded");
+  if (kept) HIPCHK(hipMemcpy(out, R.vlog, sizeof(VapEvent) * (size_t)kept, hipMemcpyDeviceToHost));
+  const int zero = 0;
+  HIPCHK(hipMemcpy(nlog, &zero, sizeof(int), hipMemcpyHostToDevice));
+  *n = kept;
+  if (recorded > kept)
+    return set_error(MDN_E_CAPACITY, "mdn_drain_vapor_log: " + std::to_string(recorded - kept) +
+                                         " events dropped (log capacity " + std::to_string(c->log_cap) + ")");
+  return MDN_OK;
+}
+
+int mdn_observe(mdn_ctx *c, int32_t r, double *out) {
+  int r0, r1;
+  int rc = replica_range(c, r, &r0, &r1);
+  if (rc) return rc;
+  if (!out) return set_error(MDN_E_ARG, "mdn_observe: null output");
+  HIPCHK(hipSetDevice(c->device));
+  hipLaunchKernelGGL(k_observe, dim3(r1 - r0), dim3(256), 0, c->stream, c->d_reps, r0, c->d_obs);
+  HIPCHK(hipGetLastError());
+  std::vector<double> o(8 * (size_t)(r1 - r0));
+  HIPCHK(hipMemcpyAsync(o.data(), c->d_obs, sizeof(double) * o.size(), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (int q = 0; q < r1 - r0; q++)
+    for (int k = 0; k < 6; k++) out[6 * (size_t)q + k] = o[8 * (size_t)q + k];
+  return MDN_OK;
+}
+
+int mdn_set_step(mdn_ctx *c, int32_t r, int64_t step) {
+  int r0, r1;
+  int rc = replica_range(c, r, &r0, &r1);
+  if (rc) return rc;
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  const long long v = step;
+  for (int q = r0; q < r1; q++)
+    HIPCHK(hipMemcpy(reinterpret_cast<char *>(c->reps[q].dev.dyn) + offsetof(Dyn, step), &v, sizeof(v),
+                     hipMemcpyHostToDevice));
+  return MDN_OK;
+}
+
+// run.cpp:42-53 as written: the loop zeroes position[0] on its first pass and
+// then subtracts zero from every other position
+int mdn_recentre(mdn_ctx *c, int32_t r) {
+  int r0, r1;
+  int rc = replica_range(c, r, &r0, &r1);
+  if (rc) return rc;
+  HIPCHK(hipSetDevice(c->device));
+  for (int q = r0; q < r1; q++) HIPCHK(hipMemsetAsync(c->reps[q].dev.mol_x, 0, 3 * sizeof(double), c->stream));
   return MDN_OK;
 }
 

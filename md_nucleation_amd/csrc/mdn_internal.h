@@ -58,4 +58,9 @@ void init_state(const SysDef &d, uint64_t seed, mdn_state *st);
 
 int set_error(int code, const std::string &msg);
 
+// engine facts the host run driver reads (committed context)
+int engine_replicas(const mdn_ctx *c);  // -1 when not committed
+const SysDef &engine_sysdef(const mdn_ctx *c, int r);
+int engine_log_cap(const mdn_ctx *c);
+
 }  // namespace mdn

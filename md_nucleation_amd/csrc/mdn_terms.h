@@ -808,6 +808,7 @@ __device__ inline void rebuild_wg(const DevReplica &R, int *s_int, double *s_dbl
   double *mol = R.mol_x, *molv = R.mol_v;
   const double HL = R.d_size * 0.5;
   const double cx = mol[0], cy = mol[1], cz = mol[2];
+  const double cvx = molv[0], cvy = molv[1], cvz = molv[2];
   // advance (out) / COM (in) / classify, one molecule per thread
   for (int i = 1 + tid; i < R.n_mol; i += nt) {
     int sp = i <= R.n_gas ? 1 : 2;
@@ -836,6 +837,19 @@ __device__ inline void rebuild_wg(const DevReplica &R, int *s_int, double *s_dbl
     // updateGasIn.cpp:21-47 / updateVaporIn.cpp:23-51
     double dx = m[0] - cx, dy = m[1] - cy, dz = m[2] - cz;
     double r2 = (dx * dx + dy * dy + dz * dz);
+    if (sp == 2 && was_in && R.vlog) {
+      // MD::Ovin / MD::Ovout (updateVaporIn.cpp:33, 39), before the wrap and
+      // the velocity draw
+      int slot = atomicAdd(&D->n_log, 1);
+      if (slot < R.log_cap) {
+        VapEvent &e = R.vlog[slot];
+        e.mol = i;
+        e.kind = r2 < kRI2 ? 0 : 1;
+        e.step = D->step;
+        e.dx[0] = dx; e.dx[1] = dy; e.dx[2] = dz;
+        e.dv[0] = mv[0] - cvx; e.dv[1] = mv[1] - cvy; e.dv[2] = mv[2] - cvz;
+      }
+    }
     if (r2 < kRI2) {
       if (!was_in) enter_molecule(R, i, sp, rb);
       R.region[i] = 1;

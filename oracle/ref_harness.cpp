@@ -29,6 +29,10 @@
 //   time N              N verlet steps as the reference runs them (with its own
 //                       random rebuilds); prints wall time and in-domain
 //                       atom-steps
+//   run                 MD::make_pair (the constructor's, md.cpp:34) then
+//                       MD::run (run.cpp): relaxation, re-centre, setNVE and
+//                       the main loop; its output files land in the working
+//                       directory, the final state on stdout
 #include <dirent.h>
 #include <math.h>
 #include <omp.h>
@@ -316,6 +320,12 @@ int main(int argc, char **argv) {
     dump_state(out, md);
   } else if (o == "make_pair") {
     md->make_pair();
+    dump_state(out, md);
+  } else if (o == "run") {
+    md->itime = 0;
+    md->make_pair();
+    md->margin_length = md->MARGIN;
+    md->run(argv);
     dump_state(out, md);
   } else if (o == "shake") {
     int mol = (int)arg;

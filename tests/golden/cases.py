@@ -22,6 +22,17 @@ def write_case(name, d):
         nvt = "scale" if name == "nvt_scale" else "250"
         return configs.Condition(pep, m["water"], 20, "N2", 200, pressure=1e6, ion_potential="AMBER",
                                  nvt_ion=nvt).write(os.path.join(d, name + ".cond"))
+    if name == "run":
+        # MD::run on the pep system in a 94 A box (every molecule inside the
+        # 110 A domain, so no rebuild draws random numbers): 100 relaxation
+        # steps under velocity scaling, 300 NVE steps, Output every 50 steps,
+        # Margin 0.5 A (several rebuilds, each logging the vapor).  No SHAKE H
+        # line: MD::verlet runs no SHAKE, and the line would leave the X-H
+        # bonds without any force
+        pep = configs.write_peptide(os.path.join(d, "pep40.txt"), seed=3, n=40)
+        return configs.Condition(pep, m["water"], 20, "N2", 200, pressure=1e6, ion_potential="AMBER",
+                                 nvt_ion="scale", relax_steps=100, total_steps=300, observe=50,
+                                 margin=0.5, shake_h=False).write(os.path.join(d, "run.cond"))
     if name == "bmh":
         nacl = configs.write_nacl_cluster(os.path.join(d, "nacl.txt"))
         return configs.Condition(nacl, m["water"], 1, "He", 1, ion_potential="Born-Mayer-Huggins-NaCl",

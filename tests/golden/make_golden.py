@@ -16,9 +16,11 @@ selected term, a no-rebuild trajectory with energies, one MD::make_pair):
   si      64-atom Si cluster: Tersoff and Stillinger-Weber (own lists)
   nvt     the pep system with "NVTion scale" and "NVTion 250": parsed mode,
           initial state, a no-rebuild trajectory with the ion thermostat
+  run     MD::make_pair + MD::run on the pep system in a 94 A box: the text
+          of every output file (K, U, ion, dump, vapor_in, vapor_out)
 plus SHAKE on perturbed methanol / water molecules and ROTATION samples.
 
-    python tests/golden/make_golden.py nvt    # rewrites only nvt.npz
+    python tests/golden/make_golden.py nvt    # rewrites only nvt.npz (or run.npz)
 """
 import os
 import subprocess
@@ -107,15 +109,39 @@ def nvt_cases(d, nsteps=30, every=10):
     print("nvt", {k: v.size for k, v in res.items() if "zeta" in k or k.endswith("nvt")})
 
 
+RUN_FILES = ("K_0.dat", "U_0.dat", "ion_300_0.dat", "out_0.dump", "vapor_in_0.dat", "vapor_out_0.dat")
+
+
+def run_case(d):
+    """The reference's MD::make_pair + MD::run from a stored state: the text
+    of every output file it writes (output.cpp), as bytes."""
+    d = os.path.join(d, "run")
+    os.makedirs(d)
+    cond = write_case("run", d)
+    init = harness(cond, None, "init", cwd=d)
+    st = {k: init[k] for k in STATE_KEYS}
+    sp = os.path.join(d, "state.txt")
+    O.write_arrays(sp, dict(st, loopPair=[0], loop_update=[0], margin_length=[0.5], zeta_ion=[0.0]))
+    fin = harness(cond, sp, "run", cwd=d)
+    res = pack("init_", st)
+    res.update(pack("final_", fin, ("atom_x", "atom_v", "mol_x", "region", "loop_update")))
+    for f in RUN_FILES:
+        with open(os.path.join(d, f), "rb") as fh:
+            res["file_" + f] = np.frombuffer(fh.read(), np.uint8)
+    np.savez_compressed(os.path.join(OUT, "run.npz"), **res)
+    print("run", {k: v.size for k, v in res.items() if k.startswith("file_")})
+
+
 def main():
     if not os.path.exists(O.REF_HARNESS):
         subprocess.check_call(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "ref"])
-    if sys.argv[1:] == ["nvt"]:
+    if sys.argv[1:] in (["nvt"], ["run"]):
         with tempfile.TemporaryDirectory() as d:
-            nvt_cases(d)
+            (nvt_cases if sys.argv[1] == "nvt" else run_case)(d)
         return
     with tempfile.TemporaryDirectory() as d:
         nvt_cases(d)
+        run_case(d)
         case("c1", write_case("c1", d), d, shake_mols=(601, 605))
         case("pep", write_case("pep", d), d, shake_mols=(201, 210))
         # the reference needs a gas line; its BMH table is indexed by ion-local
