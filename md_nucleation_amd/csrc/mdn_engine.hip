@@ -546,17 +546,22 @@ __device__ inline void dense_pair(const BatchLds<Rl> &B, const DenseI<Rl> &c, in
   }
 }
 
-// the off-diagonal run of one work item: partners k and k + 32 together
+// the off-diagonal run of one work item: partners k and k + co together
 // (two independent chains per lane), each with its travelling F_j
-// accumulator; returns through the references
+// accumulator; returns through the references.  Partner slot of lane l at
+// step k: (l + k) & jm.  A full partner block (jm = 63, co = 32) takes 64
+// steps; a partner block whose atoms all sit in its first 32 slots takes
+// 32 (jm = 31, co = 16): lanes l and l + 32 then meet the same partners, and
+// the wave rotate still hands each accumulator to the lane that meets its
+// partner next, because lane 31 receives lane 32's and lane 63 lane 0's.
 template <typename Rl, bool EF, int CLS>
 __device__ inline void dense_run_off(const BatchLds<Rl> &B, const DenseI<Rl> &c, int lane, int jb, int kb, int hL,
-                                     double &fx, double &fy, double &fz, double *a0, double *a1, double &e_ion,
-                                     double &e_vi, double &e_vv) {
+                                     int jm, int co, double &fx, double &fy, double &fz, double *a0, double *a1,
+                                     double &e_ion, double &e_vi, double &e_vv) {
   for (int k = kb; k < kb + hL; k++) {
     double gx, gy, gz, hx, hy, hz;
-    dense_pair<Rl, EF, CLS>(B, c, jb + ((lane + k) & 63), 1u, gx, gy, gz, e_ion, e_vi, e_vv);
-    dense_pair<Rl, EF, CLS>(B, c, jb + ((lane + k + 32) & 63), 1u, hx, hy, hz, e_ion, e_vi, e_vv);
+    dense_pair<Rl, EF, CLS>(B, c, jb + ((lane + k) & jm), 1u, gx, gy, gz, e_ion, e_vi, e_vv);
+    dense_pair<Rl, EF, CLS>(B, c, jb + ((lane + k + co) & jm), 1u, hx, hy, hz, e_ion, e_vi, e_vv);
     fx -= gx + hx; fy -= gy + hy; fz -= gz + hz;
     a0[0] = wave_rol1(a0[0] + gx);
     a0[1] = wave_rol1(a0[1] + gy);
@@ -591,9 +596,27 @@ __device__ inline int dense_n3(const DevReplica &R, const BatchLds<Rl> &B, int n
   const bool doIV = vi && ni > 0 && NBV > 0 && filt != kDenseIonOnly;
   const bool doVV = vv && NBV > 0 && filt != kDenseIonOnly;
   const bool bcast = ni <= kIonBroadcastMax;
-  const int n_diag = (doII ? NBI : 0) + (doVV ? NBV : 0);
-  const int n_off = (doII ? NBI * (NBI - 1) / 2 : 0) + (doIV && !bcast ? NBI * NBV : 0) +
-                    (doVV ? NBV * (NBV - 1) / 2 : 0);
+  // a range's last block may be partial; one with at most 32 atoms is a
+  // half-width partner block (32 steps instead of 64)
+  const bool ihalf = (ni & 63) != 0 && (ni & 63) <= 32;
+  const int nvap = nh - ni;
+  const bool vhalf = (nvap & 63) != 0 && (nvap & 63) <= 32;
+  // off-diagonal units: full (64 steps) and half (32 steps)
+  int n_full = 0, n_half = 0;
+  if (doII) {
+    n_full += (NBI - 1) * (NBI - 2) / 2 + (ihalf ? 0 : NBI - 1);
+    n_half += ihalf ? NBI - 1 : 0;
+  }
+  if (doIV && !bcast)
+    for (int bi = 0; bi < NBI; bi++)
+      for (int bj = 0; bj < NBV; bj++)
+        ((bj == NBV - 1 && vhalf) || (bi == NBI - 1 && ihalf) ? n_half : n_full)++;
+  if (doVV) {
+    n_full += (NBV - 1) * (NBV - 2) / 2 + (vhalf ? 0 : NBV - 1);
+    n_half += vhalf ? NBV - 1 : 0;
+  }
+  const int n_diag = (doII ? NBI : 0) + (doVV ? NBV : 0) + n_half;
+  const int n_off = n_full;
   const int n_bc = doIV && bcast ? NBV : 0;
   // run length: the fewest rounds of items per wave, counting about six
   // steps of set-up and flush per item
@@ -623,7 +646,7 @@ __device__ inline int dense_n3(const DevReplica &R, const BatchLds<Rl> &B, int n
   };
   // one block-pair unit's run: i block at ib (atoms < ilim real), j block at
   // jb (atoms < jlim real)
-  auto run = [&](int cls, bool diag, int ib, int ilim, int jb, int jlim, int part) {
+  auto run = [&](int cls, bool diag, bool half, int ib, int ilim, int jb, int jlim, int part) {
     const int i = ib + lane;
     DenseI<Rl> c;
     setup(i, ilim, jlim, c);
@@ -648,36 +671,45 @@ __device__ inline int dense_n3(const DevReplica &R, const BatchLds<Rl> &B, int n
       // (64 distinct atoms of the j block; zero sums for masked partners)
       flush(jb + ((lane + kb + L) & 63), a0x, a0y, a0z);
     } else {
-      // two independent partners per step, k and k + 32, each with its own
+      // two independent partners per step, k and k + co, each with its own
       // travelling F_j accumulator
       const int hL = L >> 1, kb = part * hL;
+      const int jm = half ? 31 : 63, co = half ? 16 : 32;
       double a0[3] = {0, 0, 0}, a1[3] = {0, 0, 0};
       if (cls == kPairIonIon)
-        dense_run_off<Rl, EF, kPairIonIon>(B, c, lane, jb, kb, hL, fx, fy, fz, a0, a1, e_ion, e_vi, e_vv);
+        dense_run_off<Rl, EF, kPairIonIon>(B, c, lane, jb, kb, hL, jm, co, fx, fy, fz, a0, a1, e_ion, e_vi, e_vv);
       else if (cls == kPairIonVap)
-        dense_run_off<Rl, EF, kPairIonVap>(B, c, lane, jb, kb, hL, fx, fy, fz, a0, a1, e_ion, e_vi, e_vv);
+        dense_run_off<Rl, EF, kPairIonVap>(B, c, lane, jb, kb, hL, jm, co, fx, fy, fz, a0, a1, e_ion, e_vi, e_vv);
       else
-        dense_run_off<Rl, EF, kPairVapVap>(B, c, lane, jb, kb, hL, fx, fy, fz, a0, a1, e_ion, e_vi, e_vv);
-      flush(jb + ((lane + kb + hL) & 63), a0[0], a0[1], a0[2]);
-      flush(jb + ((lane + kb + hL + 32) & 63), a1[0], a1[1], a1[2]);
+        dense_run_off<Rl, EF, kPairVapVap>(B, c, lane, jb, kb, hL, jm, co, fx, fy, fz, a0, a1, e_ion, e_vi, e_vv);
+      flush(jb + ((lane + kb + hL) & jm), a0[0], a0[1], a0[2]);
+      flush(jb + ((lane + kb + hL + co) & jm), a1[0], a1[1], a1[2]);
     }
     if (i < ilim) flush(i, fx, fy, fz);
   };
   int item = 0;
-  // ion-ion units
+  // ion-ion units (the partial last block, if any, on the partner side)
   if (doII)
     for (int bi = 0; bi < NBI; bi++)
       for (int bj = bi; bj < NBI; bj++) {
-        const int np = (bi == bj ? 32 : 64) / L;
+        const bool half = bi != bj && bj == NBI - 1 && ihalf;
+        const int np = (bi == bj || half ? 32 : 64) / L;
         for (int part = 0; part < np; part++, item++)
-          if (item % nw == wave) run(kPairIonIon, bi == bj, 64 * bi, ni, 64 * bj, ni, part);
+          if (item % nw == wave) run(kPairIonIon, bi == bj, half, 64 * bi, ni, 64 * bj, ni, part);
       }
-  // ion-vapor units
+  // ion-vapor units; a half-width block goes to the partner side (the ion
+  // block when only it is one: the pair form is symmetric in i and j)
   if (doIV && !bcast)
     for (int bi = 0; bi < NBI; bi++)
-      for (int bj = 0; bj < NBV; bj++)
-        for (int part = 0; part < 64 / L; part++, item++)
-          if (item % nw == wave) run(kPairIonVap, false, 64 * bi, ni, ni + 64 * bj, nh, part);
+      for (int bj = 0; bj < NBV; bj++) {
+        const bool vh = bj == NBV - 1 && vhalf, ih = bi == NBI - 1 && ihalf, swap = ih && !vh;
+        const int np = (vh || ih ? 32 : 64) / L;
+        for (int part = 0; part < np; part++, item++)
+          if (item % nw == wave) {
+            if (swap) run(kPairIonVap, false, true, ni + 64 * bj, nh, 64 * bi, ni, part);
+            else run(kPairIonVap, false, vh, 64 * bi, ni, ni + 64 * bj, nh, part);
+          }
+      }
   if (doIV && bcast)
     for (int bj = 0; bj < NBV; bj++, item++) {
       if (item % nw != wave) continue;
@@ -700,9 +732,10 @@ __device__ inline int dense_n3(const DevReplica &R, const BatchLds<Rl> &B, int n
   if (doVV)
     for (int bi = 0; bi < NBV; bi++)
       for (int bj = bi; bj < NBV; bj++) {
-        const int np = (bi == bj ? 32 : 64) / L;
+        const bool half = bi != bj && bj == NBV - 1 && vhalf;
+        const int np = (bi == bj || half ? 32 : 64) / L;
         for (int part = 0; part < np; part++, item++)
-          if (item % nw == wave) run(kPairVapVap, bi == bj, ni + 64 * bi, nh, ni + 64 * bj, nh, part);
+          if (item % nw == wave) run(kPairVapVap, bi == bj, half, ni + 64 * bi, nh, ni + 64 * bj, nh, part);
       }
   if (EF) { e[U_ION] += e_ion; e[U_VI] += e_vi; e[U_VV] += e_vv; }
   return item;
