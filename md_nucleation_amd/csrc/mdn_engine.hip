@@ -371,15 +371,23 @@ template <typename Rl> struct Pack4;
 template <> struct Pack4<float> { typedef float4 T; };
 template <> struct Pack4<double> { typedef double4 T; };
 
+// per-atom LDS record: FP64 position and the pair-precision blend weight in
+// one 32-byte slot (the dense pass finds both from one address)
+template <typename Rl>
+struct XRec {
+  double x[3];
+  Rl w;
+};
+
 template <typename Rl>
 struct BatchLds {
-  double *x, *v, *f, *xo;
+  XRec<Rl> *xr;
+  double *v, *f, *xo;
   double *kct;      // [kcap] kick factor 0.5 dt 4.184e-4 / m per species atom (ion | vapor | gas)
   // packed (q, sqrt(24 eps), sigma / 2, vapor molecule id or -1): pk per
   // atom of the ion + vapor range [cap_h], kpk per species atom [kcap]
   // (what gas atoms use)
   typename Pack4<Rl>::T *pk, *kpk;
-  Rl *wr;           // blend weight (pair precision)
   int16_t *kidx;    // per atom: index into kct
   uint32_t *allow;  // [acap] AMBER non-excluded ion-pair bitmask (rows of amber_words)
   double *red;      // [U_N + 6][16]
@@ -393,7 +401,7 @@ template <typename Rl>
 __device__ inline BatchLds<Rl> carve(char *base, int cap, int cap_h, int kcap, int acap, int scap) {
   BatchLds<Rl> B;
   double *d = reinterpret_cast<double *>(base);
-  B.x = d; d += 3 * cap;
+  B.xr = reinterpret_cast<XRec<Rl> *>(d); d += 4 * cap;
   B.v = d; d += 3 * cap;
   B.f = d; d += 3 * cap;
   B.xo = d; d += 3 * cap_h;
@@ -408,9 +416,7 @@ __device__ inline BatchLds<Rl> carve(char *base, int cap, int cap_h, int kcap, i
   d += (4 * sizeof(Rl) * (size_t)cap_h) / sizeof(double);
   B.kpk = reinterpret_cast<typename Pack4<Rl>::T *>(d);
   d += (4 * sizeof(Rl) * (size_t)kcap + sizeof(double) - 1) / sizeof(double);
-  Rl *r = reinterpret_cast<Rl *>(d);
-  B.wr = r; r += cap;
-  int *ip = reinterpret_cast<int *>(r);
+  int *ip = reinterpret_cast<int *>(d);
   B.sint = ip; ip += 64;
   B.shk.n = ip; ip += 2;
   B.shk.ij = ip; ip += 4 * (size_t)scap;
@@ -421,11 +427,11 @@ __device__ inline BatchLds<Rl> carve(char *base, int cap, int cap_h, int kcap, i
 }
 
 static size_t batch_lds_bytes(int cap, int cap_h, int kcap, int acap, int scap, size_t rl) {
-  return sizeof(double) * ((size_t)9 * cap + 3 * (size_t)cap_h + 2 * (size_t)kcap + (U_N + 6) * 16 + 16 +
+  return sizeof(double) * ((size_t)10 * cap + 3 * (size_t)cap_h + 2 * (size_t)kcap + (U_N + 6) * 16 + 16 +
                            8 * (size_t)scap + 2) +
          sizeof(int) * (2 + 4 * (size_t)scap) +
          ((4 * rl * (size_t)kcap + sizeof(double) - 1) / sizeof(double)) * sizeof(double) +
-         rl * (4 * (size_t)cap_h + (size_t)cap) + sizeof(int) * (64 + (size_t)acap) + sizeof(int16_t) * (size_t)cap;
+         rl * 4 * (size_t)cap_h + sizeof(int) * (64 + (size_t)acap) + sizeof(int16_t) * (size_t)cap;
 }
 
 // LDS store with Lorentz-Berthelot mixing from per-atom parameters
@@ -434,21 +440,21 @@ static size_t batch_lds_bytes(int cap, int cap_h, int kcap, int acap, int scap, 
 template <typename Rl>
 struct SStoreT {
   LView L;
-  double *px, *pf;
+  const XRec<Rl> *xr;
+  double *pf;
   const typename Pack4<Rl>::T *pk, *kpk;
   const int16_t *kidx;
-  const Rl *wr;
   const int *ptype;  // the ion's atom types (global memory; BMH is ion-only)
   typedef int H;
   __device__ H h(int a) const { return a; }
   // per-atom parameters: the ion + vapor range has its own, gas atoms read
   // their species atom's
   __device__ typename Pack4<Rl>::T par(H a) const { return a < L.gbase ? pk[a] : kpk[kidx[a]]; }
-  __device__ void pos(H a, double p[3]) const { p[0] = px[3 * a]; p[1] = px[3 * a + 1]; p[2] = px[3 * a + 2]; }
+  __device__ void pos(H a, double p[3]) const { p[0] = xr[a].x[0]; p[1] = xr[a].x[1]; p[2] = xr[a].x[2]; }
   __device__ void fadd(H a, double fx, double fy, double fz) const {
     atomicAdd(&pf[3 * a], fx); atomicAdd(&pf[3 * a + 1], fy); atomicAdd(&pf[3 * a + 2], fz);
   }
-  __device__ double w(H a) const { return (double)wr[a]; }
+  __device__ double w(H a) const { return (double)xr[a].w; }
   __device__ double q(H a) const { return (double)par(a).x; }
   // .y = sqrt(24 eps), .z = sigma / 2 (batch_load)
   __device__ int type(H a) const { return ptype[a]; }
@@ -512,9 +518,9 @@ template <typename Rl, bool EF, int CLS>
 __device__ inline void dense_pair(const BatchLds<Rl> &B, const DenseI<Rl> &c, int j, uint32_t valid_k, double &gx,
                                   double &gy, double &gz, double &e_ion, double &e_vi, double &e_vv) {
   const typename Pack4<Rl>::T pj = B.pk[j];
-  const double *xq = B.x + __umul24((unsigned)j, 3u);
-  const double dx = c.x0 - xq[0], dy = c.x1 - xq[1], dz = c.x2 - xq[2];
-  const Rl wj = B.wr[j];
+  const XRec<Rl> &xq = B.xr[j];
+  const double dx = c.x0 - xq.x[0], dy = c.x1 - xq.x[1], dz = c.x2 - xq.x[2];
+  const Rl wj = xq.w;
   // pair selection as integer masks (no short-circuit control flow)
   uint32_t pass;
   if (CLS == kPairIonIon) pass = c.am & (c.arow[min(j >> 5, c.wmax)] >> (j & 31)) & 1u;
@@ -631,9 +637,9 @@ __device__ inline int dense_n3(const DevReplica &R, const BatchLds<Rl> &B, int n
   const uint32_t amu = am ? 1u : 0u, viu = vi ? 1u : 0u, vvu = vv ? 1u : 0u;
   // the lane's atom of a unit whose i block starts at ib
   auto setup = [&](int i, int ilim, int jlim, DenseI<Rl> &c) {
-    c.x0 = B.x[3 * i]; c.x1 = B.x[3 * i + 1]; c.x2 = B.x[3 * i + 2];
+    c.x0 = B.xr[i].x[0]; c.x1 = B.xr[i].x[1]; c.x2 = B.xr[i].x[2];
     const typename Pack4<Rl>::T pi = B.pk[i];
-    c.qi = qq * pi.x; c.sei = pi.y; c.sgi = pi.z; c.mi = pi.w; c.wi = B.wr[i];
+    c.qi = qq * pi.x; c.sei = pi.y; c.sgi = pi.z; c.mi = pi.w; c.wi = B.xr[i].w;
     c.arow = B.allow + (size_t)(i < ni ? i : 0) * W;
     c.ok = i < ilim ? 1u : 0u;
     c.am = amu; c.vi = viu; c.vv = vvu;
@@ -825,7 +831,7 @@ __device__ inline void batch_load(const DevReplica &R, const LView &L, const Bat
     else { sp = 1; idx = (a - L.gbase) % L.na_g; kk = L.n_ion + L.na_v + idx; }
     size_t g = (size_t)gatom(R, L, a);
     for (int q = 0; q < 3; q++) {
-      B.x[3 * a + q] = R.atom_x[3 * g + q];
+      B.xr[a].x[q] = R.atom_x[3 * g + q];
       B.v[3 * a + q] = R.atom_v[3 * g + q];
       B.f[3 * a + q] = R.atom_f[3 * g + q];
     }
@@ -838,7 +844,7 @@ __device__ inline void batch_load(const DevReplica &R, const LView &L, const Bat
       B.pk[a].z = (Rl)(0.5 * S.sg[idx]);
       B.pk[a].w = (Rl)m;
     }
-    B.wr[a] = (Rl)R.atom_w[g];
+    B.xr[a].w = (Rl)R.atom_w[g];
   }
   // finite values for one block past the ion + vapor range (read by
   // dense_n3): parameters past the range, atom data past n_in
@@ -847,8 +853,8 @@ __device__ inline void batch_load(const DevReplica &R, const LView &L, const Bat
     B.pk[a].x = (Rl)0; B.pk[a].y = (Rl)0; B.pk[a].z = (Rl)0; B.pk[a].w = (Rl)-1;
   }
   for (int a = L.n_in + threadIdx.x; a < hend; a += blockDim.x) {
-    for (int q = 0; q < 3; q++) { B.x[3 * a + q] = 0.0; B.f[3 * a + q] = 0.0; }
-    B.wr[a] = (Rl)0;
+    for (int q = 0; q < 3; q++) { B.xr[a].x[q] = 0.0; B.f[3 * a + q] = 0.0; }
+    B.xr[a].w = (Rl)0;
   }
 }
 
@@ -857,11 +863,11 @@ __device__ inline void batch_store(const DevReplica &R, const LView &L, const Ba
   for (int a = threadIdx.x; a < L.n_in; a += blockDim.x) {
     size_t g = (size_t)gatom(R, L, a);
     for (int q = 0; q < 3; q++) {
-      R.atom_x[3 * g + q] = B.x[3 * a + q];
+      R.atom_x[3 * g + q] = B.xr[a].x[q];
       R.atom_v[3 * g + q] = B.v[3 * a + q];
       R.atom_f[3 * g + q] = B.f[3 * a + q];
     }
-    R.atom_w[g] = (double)B.wr[a];
+    R.atom_w[g] = (double)B.xr[a].w;
   }
 }
 
@@ -931,21 +937,22 @@ __global__ void __launch_bounds__(512) k_batched(const DevReplica *__restrict__ 
         for (int q = 0; q < 3; q++) {
           double v = B.v[3 * a + q] + B.f[3 * a + q] * kc;
           B.v[3 * a + q] = v;
-          if (keep) B.xo[3 * a + q] = B.x[3 * a + q];
-          B.x[3 * a + q] += v * R.dt;
+          if (keep) B.xo[3 * a + q] = B.xr[a].x[q];
+          B.xr[a].x[q] += v * R.dt;
           B.f[3 * a + q] = 0.0;
         }
       }
       __syncthreads();
       SStoreT<Rl> ss;
-      ss.L = L; ss.px = B.x; ss.pf = B.f; ss.pk = B.pk; ss.kpk = B.kpk; ss.kidx = B.kidx; ss.wr = B.wr;
+      ss.L = L; ss.xr = B.xr; ss.pf = B.f; ss.pk = B.pk; ss.kpk = B.kpk; ss.kidx = B.kidx;
       ss.ptype = R.sp[0].type;
       double e[U_N] = {0, 0, 0, 0, 0, 0, 0, 0};
       // SHAKE of one molecule whose first local atom is base
       auto shake_one = [&](const DevSpecies &S, int base) {
-        double *x = B.x + 3 * base, *v = B.v + 3 * base;
+        XRec<Rl> *x = B.xr + base;
+        double *v = B.v + 3 * base;
         const double *xo = B.xo + 3 * base;
-        auto X = [&](int i, int q) -> double & { return x[3 * i + q]; };
+        auto X = [&](int i, int q) -> double & { return x[i].x[q]; };
         auto XO = [&](int i, int q) -> double { return xo[3 * i + q]; };
         auto V = [&](int i, int q) -> double & { return v[3 * i + q]; };
         const int sidx = &S == &R.sp[0] ? 0 : 1;
@@ -980,14 +987,14 @@ __global__ void __launch_bounds__(512) k_batched(const DevReplica *__restrict__ 
           double c[6] = {0, 0, 0, 0, 0, 0};
           for (int i = tid & 63; i < I.natoms; i += 64) {
             double m = B.kms[i];
-            for (int q = 0; q < 3; q++) { c[q] += B.x[3 * i + q] * m; c[3 + q] += B.v[3 * i + q] * m; }
+            for (int q = 0; q < 3; q++) { c[q] += B.xr[i].x[q] * m; c[3 + q] += B.v[3 * i + q] * m; }
           }
           for (int k = 0; k < 6; k++) c[k] = wave_sum(c[k]) * (1 / I.molmass);
           if (tid < 6) B.sdbl[tid] = c[tid];
           // blend weights of the gas atoms (verlet.cpp:109-134)
           for (int a = L.gbase + tid; a < L.n_in; a += 64 * nA) {
-            double dx = B.x[3 * a] - c[0], dy = B.x[3 * a + 1] - c[1], dz = B.x[3 * a + 2] - c[2];
-            B.wr[a] = blend_weight_r<Rl>(dx * dx + dy * dy + dz * dz);
+            double dx = B.xr[a].x[0] - c[0], dy = B.xr[a].x[1] - c[1], dz = B.xr[a].x[2] - c[2];
+            B.xr[a].w = blend_weight_r<Rl>(dx * dx + dy * dy + dz * dz);
           }
           batch_terms_a<Rl>(R, ss, B, tid, 0, 64 * nA, eflag, e);
         }
@@ -995,8 +1002,8 @@ __global__ void __launch_bounds__(512) k_batched(const DevReplica *__restrict__ 
         {
           const double cx = B.sdbl[0], cy = B.sdbl[1], cz = B.sdbl[2];
           for (int a = L.n_ion + tid; a < L.gbase; a += nt) {
-            double dx = B.x[3 * a] - cx, dy = B.x[3 * a + 1] - cy, dz = B.x[3 * a + 2] - cz;
-            B.wr[a] = blend_weight_r<Rl>(dx * dx + dy * dy + dz * dz);
+            double dx = B.xr[a].x[0] - cx, dy = B.xr[a].x[1] - cy, dz = B.xr[a].x[2] - cz;
+            B.xr[a].w = blend_weight_r<Rl>(dx * dx + dy * dy + dz * dz);
           }
         }
         lp++;  // check_pairlist: no rebuild due (pairlist.cpp:58-63)
@@ -1018,7 +1025,7 @@ __global__ void __launch_bounds__(512) k_batched(const DevReplica *__restrict__ 
           double c[7] = {0, 0, 0, 0, 0, 0, 0};
           for (int i = tid; i < I.natoms; i += nt) {
             double m = B.kms[i];
-            for (int q = 0; q < 3; q++) { c[q] += B.x[3 * i + q] * m; c[3 + q] += B.v[3 * i + q] * m; }
+            for (int q = 0; q < 3; q++) { c[q] += B.xr[i].x[q] * m; c[3 + q] += B.v[3 * i + q] * m; }
             c[6] += B.v[3 * i] * B.v[3 * i] * m + B.v[3 * i + 1] * B.v[3 * i + 1] * m +
                     B.v[3 * i + 2] * B.v[3 * i + 2] * m;
           }
@@ -1038,8 +1045,8 @@ __global__ void __launch_bounds__(512) k_batched(const DevReplica *__restrict__ 
         const double cx = B.sdbl[0], cy = B.sdbl[1], cz = B.sdbl[2];
         // blend weights (verlet.cpp:109-134)
         for (int a = L.n_ion + tid; a < L.n_in; a += nt) {
-          double dx = B.x[3 * a] - cx, dy = B.x[3 * a + 1] - cy, dz = B.x[3 * a + 2] - cz;
-          B.wr[a] = blend_weight_r<Rl>(dx * dx + dy * dy + dz * dz);
+          double dx = B.xr[a].x[0] - cx, dy = B.xr[a].x[1] - cy, dz = B.xr[a].x[2] - cz;
+          B.xr[a].w = blend_weight_r<Rl>(dx * dx + dy * dy + dz * dz);
         }
         // check_pairlist (pairlist.cpp:58-63)
         lp++;
