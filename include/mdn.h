@@ -207,6 +207,13 @@ int mdn_recentre(mdn_ctx *c, int32_t replica);
 enum { MDN_RUN_VERBOSE = 1 };
 int mdn_run(mdn_ctx *c, const char *outdir, int32_t calc0, int32_t precision, uint32_t opts);
 
+/* atom positions (reference order, [n_atoms][3]) into the replica's device
+ * state, nothing else re-uploaded; and the replica's atom forces back.  No
+ * reference counterpart: for callers that move atoms themselves between
+ * force evaluations (the trajectory-segment adjoint) */
+int mdn_set_positions(mdn_ctx *c, int32_t replica, const double *atom_x);
+int mdn_get_forces(mdn_ctx *c, int32_t replica, double *atom_f);
+
 /* drop-in path: one replica, FP64, one launch per reference call */
 int mdn_make_pair(mdn_ctx *c, int32_t replica);
 int mdn_zero_forces(mdn_ctx *c, int32_t replica);

@@ -81,6 +81,8 @@ _SIGS = {
     "mdn_set_step": (C.c_int, [C.c_void_p, C.c_int32, C.c_int64]),
     "mdn_recentre": (C.c_int, [C.c_void_p, C.c_int32]),
     "mdn_run": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int32, C.c_int32, C.c_uint32]),
+    "mdn_set_positions": (C.c_int, [C.c_void_p, C.c_int32, _dp]),
+    "mdn_get_forces": (C.c_int, [C.c_void_p, C.c_int32, _dp]),
     "mdn_make_pair": (C.c_int, [C.c_void_p, C.c_int32]),
     "mdn_zero_forces": (C.c_int, [C.c_void_p, C.c_int32]),
     "mdn_compute": (C.c_int, [C.c_void_p, C.c_int32, C.c_uint32, C.c_int32, C.c_int32]),
@@ -322,6 +324,17 @@ class Engine:
         st = State(sd.n_mol, sd.n_atoms, cap=cap if cap is not None else 4 * sd.n_mol + 1024)
         check(lib().mdn_get_state(self.h, int(r), C.byref(st.c)), "mdn_get_state")
         return st
+
+    def set_positions(self, r, atom_x):
+        x = np.ascontiguousarray(atom_x, np.float64).reshape(-1)
+        if x.size != 3 * self.sys[r].n_atoms:
+            raise ValueError("atom_x must hold 3 * n_atoms values")
+        check(lib().mdn_set_positions(self.h, int(r), x.ctypes.data_as(_dp)), "mdn_set_positions")
+
+    def get_forces(self, r):
+        f = np.zeros(3 * self.sys[r].n_atoms)
+        check(lib().mdn_get_forces(self.h, int(r), f.ctypes.data_as(_dp)), "mdn_get_forces")
+        return f.reshape(-1, 3)
 
     def make_pair(self, r):
         check(lib().mdn_make_pair(self.h, int(r)), "mdn_make_pair")

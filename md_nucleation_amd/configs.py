@@ -296,6 +296,19 @@ def config5(d, n_gas=100000, n_vap=200, seed=1, radius=40.0, density=1.0):
                      cluster=(radius, density)).write(os.path.join(d, "c5_%d.cond" % seed))
 
 
+def segment_condition(d, n_gas=200, seed=1, radius=40.0, density=1.0):
+    """The force-matching system for trajectory-segment gradients: the
+    config-5 chain and 200 packed water, integrated as MD::verlet ships it
+    (no SHAKE, so no 'SHAKE H' line: the water's O-H bonds keep their
+    harmonic terms); the gas does not interact with ion or vapor."""
+    m = molecule_files(d)
+    pep = write_peptide(os.path.join(d, "peptide_%d.txt" % seed), seed)
+    return Condition(pep, m["water"], 200, "N2", n_gas, total_steps=100, ion_potential="AMBER",
+                     cluster=(radius, density), shake_h=False,
+                     interactions={"gg": "OFF", "gi": "OFF", "gv": "OFF", "vi": "LJcoul", "vv": "LJcoul"}).write(
+        os.path.join(d, "seg_%d.cond" % seed))
+
+
 def config4_replica(d, r, rng, n_gas=5000):
     """One replica of the batched workload: ion in {Na+, Cl-, chain},
     vapor U[10,400] methanol or water, 5,000 N2."""

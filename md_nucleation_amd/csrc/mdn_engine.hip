@@ -1820,6 +1820,28 @@ int mdn_get_state(mdn_ctx *c, int32_t r, mdn_state *st) {
   return status_to_code(dyn.status) ? set_error(status_to_code(dyn.status), "replica status flags set") : MDN_OK;
 }
 
+int mdn_set_positions(mdn_ctx *c, int32_t r, const double *atom_x) {
+  int rc = check_replica(c, r);
+  if (rc) return rc;
+  if (!atom_x) return set_error(MDN_E_ARG, "mdn_set_positions: null array");
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipMemcpyAsync(c->reps[r].dev.atom_x, atom_x, sizeof(double) * 3 * (size_t)c->reps[r].def.n_atoms,
+                        hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return MDN_OK;
+}
+
+int mdn_get_forces(mdn_ctx *c, int32_t r, double *atom_f) {
+  int rc = check_replica(c, r);
+  if (rc) return rc;
+  if (!atom_f) return set_error(MDN_E_ARG, "mdn_get_forces: null array");
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipMemcpyAsync(atom_f, c->reps[r].dev.atom_f, sizeof(double) * 3 * (size_t)c->reps[r].def.n_atoms,
+                        hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return MDN_OK;
+}
+
 int mdn_make_pair(mdn_ctx *c, int32_t r) {
   int rc = check_replica(c, r);
   if (rc) return rc;

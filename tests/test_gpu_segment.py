@@ -1,0 +1,153 @@
+"""Trajectory-segment gradients (config 5: 100-step segments, checkpoints
+every 10 steps) on the GPU.
+
+System: the 150-atom AMBER chain with 200 packed water, integrated as
+MD::verlet ships it (no SHAKE), after 100 warm-up steps.  Loss: mean squared
+distance of the ion and water atoms to a target trajectory (same start, ion
+parameters perturbed by +-10%) at steps 10, 20, ..., 100.
+
+  dynamics   the segment's host-integrated steps (FP64, libmdn forces) ==
+             the oracle's MD::verlet from the same state (1e-10)
+  gradient   the checkpointed reverse pass == central differences (h = 1e-6)
+             of the GPU segment loss, and == central differences of the
+             oracle's own 100-step trajectory loss with the perturbed sigma
+             and eps mixed into its pair table (the reference-derived
+             gradient)
+  spacing    checkpoints every 10 and every 25 steps give the same gradient
+  autograd   TrajectoryLoss.backward == the reverse pass
+"""
+import numpy as np
+import pytest
+import torch
+
+from md_nucleation_amd import _lib as M
+from md_nucleation_amd import adjoint as A
+from md_nucleation_amd import configs
+from md_nucleation_amd import segment as SG
+from oracle import oracle as O
+from tests import engine_util as E
+
+pytestmark = pytest.mark.gpu
+
+T, EVERY = 100, 10
+OBS = set(range(10, T + 1, 10))
+H = 1e-6
+
+
+@pytest.fixture(scope="module")
+def seg(tmp_path_factory):
+    d = str(tmp_path_factory.mktemp("seg"))
+    sd = M.SysDef(configs.segment_condition(d, n_gas=200, seed=1))
+    st = sd.init_state(7)
+    eng = M.Engine(0)
+    eng.add(sd, st, seed=7)
+    eng.commit()
+    eng.make_pair(0)
+    print("warm-up: 100 FP64 steps", flush=True)
+    eng.verlet(0, 100, 0, 0, M.FP64)
+    st1 = eng.get_state(0)
+    eng.close()
+    s = SG.Segment(sd, st1)
+    x0, v0 = s.state(st1)
+    theta = tuple(t.numpy() for t in A.ion_params(sd))
+    rng = np.random.default_rng(5)
+    pert = tuple(a * (1 + rng.uniform(-0.1, 0.1, a.size)) for a in theta)
+    xs, _, _ = s.run(pert, x0, v0, T)
+    tgt = {n: xs[n] for n in OBS}
+    yield sd, st1, s, x0, v0, theta, tgt
+    s.close()
+
+
+def _oracle_run(sd, st1, nsteps, sig=None, eps=None):
+    """the oracle's MD::verlet from st1 (no rebuild), optionally with the
+    ion's sigma / eps mixed into the ion-water pair table"""
+    v = dict(sd.view)
+    pc = np.array(v["pair_coeff"], np.float64)
+    if sig is not None:
+        nt = int(round(np.sqrt(pc.size / 2)))
+        ts, te = np.array(v["type_sigma"], np.float64), np.array(v["type_eps"], np.float64)
+        it = np.asarray(v["species"][0]["type"], np.int64)
+        ts[it], te[it] = sig, eps
+        wt = np.unique(np.asarray(v["species"][2]["type"], np.int64))
+        pc = pc.reshape(nt, nt, 2)
+        for a in it:
+            for b in wt:
+                e, s = np.sqrt(te[a] * te[b]), (ts[a] + ts[b]) / 2
+                pc[a, b] = pc[b, a] = (48 * e * s ** 12, 24 * e * s ** 6)
+    v["pair_coeff"] = pc.reshape(-1)
+    orc = O.Oracle(E.oracle_sysd(v))
+    S = orc.new_state(dict(st1.as_dict(), loopPair=0, loop_update=10 ** 8), cap=4096)
+    # the first half kick uses the forces of these parameters at x_0 (the
+    # segment is a function of x_0, v_0 and the parameters)
+    orc.forces(S, eflag=0, zero=True)
+    xs = []
+    for k in range(nsteps // EVERY):
+        assert orc.verlet(S, EVERY)[0] == 0
+        xs.append(S.atom_x.reshape(-1, 3).copy())
+    return xs
+
+
+def test_segment_dynamics_is_md_verlet(seg):
+    sd, st1, s, x0, v0, theta, tgt = seg
+    xs, _, _ = s.run(theta, x0, v0, T)
+    ox = _oracle_run(sd, st1, T)
+    for k, xo in enumerate(ox):
+        err = E.rel(xs[(k + 1) * EVERY], xo[s.idx])
+        assert err < 1e-10, (k, err)
+    moved = np.abs(xs[-1] - x0).max()
+    print("segment: max displacement %.3f A over %d steps" % (moved, T))
+    assert moved > 0.1
+
+
+def test_gradient_vs_gpu_finite_differences(seg):
+    sd, st1, s, x0, v0, theta, tgt = seg
+    L, g = SG.loss_and_grad(s, theta, x0, v0, T, tgt, OBS, every=EVERY)
+    print("loss %.6e, force evaluations so far %d" % (L, s.n_force_calls))
+    assert L > 0
+    scale = [np.abs(a).max() for a in g]
+    for p, i in ((0, 3), (0, 90), (1, 10), (1, 149), (2, 0), (2, 77)):
+        tp = [a.copy() for a in theta]
+        tm = [a.copy() for a in theta]
+        tp[p][i] += H
+        tm[p][i] -= H
+        fd = (SG.segment_loss(s, tp, x0, v0, T, tgt, OBS) - SG.segment_loss(s, tm, x0, v0, T, tgt, OBS)) / (2 * H)
+        print("param", p, "atom", i, "reverse", g[p][i], "fd", fd)
+        assert abs(g[p][i] - fd) <= 1e-5 * scale[p], (p, i, g[p][i], fd)
+
+
+def test_gradient_vs_oracle_trajectory(seg):
+    """the reference-derived gradient: central differences of the oracle's
+    own trajectory loss (its MD::verlet, its pair table)"""
+    sd, st1, s, x0, v0, theta, tgt = seg
+    _, g = SG.loss_and_grad(s, theta, x0, v0, T, tgt, OBS, every=EVERY)
+    N = s.idx.size
+    for p, i in ((0, 42), (1, 120)):
+        vals = []
+        for sgn in (1, -1):
+            t = [a.copy() for a in theta]
+            t[p][i] += sgn * H
+            ox = _oracle_run(sd, st1, T, t[0], t[1])
+            vals.append(sum(float(((xo[s.idx] - tgt[(k + 1) * EVERY]) ** 2).sum()) / N for k, xo in enumerate(ox)))
+        fd = (vals[0] - vals[1]) / (2 * H)
+        print("param", p, "atom", i, "reverse", g[p][i], "oracle fd", fd)
+        assert abs(g[p][i] - fd) <= 1e-5 * np.abs(g[p]).max()
+
+
+def test_checkpoint_spacing(seg):
+    sd, st1, s, x0, v0, theta, tgt = seg
+    L1, g1 = SG.loss_and_grad(s, theta, x0, v0, T, tgt, OBS, every=10)
+    L2, g2 = SG.loss_and_grad(s, theta, x0, v0, T, tgt, OBS, every=25)
+    assert abs(L1 - L2) <= 1e-12 * L1
+    for a, b in zip(g1, g2):
+        np.testing.assert_allclose(a, b, rtol=1e-7, atol=1e-9 * np.abs(a).max())
+
+
+def test_autograd_wrapper(seg):
+    sd, st1, s, x0, v0, theta, tgt = seg
+    sig, eps, q = A.ion_params(sd, requires_grad=True)
+    L = SG.trajectory_loss_fn(sig, eps, q, s, x0, v0, T, tgt, OBS, EVERY)
+    (3.0 * L).backward()
+    L0, g = SG.loss_and_grad(s, theta, x0, v0, T, tgt, OBS, every=EVERY)
+    assert abs(float(L) - L0) <= 1e-12 * L0
+    for t, w in zip((sig, eps, q), g):
+        np.testing.assert_allclose(t.grad.numpy(), 3.0 * w, rtol=1e-7, atol=1e-9 * np.abs(w).max())
