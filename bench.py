@@ -37,10 +37,10 @@ sys.path.insert(0, ROOT)
 METRIC = "atom-steps/s: batched mixed-prec vs plain FP64 GPU at ≤1e-5 rel force err"
 PEAK_FP32_TFLOPS = 157.3
 PEAK_FP64_TFLOPS = 78.6
-# HBM bytes per step of the heavy launch (155 replicas with a CU each), from
+# HBM bytes per step of the heavy launch (156 replicas with a CU each), from
 # the FETCH_SIZE / WRITE_SIZE passes in profiles/k_batched_mixed_512replicas.md
-# (copy-calibrated: FETCH_SIZE x 2.0, WRITE_SIZE x 1.0)
-HEAVY_HBM_BYTES_PER_STEP = 0.561e6
+# (copy-calibrated: FETCH_SIZE x 2.0, WRITE_SIZE x 1.0; 31.2 MB per 200 steps)
+HEAVY_HBM_BYTES_PER_STEP = 0.156e6
 # algorithmic FLOPs per unique pair / bonded term, as written in the reference
 FLOP = {"ljcoul_w": 31, "ljcoul": 30, "lj": 24, "bond": 22, "angle": 70, "dihedral": 150}
 
@@ -251,7 +251,45 @@ def adjoint_bench(d, n_snap=256, repeats=5):
     return {"workload": "%d FP64 frames: 150-atom AMBER chain + 200 packed water (750 atoms)" % n_snap,
             "pairs_per_pass": pairs, "forward_ms": f * 1e3, "adjoint_ms": a * 1e3,
             "forward_pairs_per_s": pairs / f, "adjoint_pairs_per_s": pairs / a,
-            "note": "wall time per C-ABI call including host-buffer copies"}
+            "note": "wall time per C-ABI call including host-buffer copies",
+            "segment": segment_bench(d)}
+
+
+def segment_bench(d, nsteps=100, every=10):
+    """Gradient of a 100-step trajectory loss (checkpoints every 10 steps)
+    with respect to the ion's 450 vi parameters: wall time of one reverse
+    pass (forward with checkpoints, recomputation, Hessian-vector products,
+    adjoint-kernel calls)."""
+    from md_nucleation_amd import _lib as M
+    from md_nucleation_amd import adjoint as A
+    from md_nucleation_amd import configs
+    from md_nucleation_amd import segment as SG
+    sd = M.SysDef(configs.segment_condition(d, n_gas=200, seed=1))
+    eng = M.Engine(0)
+    eng.add(sd, sd.init_state(7), seed=7)
+    eng.commit()
+    eng.make_pair(0)
+    eng.verlet(0, 100, 0, 0, M.FP64)
+    st = eng.get_state(0)
+    eng.close()
+    seg = SG.Segment(sd, st)
+    try:
+        x0, v0 = seg.state(st)
+        theta = tuple(t.numpy() for t in A.ion_params(sd))
+        rng = np.random.default_rng(5)
+        pert = tuple(a * (1 + rng.uniform(-0.1, 0.1, a.size)) for a in theta)
+        xs, _, _ = seg.run(pert, x0, v0, nsteps)
+        obs = set(range(every, nsteps + 1, every))
+        tgt = {n: xs[n] for n in obs}
+        seg.n_force_calls = 0
+        t0 = time.perf_counter()
+        SG.loss_and_grad(seg, theta, x0, v0, nsteps, tgt, obs, every=every)
+        dt = time.perf_counter() - t0
+        return {"workload": "%d-step segment of the chain + 200 water (750 atoms), checkpoints every %d" % (nsteps, every),
+                "gradient_s": dt, "force_evaluations": seg.n_force_calls,
+                "ms_per_force_evaluation": dt * 1e3 / max(seg.n_force_calls, 1)}
+    finally:
+        seg.close()
 
 
 def main():
@@ -362,6 +400,7 @@ def main():
         "config": {"workload": "batched replicas: %d x (Na+ | Cl- | 150-atom AMBER chain) + 10-400 methanol|water "
                                "+ 5000 N2, 300 K, dt 1 fs, SHAKE H, gi/gv LJ, vi/vv LJcoul" % n,
                    "replicas": n, "mean_in_domain_atoms": n_in},
+        "replica_steps_per_s": n * args.steps / el,
         "plain_fp64_gpu": {"value": plain, "unit": "atom-steps/s",
                            "sample": "%d replicas x %d steps, per-call kernels, one replica after another" %
                                      (pr, args.plain_steps)},

@@ -148,6 +148,6 @@ def test_autograd_wrapper(seg):
     L = SG.trajectory_loss_fn(sig, eps, q, s, x0, v0, T, tgt, OBS, EVERY)
     (3.0 * L).backward()
     L0, g = SG.loss_and_grad(s, theta, x0, v0, T, tgt, OBS, every=EVERY)
-    assert abs(float(L) - L0) <= 1e-12 * L0
+    assert abs(float(L.detach()) - L0) <= 1e-12 * L0
     for t, w in zip((sig, eps, q), g):
         np.testing.assert_allclose(t.grad.numpy(), 3.0 * w, rtol=1e-7, atol=1e-9 * np.abs(w).max())
