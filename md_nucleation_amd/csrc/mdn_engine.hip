@@ -2027,9 +2027,11 @@ int mdn_run_batched(mdn_ctx *c, int32_t nsteps, int32_t eflag_every, int64_t ste
     // Work per step grows with the square of the ion + vapor atom count.  The
     // heaviest replicas get a whole CU each (512-thread workgroups: at 256
     // VGPRs one fills a CU); the rest share CUs in 128-thread workgroups, in
-    // two size classes whose LDS lets four (<= 40 KB) or three (<= 53 KB)
-    // share a CU.  The heavy count is the largest (among replicas well above
-    // the median cost) that lets every workgroup be resident at once.
+    // two classes whose LDS lets four (<= 40 KB) or three (<= 53 KB) share a
+    // CU.  The heavy count is the largest (among replicas well above the
+    // median cost) that lets every workgroup be resident at once; the CUs
+    // left over then go to the costliest of the rest, which move from four
+    // to three per CU while the CU count allows.
     std::vector<double> cost(n);
     for (int r = 0; r < n; r++) {
       const SysDef &d = c->reps[r].def;
@@ -2058,8 +2060,16 @@ int mdn_run_batched(mdn_ctx *c, int32_t nsteps, int32_t eflag_every, int64_t ste
           else if (own[r] <= kB) clsB.push_back(r);
           else heavy.push_back(r);
         }
-        const int cus = (int)heavy.size() + ((int)clsA.size() + 3) / 4 + ((int)clsB.size() + 2) / 3;
-        if (cus <= c->n_cu) break;
+        auto cus = [&](int na, int nb) { return (int)heavy.size() + (na + 3) / 4 + (nb + 2) / 3; };
+        if (cus((int)clsA.size(), (int)clsB.size()) > c->n_cu) continue;
+        // clsA is in descending cost order
+        size_t moved = 0;
+        while (moved < clsA.size() &&
+               cus((int)(clsA.size() - moved - 1), (int)(clsB.size() + moved + 1)) <= c->n_cu)
+          moved++;
+        clsB.insert(clsB.end(), clsA.begin(), clsA.begin() + moved);
+        clsA.erase(clsA.begin(), clsA.begin() + moved);
+        break;
       }
       order.clear();
       order.insert(order.end(), heavy.begin(), heavy.end());
