@@ -46,7 +46,17 @@ struct GStore {
   __device__ void pos(const H &q, double p[3]) const {
     const double *x = R->atom_x + 3 * (size_t)q.g; p[0] = x[0]; p[1] = x[1]; p[2] = x[2];
   }
+  // When a whole wave adds to one atom (a one-atom ion in vi, a gas atom
+  // shared by a run of list entries) the wave sums first and one lane adds:
+  // 64 atomics on one address would serialise in the atomic unit.
   __device__ void fadd(const H &q, double fx, double fy, double fz) const {
+    const int g0 = __builtin_amdgcn_readfirstlane(q.g);
+    if (__ballot(1) == ~0ull && __ballot(q.g == g0) == ~0ull) {
+      fx = wave_sum(fx);
+      fy = wave_sum(fy);
+      fz = wave_sum(fz);
+      if ((threadIdx.x & 63) != 0) return;
+    }
     double *f = R->atom_f + 3 * (size_t)q.g;
     atomicAdd(f, fx); atomicAdd(f + 1, fy); atomicAdd(f + 2, fz);
   }
