@@ -328,7 +328,7 @@ __global__ void __launch_bounds__(kVVTile) k_vv_tiled(const DevReplica *__restri
     __syncthreads();
     int nj = min(kVVTile, j1 - jt);
     if (act) {
-      Rl ax = 0, ay = 0, az = 0;
+      // pair arithmetic in Rl, every pair's force accumulated in FP64
       for (int b = 0; b < nj; b++) {
         if (sm[b] == mi) continue;
         double dx = xi[0] - sx[3 * b], dy = xi[1] - sx[3 * b + 1], dz = xi[2] - sx[3 * b + 2];
@@ -336,9 +336,9 @@ __global__ void __launch_bounds__(kVVTile) k_vv_tiled(const DevReplica *__restri
         Rl rsq = rx * rx + ry * ry + rz * rz;
         int c = 2 * (ti * ntv + st[b]);
         Rl fp = ljcoul_fpair<Rl>(rsq, (Rl)sc[c], (Rl)sc[c + 1], (Rl)(qi * sq[b]), (Rl)(wi * sw[b]), eflag != 0, e);
-        ax += fp * rx; ay += fp * ry; az += fp * rz;
+        const double fpd = (double)fp;  // the FP32 scalar times the FP64 displacement
+        fx = fma(fpd, dx, fx); fy = fma(fpd, dy, fy); fz = fma(fpd, dz, fz);
       }
-      fx += (double)ax; fy += (double)ay; fz += (double)az;
     }
   }
   if (act) {
@@ -542,10 +542,11 @@ __device__ inline void dense_pair(const BatchLds<Rl> &B, const DenseI<Rl> &c, in
   const Rl u = sg * sg * r2inv, u3 = u * u * u;
   const Rl flj = e24 * u3 * ((Rl)2 * u3 - (Rl)1);
   const Rl fc = c.qi * pj.x * rinv;
-  const Rl fn = -((flj + fc) * r2inv * (w * m));
-  gx = (double)(fn * rx);
-  gy = (double)(fn * ry);
-  gz = (double)(fn * rz);
+  // the FP32 scalar force over r times the FP64 displacement
+  const double fnd = (double)(-((flj + fc) * r2inv * (w * m)));
+  gx = fnd * dx;
+  gy = fnd * dy;
+  gz = fnd * dz;
   if (EF) {
     const double ep = (double)((e24 * (Rl)(1.0 / 6.0) * u3 * (u3 - (Rl)1) + fc) * m);
     (CLS == kPairIonIon ? e_ion : CLS == kPairIonVap ? e_vi : e_vv) += ep;
