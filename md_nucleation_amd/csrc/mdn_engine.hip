@@ -332,8 +332,7 @@ __global__ void __launch_bounds__(kVVTile) k_vv_tiled(const DevReplica *__restri
       for (int b = 0; b < nj; b++) {
         if (sm[b] == mi) continue;
         double dx = xi[0] - sx[3 * b], dy = xi[1] - sx[3 * b + 1], dz = xi[2] - sx[3 * b + 2];
-        Rl rx = (Rl)dx, ry = (Rl)dy, rz = (Rl)dz;
-        Rl rsq = rx * rx + ry * ry + rz * rz;
+        Rl rsq = (Rl)(dx * dx + dy * dy + dz * dz);
         int c = 2 * (ti * ntv + st[b]);
         Rl fp = ljcoul_fpair<Rl>(rsq, (Rl)sc[c], (Rl)sc[c + 1], (Rl)(qi * sq[b]), (Rl)(wi * sw[b]), eflag != 0, e);
         const double fpd = (double)fp;  // the FP32 scalar times the FP64 displacement
@@ -528,8 +527,8 @@ __device__ inline void dense_pair(const BatchLds<Rl> &B, const DenseI<Rl> &c, in
   else pass = c.vv & (pj.w != c.mi ? 1u : 0u);
   const uint32_t on = c.ok & (j < c.jlim ? 1u : 0u) & valid_k & pass;
   const Rl m = on ? (Rl)1 : (Rl)0;
-  const Rl rx = (Rl)dx, ry = (Rl)dy, rz = (Rl)dz;
-  const Rl r2 = rx * rx + ry * ry + rz * rz;
+  // r^2 from the FP64 displacement, rounded once to pair precision
+  const Rl r2 = (Rl)(dx * dx + dy * dy + dz * dz);
   const Rl rsq = on ? r2 : (Rl)1;
   // 24 eps_ij and sigma_ij from the packed per-atom sqrt(24 eps) and
   // sigma / 2; with u3 = (sigma_ij / r)^6 the reference's r^-6 (c0 r^-6 -
