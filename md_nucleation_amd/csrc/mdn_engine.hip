@@ -486,9 +486,12 @@ struct SStoreT {
 // an off-diagonal one runs k and k + 32 together (two independent chains per
 // lane), k = 0..31.  Ion-ion pairs are selected by the AMBER exclusion
 // bitmask, ion-vapor pairs by vi, vapor-vapor pairs of distinct molecules by
-// vv.  For an ion of a few atoms the ion-vapor units are instead one per
-// vapor block: each lane holds a vapor atom and loops over the ion atoms,
-// whose forces are wave sums.
+// vv.  A work item reads its lane's 64 selection bits for the partner block
+// once (real partner atom, and for ion-ion pairs not excluded) and rotates
+// them one slot per step; F_i and the F_j accumulators take s (x_i - x_j)
+// by FP64 fused multiply-adds.  For an ion of a few atoms the ion-vapor
+// units are instead one per vapor block: each lane holds a vapor atom and
+// loops over the ion atoms, whose forces are wave sums.
 //
 // Work items are runs of L pairs per lane of one unit (L = 32, 16 or 8,
 // whichever gives the waves the fewest rounds), dealt round-robin to the
@@ -497,36 +500,51 @@ struct SStoreT {
 // loads and adds unconditionally and the loop body has no branches: masked
 // pairs carry a zero force.
 
-// the work item's atom i (lane) and pair-selection flags
+// the work item's atom i (lane)
 template <typename Rl>
 struct DenseI {
   double x0, x1, x2;
   Rl qi, sei, sgi, mi, wi;
-  const uint32_t *arow;
-  uint32_t ok, am, vi, vv;
-  int jlim, wmax;  // partners j < jlim are real
+  uint32_t ok;  // the lane's atom is real
 };
 
-// one pair (i, j): g = minus the pair force vector (F_j += g, F_i -= g),
-// zero for masked pairs; energies by reference label
 // Unit kinds: the pair selection that cannot change within a unit is
 // compiled out.
 enum { kPairIonIon = 1, kPairIonVap = 2, kPairVapVap = 3 };
 
-template <typename Rl, bool EF, int CLS>
-__device__ inline void dense_pair(const BatchLds<Rl> &B, const DenseI<Rl> &c, int j, uint32_t valid_k, double &gx,
-                                  double &gy, double &gz, double &e_ion, double &e_vi, double &e_vv) {
-  const typename Pack4<Rl>::T pj = B.pk[j];
+// one pair (i, j) with its selection bit from the unit's partner mask:
+// returns s with the pair force on i = s (x_i - x_j) and on j the opposite,
+// zero for masked pairs; the FP64 displacement x_i - x_j in (dx, dy, dz);
+// energies by reference label
+// a partner's LDS data
+template <typename Rl>
+struct DenseJ {
+  typename Pack4<Rl>::T p;
+  double x0, x1, x2;
+  Rl w;
+};
+
+template <typename Rl>
+__device__ inline DenseJ<Rl> dense_load(const BatchLds<Rl> &B, int j) {
+  DenseJ<Rl> d;
+  d.p = B.pk[j];
   const XRec<Rl> &xq = B.xr[j];
-  const double dx = c.x0 - xq.x[0], dy = c.x1 - xq.x[1], dz = c.x2 - xq.x[2];
-  const Rl wj = xq.w;
-  // pair selection as integer masks (no short-circuit control flow)
-  uint32_t pass;
-  if (CLS == kPairIonIon) pass = c.am & (c.arow[min(j >> 5, c.wmax)] >> (j & 31)) & 1u;
-  else if (CLS == kPairIonVap) pass = c.vi;
-  else pass = c.vv & (pj.w != c.mi ? 1u : 0u);
-  const uint32_t on = c.ok & (j < c.jlim ? 1u : 0u) & valid_k & pass;
-  const Rl m = on ? (Rl)1 : (Rl)0;
+  d.x0 = xq.x[0]; d.x1 = xq.x[1]; d.x2 = xq.x[2];
+  d.w = xq.w;
+  return d;
+}
+
+template <typename Rl, bool EF, int CLS>
+__device__ inline double dense_pair(const DenseJ<Rl> &q, const DenseI<Rl> &c, uint32_t pass, double &dx,
+                                    double &dy, double &dz, double &e_ion, double &e_vi, double &e_vv) {
+  const typename Pack4<Rl>::T pj = q.p;
+  dx = c.x0 - q.x0;
+  dy = c.x1 - q.x1;
+  dz = c.x2 - q.x2;
+  const Rl wj = q.w;
+  // vapor-vapor pairs of one molecule are intramolecular (integer test, no
+  // short-circuit control flow)
+  const uint32_t on = CLS == kPairVapVap ? pass & (pj.w != c.mi ? 1u : 0u) : pass;
   // r^2 from the FP64 displacement, rounded once to pair precision
   const Rl r2 = (Rl)(dx * dx + dy * dy + dz * dz);
   const Rl rsq = on ? r2 : (Rl)1;
@@ -536,20 +554,26 @@ __device__ inline void dense_pair(const BatchLds<Rl> &B, const DenseI<Rl> &c, in
   const Rl e24 = c.sei * pj.y;
   const Rl sg = c.sgi + pj.z;
   const Rl w = CLS == kPairIonIon ? (Rl)1 : c.wi * wj;
+  const Rl wm = on ? w : (Rl)0;
   Rl r2inv, rinv;
   inv_r<Rl>(rsq, r2inv, rinv);
   const Rl u = sg * sg * r2inv, u3 = u * u * u;
   const Rl flj = e24 * u3 * ((Rl)2 * u3 - (Rl)1);
   const Rl fc = c.qi * pj.x * rinv;
-  // the FP32 scalar force over r times the FP64 displacement
-  const double fnd = (double)(-((flj + fc) * r2inv * (w * m)));
-  gx = fnd * dx;
-  gy = fnd * dy;
-  gz = fnd * dz;
   if (EF) {
-    const double ep = (double)((e24 * (Rl)(1.0 / 6.0) * u3 * (u3 - (Rl)1) + fc) * m);
+    const double ep = on ? (double)(e24 * (Rl)(1.0 / 6.0) * u3 * (u3 - (Rl)1) + fc) : 0.0;
     (CLS == kPairIonIon ? e_ion : CLS == kPairIonVap ? e_vi : e_vv) += ep;
   }
+  // the pair-precision scalar force over r, applied to the FP64
+  // displacement by the caller
+  return (double)((flj + fc) * r2inv * wm);
+}
+
+// one bit per partner slot, rotated by r at the start of a work item so that
+// bit t belongs to the item's step t
+__device__ inline uint64_t rotr64(uint64_t v, int r) {
+  r &= 63;
+  return r ? (v >> r) | (v << (64 - r)) : v;
 }
 
 // the off-diagonal run of one work item: partners k and k + co together
@@ -560,21 +584,40 @@ __device__ inline void dense_pair(const BatchLds<Rl> &B, const DenseI<Rl> &c, in
 // 32 (jm = 31, co = 16): lanes l and l + 32 then meet the same partners, and
 // the wave rotate still hands each accumulator to the lane that meets its
 // partner next, because lane 31 receives lane 32's and lane 63 lane 0's.
+// Selection bits: bit t of sa for the partner of step kb + t, of sb for that
+// of step kb + t + co (a run has at most 16 steps).
 template <typename Rl, bool EF, int CLS>
 __device__ inline void dense_run_off(const BatchLds<Rl> &B, const DenseI<Rl> &c, int lane, int jb, int kb, int hL,
-                                     int jm, int co, double &fx, double &fy, double &fz, double *a0, double *a1,
-                                     double &e_ion, double &e_vi, double &e_vv) {
-  for (int k = kb; k < kb + hL; k++) {
-    double gx, gy, gz, hx, hy, hz;
-    dense_pair<Rl, EF, CLS>(B, c, jb + ((lane + k) & jm), 1u, gx, gy, gz, e_ion, e_vi, e_vv);
-    dense_pair<Rl, EF, CLS>(B, c, jb + ((lane + k + co) & jm), 1u, hx, hy, hz, e_ion, e_vi, e_vv);
-    fx -= gx + hx; fy -= gy + hy; fz -= gz + hz;
-    a0[0] = wave_rol1(a0[0] + gx);
-    a0[1] = wave_rol1(a0[1] + gy);
-    a0[2] = wave_rol1(a0[2] + gz);
-    a1[0] = wave_rol1(a1[0] + hx);
-    a1[1] = wave_rol1(a1[1] + hy);
-    a1[2] = wave_rol1(a1[2] + hz);
+                                     int jm, int co, uint32_t sa, uint32_t sb, double &fx, double &fy, double &fz,
+                                     double *a0, double *a1, double &e_ion, double &e_vi, double &e_vv) {
+  auto step = [&](const DenseJ<Rl> &qa, const DenseJ<Rl> &qb, int t) {
+    const uint32_t pa = (sa >> t) & 1u, pb = (sb >> t) & 1u;
+    double dx, dy, dz, ex, ey, ez;
+    const double s = dense_pair<Rl, EF, CLS>(qa, c, pa, dx, dy, dz, e_ion, e_vi, e_vv);
+    const double u = dense_pair<Rl, EF, CLS>(qb, c, pb, ex, ey, ez, e_ion, e_vi, e_vv);
+    fx = fma(s, dx, fx); fy = fma(s, dy, fy); fz = fma(s, dz, fz);
+    fx = fma(u, ex, fx); fy = fma(u, ey, fy); fz = fma(u, ez, fz);
+    a0[0] = wave_rol1(fma(-s, dx, a0[0]));
+    a0[1] = wave_rol1(fma(-s, dy, a0[1]));
+    a0[2] = wave_rol1(fma(-s, dz, a0[2]));
+    a1[0] = wave_rol1(fma(-u, ex, a1[0]));
+    a1[1] = wave_rol1(fma(-u, ey, a1[1]));
+    a1[2] = wave_rol1(fma(-u, ez, a1[2]));
+  };
+  // partners of step t: slots (lane + kb + t) & jm and (lane + kb + t + co) & jm;
+  // the loads of the next step are issued before this step's arithmetic
+  // (the last prefetch wraps within the block: in bounds, unused)
+  const int l0 = lane + kb;
+  DenseJ<Rl> qa = dense_load(B, jb + (l0 & jm)), qb = dense_load(B, jb + ((l0 + co) & jm));
+  for (int t = 0; t < hL; t += 2) {
+    const DenseJ<Rl> ra = dense_load(B, jb + ((l0 + t + 1) & jm)), rb = dense_load(B, jb + ((l0 + t + 1 + co) & jm));
+    // keep the loads ahead of the arithmetic that does not need them
+    __builtin_amdgcn_sched_barrier(0);
+    step(qa, qb, t);
+    qa = dense_load(B, jb + ((l0 + t + 2) & jm));
+    qb = dense_load(B, jb + ((l0 + t + 2 + co) & jm));
+    __builtin_amdgcn_sched_barrier(0);
+    step(ra, rb, t + 1);
   }
 }
 
@@ -634,16 +677,25 @@ __device__ inline int dense_n3(const DevReplica &R, const BatchLds<Rl> &B, int n
   const int W = __builtin_amdgcn_readfirstlane(R.amber_words);
   double *__restrict__ pf = B.f;
   double e_ion = 0, e_vi = 0, e_vv = 0;
-  const uint32_t amu = am ? 1u : 0u, viu = vi ? 1u : 0u, vvu = vv ? 1u : 0u;
   // the lane's atom of a unit whose i block starts at ib
-  auto setup = [&](int i, int ilim, int jlim, DenseI<Rl> &c) {
+  auto setup = [&](int i, int ilim, DenseI<Rl> &c) {
     c.x0 = B.xr[i].x[0]; c.x1 = B.xr[i].x[1]; c.x2 = B.xr[i].x[2];
     const typename Pack4<Rl>::T pi = B.pk[i];
     c.qi = qq * pi.x; c.sei = pi.y; c.sgi = pi.z; c.mi = pi.w; c.wi = B.xr[i].w;
-    c.arow = B.allow + (size_t)(i < ni ? i : 0) * W;
     c.ok = i < ilim ? 1u : 0u;
-    c.am = amu; c.vi = viu; c.vv = vvu;
-    c.jlim = jlim; c.wmax = W > 0 ? W - 1 : 0;
+  };
+  // selection bits of the lane's partners in the j block at jb, bit s for
+  // slot s: a real atom (jb + s < jlim), for ion-ion pairs not excluded
+  // (row i of the AMBER bitmask), none for a padding lane
+  auto partner_bits = [&](int cls, int i, const DenseI<Rl> &c, int jb, int jlim) -> uint64_t {
+    const int jv = min(max(jlim - jb, 0), 64);
+    uint64_t V = jv >= 64 ? ~0ull : ((1ull << jv) - 1ull);
+    if (cls == kPairIonIon) {
+      const uint32_t *row = B.allow + (size_t)(i < ni ? i : 0) * W;
+      const int w0 = jb >> 5, wm = W > 0 ? W - 1 : 0;
+      V &= (uint64_t)row[min(w0, wm)] | ((uint64_t)row[min(w0 + 1, wm)] << 32);
+    }
+    return c.ok ? V : 0ull;
   };
   auto flush = [&](int idx, double ax, double ay, double az) {
     atomicAdd(&pf[3 * idx], ax);
@@ -655,23 +707,36 @@ __device__ inline int dense_n3(const DevReplica &R, const BatchLds<Rl> &B, int n
   auto run = [&](int cls, bool diag, bool half, int ib, int ilim, int jb, int jlim, int part) {
     const int i = ib + lane;
     DenseI<Rl> c;
-    setup(i, ilim, jlim, c);
+    setup(i, ilim, c);
+    uint64_t V = partner_bits(cls, i, c, jb, jlim);
     double fx = 0, fy = 0, fz = 0;
     if (diag) {
-      // k = 1..32 in runs of L; k = 32 only on lanes 0..31
+      // k = 1..32 in runs of L; k = 32 only on lanes 0..31 (lane l >= 32
+      // meets slot l - 32 at k = 32, which lane l - 32 visits at k = 32)
       const int kb = 1 + part * L;
-      const uint32_t half_ok = lane < 32 ? 1u : 0u;
+      if (lane >= 32) V &= ~(1ull << (lane - 32));
+      const uint32_t sa = (uint32_t)rotr64(V, (lane + kb) & 63);  // bit t: step kb + t (L <= 32)
       double a0x = 0, a0y = 0, a0z = 0;
-      for (int k = kb; k < kb + L; k++) {
-        double gx, gy, gz;
-        const int j = jb + ((lane + k) & 63);
-        const uint32_t vk = k != 32 ? 1u : half_ok;
-        if (cls == kPairIonIon) dense_pair<Rl, EF, kPairIonIon>(B, c, j, vk, gx, gy, gz, e_ion, e_vi, e_vv);
-        else dense_pair<Rl, EF, kPairVapVap>(B, c, j, vk, gx, gy, gz, e_ion, e_vi, e_vv);
-        fx -= gx; fy -= gy; fz -= gz;
-        a0x = wave_rol1(a0x + gx);
-        a0y = wave_rol1(a0y + gy);
-        a0z = wave_rol1(a0z + gz);
+      auto step = [&](const DenseJ<Rl> &q, int t) {
+        const uint32_t pa = (sa >> t) & 1u;
+        double dx, dy, dz, sv;
+        if (cls == kPairIonIon) sv = dense_pair<Rl, EF, kPairIonIon>(q, c, pa, dx, dy, dz, e_ion, e_vi, e_vv);
+        else sv = dense_pair<Rl, EF, kPairVapVap>(q, c, pa, dx, dy, dz, e_ion, e_vi, e_vv);
+        fx = fma(sv, dx, fx); fy = fma(sv, dy, fy); fz = fma(sv, dz, fz);
+        a0x = wave_rol1(fma(-sv, dx, a0x));
+        a0y = wave_rol1(fma(-sv, dy, a0y));
+        a0z = wave_rol1(fma(-sv, dz, a0z));
+      };
+      // one step of prefetch, as in the off-diagonal runs (L is even)
+      const int l0 = lane + kb;
+      DenseJ<Rl> q0 = dense_load(B, jb + (l0 & 63));
+      for (int t = 0; t < L; t += 2) {
+        const DenseJ<Rl> q1 = dense_load(B, jb + ((l0 + t + 1) & 63));
+        __builtin_amdgcn_sched_barrier(0);
+        step(q0, t);
+        q0 = dense_load(B, jb + ((l0 + t + 2) & 63));
+        __builtin_amdgcn_sched_barrier(0);
+        step(q1, t + 1);
       }
       // each accumulator now belongs to the partner one step past the run
       // (64 distinct atoms of the j block; zero sums for masked partners)
@@ -681,13 +746,21 @@ __device__ inline int dense_n3(const DevReplica &R, const BatchLds<Rl> &B, int n
       // travelling F_j accumulator
       const int hL = L >> 1, kb = part * hL;
       const int jm = half ? 31 : 63, co = half ? 16 : 32;
+      // a half block's 32 slots repeat in both halves, so a rotate by r < 32
+      // rotates each half; bits co.. then hold the partners k + co
+      if (half) V = (V & 0xffffffffull) | (V << 32);
+      V = rotr64(V, (lane + kb) & jm);
+      const uint32_t sa = (uint32_t)V, sb = (uint32_t)(V >> co);
       double a0[3] = {0, 0, 0}, a1[3] = {0, 0, 0};
       if (cls == kPairIonIon)
-        dense_run_off<Rl, EF, kPairIonIon>(B, c, lane, jb, kb, hL, jm, co, fx, fy, fz, a0, a1, e_ion, e_vi, e_vv);
+        dense_run_off<Rl, EF, kPairIonIon>(B, c, lane, jb, kb, hL, jm, co, sa, sb, fx, fy, fz, a0, a1, e_ion, e_vi,
+                                           e_vv);
       else if (cls == kPairIonVap)
-        dense_run_off<Rl, EF, kPairIonVap>(B, c, lane, jb, kb, hL, jm, co, fx, fy, fz, a0, a1, e_ion, e_vi, e_vv);
+        dense_run_off<Rl, EF, kPairIonVap>(B, c, lane, jb, kb, hL, jm, co, sa, sb, fx, fy, fz, a0, a1, e_ion, e_vi,
+                                           e_vv);
       else
-        dense_run_off<Rl, EF, kPairVapVap>(B, c, lane, jb, kb, hL, jm, co, fx, fy, fz, a0, a1, e_ion, e_vi, e_vv);
+        dense_run_off<Rl, EF, kPairVapVap>(B, c, lane, jb, kb, hL, jm, co, sa, sb, fx, fy, fz, a0, a1, e_ion, e_vi,
+                                           e_vv);
       flush(jb + ((lane + kb + hL) & jm), a0[0], a0[1], a0[2]);
       flush(jb + ((lane + kb + hL + co) & jm), a1[0], a1[1], a1[2]);
     }
@@ -723,13 +796,13 @@ __device__ inline int dense_n3(const DevReplica &R, const BatchLds<Rl> &B, int n
       // F_vapor in registers, F_ion a wave sum per ion atom
       const int v = ni + 64 * bj + lane;
       DenseI<Rl> c;
-      setup(v, nh, ni, c);
+      setup(v, nh, c);
       double fx = 0, fy = 0, fz = 0;
       for (int i = 0; i < ni; i++) {
-        double gx, gy, gz;
-        dense_pair<Rl, EF, kPairIonVap>(B, c, i, 1u, gx, gy, gz, e_ion, e_vi, e_vv);
-        fx -= gx; fy -= gy; fz -= gz;
-        const double sx = wave_sum(gx), sy = wave_sum(gy), sz = wave_sum(gz);
+        double dx, dy, dz;
+        const double sv = dense_pair<Rl, EF, kPairIonVap>(dense_load(B, i), c, c.ok, dx, dy, dz, e_ion, e_vi, e_vv);
+        fx = fma(sv, dx, fx); fy = fma(sv, dy, fy); fz = fma(sv, dz, fz);
+        const double sx = wave_sum(-sv * dx), sy = wave_sum(-sv * dy), sz = wave_sum(-sv * dz);
         if (lane == 0) flush(i, sx, sy, sz);
       }
       if (v < nh) flush(v, fx, fy, fz);
