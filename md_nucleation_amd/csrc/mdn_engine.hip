@@ -1054,7 +1054,11 @@ __global__ void __launch_bounds__(512) k_batched(const DevReplica *__restrict__ 
         const int nS = min(nwv - 1, (L.nv + 63) >> 6), nA = nwv - nS;
         const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
         if (wave >= nA) {
+          // the SHAKE chain is a long run of dependent instructions: let it
+          // win instruction issue on its SIMD over the waves of force work
+          __builtin_amdgcn_s_setprio(2);
           for (int m = tid - 64 * nA; m < L.nv; m += 64 * nS) shake_one(Vs, L.vbase + m * L.na_v);
+          __builtin_amdgcn_s_setprio(0);
         } else {
           // ion COM (verlet.cpp:90-107), every phase-A wave for itself
           double c[6] = {0, 0, 0, 0, 0, 0};
@@ -2019,46 +2023,62 @@ namespace mdn {
 // LDS capacities for a set of replicas from their current in-domain counts,
 // with headroom for molecules that enter during a launch
 struct BatchCaps { int cap, cap_h, kcap, acap, scap; size_t lds; };
-static BatchCaps batch_caps(const mdn_ctx *c, const std::vector<Dyn> &dyn, const std::vector<int> &set, size_t rl,
-                            size_t budget) {
+// what a set of replicas needs of a launch's LDS arrays: the largest of each
+// replica's counts (in-domain atoms, ion + vapor atoms, their maxima, species
+// atoms, exclusion-mask words, constraints)
+struct CapNeed {
   int need = 16, need_h = 16, most = 16, most_h = 16, kcap = 8, acap = 1, scap = 1;
-  for (int r : set) {
-    const SysDef &d = c->reps[r].def;
-    scap = std::max(scap, (int)std::max(d.sp[0].rigid.size(), d.sp[2].rigid.size()) / 3);
-    int nh = d.sp[0].natoms + dyn[r].n_in_vap * d.sp[2].natoms;
-    need_h = std::max(need_h, nh);
-    need = std::max(need, nh + dyn[r].n_in_gas * d.sp[1].natoms);
-    most = std::max(most, d.n_atoms);
-    most_h = std::max(most_h, d.sp[0].natoms + d.n_vap * d.sp[2].natoms);
-    kcap = std::max(kcap, d.sp[0].natoms + d.sp[1].natoms + d.sp[2].natoms);
-    if (d.flags & MDN_ION_AMBER) acap = std::max(acap, d.sp[0].natoms * ((d.sp[0].natoms + 31) / 32));
+  void add(const CapNeed &o) {
+    need = std::max(need, o.need); need_h = std::max(need_h, o.need_h);
+    most = std::max(most, o.most); most_h = std::max(most_h, o.most_h);
+    kcap = std::max(kcap, o.kcap); acap = std::max(acap, o.acap); scap = std::max(scap, o.scap);
   }
+};
+static CapNeed cap_need(const mdn_ctx *c, const Dyn &dy, int r) {
+  const SysDef &d = c->reps[r].def;
+  CapNeed q;
+  q.scap = std::max(q.scap, (int)std::max(d.sp[0].rigid.size(), d.sp[2].rigid.size()) / 3);
+  const int nh = d.sp[0].natoms + dy.n_in_vap * d.sp[2].natoms;
+  q.need_h = std::max(q.need_h, nh);
+  q.need = std::max(q.need, nh + dy.n_in_gas * d.sp[1].natoms);
+  q.most = std::max(q.most, d.n_atoms);
+  q.most_h = std::max(q.most_h, d.sp[0].natoms + d.n_vap * d.sp[2].natoms);
+  q.kcap = std::max(q.kcap, d.sp[0].natoms + d.sp[1].natoms + d.sp[2].natoms);
+  if (d.flags & MDN_ION_AMBER) q.acap = std::max(q.acap, d.sp[0].natoms * ((d.sp[0].natoms + 31) / 32));
+  return q;
+}
+static BatchCaps caps_of(const mdn_ctx *c, const CapNeed &q, size_t rl, size_t budget) {
   // the ion + vapor capacity has a block (64 atoms) of slack past the range
   // (dense_n3 reads whole blocks of positions, weights, forces and
   // parameters), the total a multiple of 32 and at least that
   auto r32 = [](int v) { return (v + 31) / 32 * 32; };
   auto r64 = [](int v) { return (v + 63) / 64 * 64; };
-  most = r32(most);
-  most_h = r64(most_h) + 64;
+  const int need = q.need, need_h = q.need_h, most = r32(q.most), most_h = r64(q.most_h) + 64;
   int lo_h = std::min(most_h, r64(need_h + 32) + 64), hi_h = std::min(most_h, r64(need_h * 3 / 2 + 32) + 64);
   int lo = std::min(most, r32(need + need / 8 + 64)), hi = std::min(most, r32(need * 3 / 2 + 64));
   BatchCaps b;
-  kcap += kcap & 1;  // keeps the LDS arrays after it 16-byte aligned
-  b.kcap = kcap; b.acap = acap; b.scap = scap;
+  const int kcap = q.kcap + (q.kcap & 1);  // keeps the LDS arrays after it 16-byte aligned
+  b.kcap = kcap; b.acap = q.acap; b.scap = q.scap;
   if (c->batch_cap > 0) {
     b.cap = r32(c->batch_cap);
     b.cap_h = std::min(hi_h, b.cap & ~63);
   } else {
     b.cap = hi; b.cap_h = hi_h;
-    while (batch_lds_bytes(std::max(b.cap, b.cap_h), b.cap_h, kcap, acap, scap, rl) > budget &&
+    while (batch_lds_bytes(std::max(b.cap, b.cap_h), b.cap_h, kcap, q.acap, q.scap, rl) > budget &&
            (b.cap > lo || b.cap_h > lo_h)) {
       if (b.cap > lo) b.cap -= 32;
       if (b.cap_h > lo_h) b.cap_h -= 64;
     }
   }
   b.cap = std::max(b.cap, b.cap_h);
-  b.lds = batch_lds_bytes(b.cap, b.cap_h, kcap, acap, scap, rl);
+  b.lds = batch_lds_bytes(b.cap, b.cap_h, kcap, q.acap, q.scap, rl);
   return b;
+}
+static BatchCaps batch_caps(const mdn_ctx *c, const std::vector<Dyn> &dyn, const std::vector<int> &set, size_t rl,
+                            size_t budget) {
+  CapNeed q;
+  for (int r : set) q.add(cap_need(c, dyn[r], r));
+  return caps_of(c, q, rl, budget);
 }
 
 template <typename Rl>
@@ -2077,6 +2097,10 @@ static int launch_batched(mdn_ctx *c, hipStream_t st, const int *order, int n, i
   return MDN_OK;
 }
 }  // namespace mdn
+
+// XCDs of the device: CDNA3/4 parts are built from 32-CU XCDs (MI355X: 8 x
+// 32); other CU counts are taken as one
+static int xcds(int n_cu) { return n_cu >= 64 && n_cu % 32 == 0 ? n_cu / 32 : 1; }
 
 extern "C" {
 
@@ -2104,7 +2128,11 @@ int mdn_run_batched(mdn_ctx *c, int32_t nsteps, int32_t eflag_every, int64_t ste
     // CU.  The heavy count is the largest (among replicas well above the
     // median cost) that lets every workgroup be resident at once; the CUs
     // left over then go to the costliest of the rest, which move from four
-    // to three per CU while the CU count allows.
+    // to three per CU while the CU count allows.  Workgroups of a launch are
+    // dealt round-robin over the XCDs (32 CUs each), so the count that must
+    // fit is that of the fullest XCD.  Within a launch the replicas go in
+    // descending cost: a workgroup that finds no room starts when one ends,
+    // and the cheapest can best afford that.
     std::vector<double> cost(n);
     for (int r = 0; r < n; r++) {
       const SysDef &d = c->reps[r].def;
@@ -2118,11 +2146,30 @@ int mdn_run_batched(mdn_ctx *c, int32_t nsteps, int32_t eflag_every, int64_t ste
     std::vector<int> heavy, clsA, clsB;
     if (c->batch_split && c->batch_threads == 256 && n > c->n_cu) {
       // each replica's own LDS need at the smallest headroom
+      std::vector<CapNeed> rq(n);
       std::vector<size_t> own(n);
-      for (int r = 0; r < n; r++) own[r] = batch_caps(c, dyn, std::vector<int>{r}, rl, 0).lds;
+      for (int r = 0; r < n; r++) {
+        rq[r] = cap_need(c, dyn[r], r);
+        own[r] = caps_of(c, rq[r], rl, 0).lds;
+      }
       double med = cost[order[n / 2]];
       int Hmax = 0;
       while (Hmax < n && Hmax < c->n_cu && cost[order[Hmax]] > 2.0 * med) Hmax++;
+      const int nx = xcds(c->n_cu), per = c->n_cu / nx;
+      auto up = [&](int k) { return (k + nx - 1) / nx; };
+      // workgroups per CU of a class: its LDS (the largest need of its
+      // replicas, at the class budget) and 8 waves per CU at 256 VGPRs
+      auto per_cu = [&](int cnt, const CapNeed &q, size_t budget) {
+        if (cnt == 0) return 4;
+        const size_t l = (caps_of(c, q, rl, budget).lds + 1023) / 1024 * 1024;
+        return (int)std::min<size_t>(4, (160 * 1024) / l);
+      };
+      // CUs of the fullest XCD
+      auto cus = [&](int nh, int na, const CapNeed &qa, int nb, const CapNeed &qb) {
+        const int fa = per_cu(na, qa, kA), fb = per_cu(nb, qb, kB);
+        if (fa == 0 || fb == 0) return per + 1;
+        return up(nh) + (up(na) + fa - 1) / fa + (up(nb) + fb - 1) / fb;
+      };
       for (int H = Hmax; H >= 0; H--) {
         heavy.assign(order.begin(), order.begin() + H);
         clsA.clear();
@@ -2133,15 +2180,29 @@ int mdn_run_batched(mdn_ctx *c, int32_t nsteps, int32_t eflag_every, int64_t ste
           else if (own[r] <= kB) clsB.push_back(r);
           else heavy.push_back(r);
         }
-        auto cus = [&](int na, int nb) { return (int)heavy.size() + (na + 3) / 4 + (nb + 2) / 3; };
-        if (cus((int)clsA.size(), (int)clsB.size()) > c->n_cu) continue;
-        // clsA is in descending cost order
-        size_t moved = 0;
-        while (moved < clsA.size() &&
-               cus((int)(clsA.size() - moved - 1), (int)(clsB.size() + moved + 1)) <= c->n_cu)
+        // needs of clsA's suffixes (its costliest leave from the front)
+        const int na = (int)clsA.size();
+        std::vector<CapNeed> sufA(na + 1);
+        for (int k = na - 1; k >= 0; k--) { sufA[k] = sufA[k + 1]; sufA[k].add(rq[clsA[k]]); }
+        CapNeed qb;
+        for (int r : clsB) qb.add(rq[r]);
+        const int nh = (int)heavy.size(), nb = (int)clsB.size();
+        if (cus(nh, na, sufA[0], nb, qb) > per) continue;
+        // move clsA's costliest (it is in descending cost order) to clsB
+        // while the fullest XCD still holds every workgroup
+        int moved = 0;
+        while (moved < na) {
+          CapNeed q2 = qb;
+          q2.add(rq[clsA[moved]]);
+          if (cus(nh, na - moved - 1, sufA[moved + 1], nb + moved + 1, q2) > per) break;
+          qb = q2;
           moved++;
+        }
         clsB.insert(clsB.end(), clsA.begin(), clsA.begin() + moved);
         clsA.erase(clsA.begin(), clsA.begin() + moved);
+        auto by_cost = [&](int a2, int b2) { return cost[a2] > cost[b2]; };
+        std::stable_sort(heavy.begin(), heavy.end(), by_cost);
+        std::stable_sort(clsB.begin(), clsB.end(), by_cost);
         break;
       }
       order.clear();
