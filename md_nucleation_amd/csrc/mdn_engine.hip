@@ -978,6 +978,11 @@ __global__ void __launch_bounds__(512) k_batched(const DevReplica *__restrict__ 
     if (tid == 0) { B.sint[32] = 0; B.sint[33] = 0; }
     __syncthreads();
     bool dead = false;
+    // under NVE the second half-kick of a step is carried into the first of
+    // the next (the same per-atom operations in the same order: nothing
+    // between them reads the velocities); the last step of a launch applies
+    // its own
+    bool pend = false;
     for (int s = 0; s < nsteps && !dead; s++) {
       const bool eflag = eflag_every > 0 && ((step0 + s + 1) % eflag_every) == 0;
       // MD::velocity_scaling (verlet.cpp:22): the ion is kicked first, its
@@ -1003,12 +1008,16 @@ __global__ void __launch_bounds__(512) k_batched(const DevReplica *__restrict__ 
         for (int a = tid; a < L.n_ion; a += nt)
           for (int q = 0; q < 3; q++) B.v[3 * a + q] *= ratio;
       }
-      // first half-kick + drift + force clear (verlet.cpp:21-25)
+      // first half-kick + drift + force clear (verlet.cpp:21-25), after the
+      // previous step's second half-kick when it was carried over
       for (int a = tid; a < L.n_in; a += nt) {
-        double kc = vscale && a < L.n_ion ? 0.0 : B.kct[B.kidx[a]];
+        const double kf = B.kct[B.kidx[a]];
+        double kc = vscale && a < L.n_ion ? 0.0 : kf;
         bool keep = shake && a < L.gbase;
         for (int q = 0; q < 3; q++) {
-          double v = B.v[3 * a + q] + B.f[3 * a + q] * kc;
+          double v = B.v[3 * a + q];
+          if (pend) v = v + B.f[3 * a + q] * kf;
+          v = v + B.f[3 * a + q] * kc;
           B.v[3 * a + q] = v;
           if (keep) B.xo[3 * a + q] = B.xr[a].x[q];
           B.xr[a].x[q] += v * R.dt;
@@ -1152,7 +1161,8 @@ __global__ void __launch_bounds__(512) k_batched(const DevReplica *__restrict__ 
       __syncthreads();
       // second half-kick (verlet.cpp:34), then MD::nosehoover_ion on the ion
       // (verlet.cpp:36)
-      {
+      pend = thermo == THERMO_NVE && s + 1 < nsteps;
+      if (!pend) {
         const double fac = thermo == THERMO_NH ? nh_factor(zeta, R.dt) : 1.0;
         for (int a = tid; a < L.n_in; a += nt) {
           double kc = B.kct[B.kidx[a]];
@@ -1243,6 +1253,11 @@ struct mdn_ctx {
   int any_shake = 0;
   int log_cap = 0;          // vapor events per replica between drains (0: off)
   double *d_obs = nullptr;  // [8 * n_rep] observables
+  // pinned host copies for the batched chunk loop: the replicas' counts
+  // (read back behind each chunk's launches) and the launch order
+  mdn::Dyn *h_dyn = nullptr;
+  int *h_order = nullptr;
+  int h_cap = 0;
 };
 
 namespace mdn {
@@ -1631,6 +1646,8 @@ void mdn_destroy(mdn_ctx *c) {
   drop_graphs(c);
   if (c->stream) hipStreamSynchronize(c->stream);
   if (c->dbuf) hipFree(c->dbuf);
+  if (c->h_dyn) hipHostFree(c->h_dyn);
+  if (c->h_order) hipHostFree(c->h_order);
   if (c->ev0) hipEventDestroy(c->ev0);
   if (c->ev1) hipEventDestroy(c->ev1);
   if (c->evj) hipEventDestroy(c->evj);
@@ -2114,13 +2131,24 @@ int mdn_run_batched(mdn_ctx *c, int32_t nsteps, int32_t eflag_every, int64_t ste
   const size_t rl = precision == MDN_MIXED ? sizeof(float) : sizeof(double);
   const int n = (int)c->reps.size();
   const int chunk = c->batch_chunk > 0 ? c->batch_chunk : nsteps;
+  if (c->h_cap < n) {
+    if (c->h_dyn) HIPCHK(hipHostFree(c->h_dyn));
+    if (c->h_order) HIPCHK(hipHostFree(c->h_order));
+    c->h_dyn = nullptr; c->h_order = nullptr; c->h_cap = 0;
+    HIPCHK(hipHostMalloc((void **)&c->h_dyn, sizeof(Dyn) * n, hipHostMallocDefault));
+    HIPCHK(hipHostMalloc((void **)&c->h_order, sizeof(int) * n, hipHostMallocDefault));
+    c->h_cap = n;
+  }
+  // the counts of every replica at the start of each chunk: read back
+  // behind the previous chunk's launches on the same stream
   std::vector<Dyn> dyn(c->reps.size());
   std::vector<int> order(n);
   c->last_ms = 0;
+  HIPCHK(hipMemcpyAsync(c->h_dyn, c->d_dyn, sizeof(Dyn) * n, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
   for (int done = 0; done < nsteps; done += chunk) {
     const int ns = std::min(chunk, nsteps - done);
-    HIPCHK(hipStreamSynchronize(c->stream));
-    HIPCHK(hipMemcpy(dyn.data(), c->d_dyn, sizeof(Dyn) * dyn.size(), hipMemcpyDeviceToHost));
+    memcpy(dyn.data(), c->h_dyn, sizeof(Dyn) * n);
     // Work per step grows with the square of the ion + vapor atom count.  The
     // heaviest replicas get a whole CU each (512-thread workgroups: at 256
     // VGPRs one fills a CU); the rest share CUs in 128-thread workgroups, in
@@ -2170,7 +2198,13 @@ int mdn_run_batched(mdn_ctx *c, int32_t nsteps, int32_t eflag_every, int64_t ste
         if (fa == 0 || fb == 0) return per + 1;
         return up(nh) + (up(na) + fa - 1) / fa + (up(nb) + fb - 1) / fb;
       };
-      for (int H = Hmax; H >= 0; H--) {
+      // the classes for H replicas with a CU of their own; true when every
+      // workgroup can be resident at once.  Fewer own-CU replicas need
+      // fewer CUs, so the largest H that fits is found by bisection; when
+      // not even H = 0 fits, its classes are used as they are.
+      std::vector<CapNeed> sufA;
+      CapNeed qb;
+      auto attempt = [&](int H) {
         heavy.assign(order.begin(), order.begin() + H);
         clsA.clear();
         clsB.clear();
@@ -2182,14 +2216,23 @@ int mdn_run_batched(mdn_ctx *c, int32_t nsteps, int32_t eflag_every, int64_t ste
         }
         // needs of clsA's suffixes (its costliest leave from the front)
         const int na = (int)clsA.size();
-        std::vector<CapNeed> sufA(na + 1);
+        sufA.assign(na + 1, CapNeed());
         for (int k = na - 1; k >= 0; k--) { sufA[k] = sufA[k + 1]; sufA[k].add(rq[clsA[k]]); }
-        CapNeed qb;
+        qb = CapNeed();
         for (int r : clsB) qb.add(rq[r]);
-        const int nh = (int)heavy.size(), nb = (int)clsB.size();
-        if (cus(nh, na, sufA[0], nb, qb) > per) continue;
+        return cus((int)heavy.size(), na, sufA[0], (int)clsB.size(), qb) <= per;
+      };
+      if (attempt(0)) {
+        int lo = 0, hi = Hmax;
+        while (lo < hi) {
+          const int mid = (lo + hi + 1) / 2;
+          if (attempt(mid)) lo = mid;
+          else hi = mid - 1;
+        }
+        attempt(lo);
         // move clsA's costliest (it is in descending cost order) to clsB
         // while the fullest XCD still holds every workgroup
+        const int nh = (int)heavy.size(), na = (int)clsA.size(), nb = (int)clsB.size();
         int moved = 0;
         while (moved < na) {
           CapNeed q2 = qb;
@@ -2200,18 +2243,18 @@ int mdn_run_batched(mdn_ctx *c, int32_t nsteps, int32_t eflag_every, int64_t ste
         }
         clsB.insert(clsB.end(), clsA.begin(), clsA.begin() + moved);
         clsA.erase(clsA.begin(), clsA.begin() + moved);
-        auto by_cost = [&](int a2, int b2) { return cost[a2] > cost[b2]; };
-        std::stable_sort(heavy.begin(), heavy.end(), by_cost);
-        std::stable_sort(clsB.begin(), clsB.end(), by_cost);
-        break;
       }
+      auto by_cost = [&](int a2, int b2) { return cost[a2] > cost[b2]; };
+      std::stable_sort(heavy.begin(), heavy.end(), by_cost);
+      std::stable_sort(clsB.begin(), clsB.end(), by_cost);
       order.clear();
       order.insert(order.end(), heavy.begin(), heavy.end());
       order.insert(order.end(), clsB.begin(), clsB.end());
       order.insert(order.end(), clsA.begin(), clsA.end());
     }
     const int H = (int)heavy.size(), NB_ = (int)clsB.size(), NA = (int)clsA.size();
-    HIPCHK(hipMemcpyAsync(c->d_order, order.data(), sizeof(int) * n, hipMemcpyHostToDevice, c->stream));
+    memcpy(c->h_order, order.data(), sizeof(int) * n);
+    HIPCHK(hipMemcpyAsync(c->d_order, c->h_order, sizeof(int) * n, hipMemcpyHostToDevice, c->stream));
     const long long s0 = (long long)(step0 + done);
     HIPCHK(hipEventRecord(c->ev0, c->stream));
     int rc;
@@ -2254,8 +2297,10 @@ int mdn_run_batched(mdn_ctx *c, int32_t nsteps, int32_t eflag_every, int64_t ste
     }
     c->last_heavy = H;
     HIPCHK(hipEventRecord(c->ev1, c->stream));
+    if (done + chunk < nsteps)
+      HIPCHK(hipMemcpyAsync(c->h_dyn, c->d_dyn, sizeof(Dyn) * n, hipMemcpyDeviceToHost, c->stream));
     float ms = 0;
-    HIPCHK(hipEventSynchronize(c->ev1));
+    HIPCHK(hipStreamSynchronize(c->stream));
     HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
     c->last_ms += ms;
   }
