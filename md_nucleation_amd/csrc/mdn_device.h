@@ -328,6 +328,37 @@ __device__ inline double wave_sum(double v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
   return v;
 }
+// a DPP move of a double: lanes whose source is outside the control's
+// reach (or in a row the row mask leaves out) read 0
+template <int CTRL, int ROWS>
+__device__ inline double dpp_take(double v) {
+  const long long b = __double_as_longlong(v);
+  int lo, hi;
+  if (ROWS == 0xf) {  // every row: bound_ctrl gives the 0
+    lo = __builtin_amdgcn_mov_dpp((int)b, CTRL, 0xf, 0xf, true);
+    hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xf, 0xf, true);
+  } else {  // rows left out keep the old value, 0
+    lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, ROWS, 0xf, false);
+    hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROWS, 0xf, false);
+  }
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+// wave_sum for a wave whose 64 lanes are all active, without the LDS
+// permute path: inclusive sums within each row of 16 lanes (row_shr 1, 2,
+// 4, 8), then row 0's total into row 1 and row 2's into row 3
+// (row_bcast:15), rows 0-1 into rows 2-3 (row_bcast:31); lane 63 then
+// holds the total, which every lane reads
+__device__ inline double wave_sum_full(double v) {
+  v += dpp_take<0x111, 0xf>(v);
+  v += dpp_take<0x112, 0xf>(v);
+  v += dpp_take<0x114, 0xf>(v);
+  v += dpp_take<0x118, 0xf>(v);
+  v += dpp_take<0x142, 0xa>(v);
+  v += dpp_take<0x143, 0xc>(v);
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)b, 63), hi = __builtin_amdgcn_readlane((int)(b >> 32), 63);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
 // Tion of Observer::computeProps (observer.cpp:27-36) from sum m v^2 over
 // the N ion atoms
 __host__ __device__ inline double ion_temperature(double mv2, int n) {

@@ -802,7 +802,7 @@ __device__ inline int dense_n3(const DevReplica &R, const BatchLds<Rl> &B, int n
         double dx, dy, dz;
         const double sv = dense_pair<Rl, EF, kPairIonVap>(dense_load(B, i), c, c.ok, dx, dy, dz, e_ion, e_vi, e_vv);
         fx = fma(sv, dx, fx); fy = fma(sv, dy, fy); fz = fma(sv, dz, fz);
-        const double sx = wave_sum(-sv * dx), sy = wave_sum(-sv * dy), sz = wave_sum(-sv * dz);
+        const double sx = wave_sum_full(-sv * dx), sy = wave_sum_full(-sv * dy), sz = wave_sum_full(-sv * dz);
         if (lane == 0) flush(i, sx, sy, sz);
       }
       if (v < nh) flush(v, fx, fy, fz);
@@ -1075,7 +1075,7 @@ __global__ void __launch_bounds__(512) k_batched(const DevReplica *__restrict__ 
             double m = B.kms[i];
             for (int q = 0; q < 3; q++) { c[q] += B.xr[i].x[q] * m; c[3 + q] += B.v[3 * i + q] * m; }
           }
-          for (int k = 0; k < 6; k++) c[k] = wave_sum(c[k]) * (1 / I.molmass);
+          for (int k = 0; k < 6; k++) c[k] = wave_sum_full(c[k]) * (1 / I.molmass);
           if (tid < 6) B.sdbl[tid] = c[tid];
           // blend weights of the gas atoms (verlet.cpp:109-134)
           for (int a = L.gbase + tid; a < L.n_in; a += 64 * nA) {
