@@ -47,6 +47,12 @@ struct DevSpecies {
   const double *mass, *invmass, *charge, *tmpl;
   const double *se, *sg;  // per atom sqrt(eps) and sigma of its type
   const int *type, *bond, *angle, *dih, *rigid;
+  // the constraints in groups that share no atom (at most 4; one when the
+  // molecule has more than 64 atoms): group g is rigid triples
+  // [shake_gs[g], shake_gs[g + 1]) and moves the atoms of bit mask
+  // shake_gm[g].  Within a group the constraints keep their file order.
+  int shake_ng, shake_gs[5];
+  unsigned long long shake_gm[4];
 };
 
 // mutable per-replica scalars

@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <functional>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -410,6 +411,7 @@ __device__ inline BatchLds<Rl> carve(char *base, int cap, int cap_h, int kcap, i
   B.kms = d; d += kcap;
   B.shk.c = d; d += 8 * (size_t)scap;
   B.shk.cons = reinterpret_cast<unsigned long long *>(d); d += 2;
+  B.shk.gm = reinterpret_cast<unsigned long long *>(d); d += 8;
   B.shk.cap = scap;
   B.pk = reinterpret_cast<typename Pack4<Rl>::T *>(d);
   d += (4 * sizeof(Rl) * (size_t)cap_h) / sizeof(double);
@@ -419,6 +421,8 @@ __device__ inline BatchLds<Rl> carve(char *base, int cap, int cap_h, int kcap, i
   B.sint = ip; ip += 64;
   B.shk.n = ip; ip += 2;
   B.shk.ij = ip; ip += 4 * (size_t)scap;
+  B.shk.ng = ip; ip += 2;
+  B.shk.gs = ip; ip += 10;
   B.allow = reinterpret_cast<uint32_t *>(ip); ip += acap;
   int16_t *hp = reinterpret_cast<int16_t *>(ip);
   B.kidx = hp; hp += cap;
@@ -427,8 +431,8 @@ __device__ inline BatchLds<Rl> carve(char *base, int cap, int cap_h, int kcap, i
 
 static size_t batch_lds_bytes(int cap, int cap_h, int kcap, int acap, int scap, size_t rl) {
   return sizeof(double) * ((size_t)10 * cap + 3 * (size_t)cap_h + 2 * (size_t)kcap + (U_N + 6) * 16 + 16 +
-                           8 * (size_t)scap + 2) +
-         sizeof(int) * (2 + 4 * (size_t)scap) +
+                           8 * (size_t)scap + 2 + 8) +
+         sizeof(int) * (2 + 4 * (size_t)scap + 12) +
          ((4 * rl * (size_t)kcap + sizeof(double) - 1) / sizeof(double)) * sizeof(double) +
          rl * 4 * (size_t)cap_h + sizeof(int) * (64 + (size_t)acap) + sizeof(int16_t) * (size_t)cap;
 }
@@ -1048,6 +1052,34 @@ __global__ void __launch_bounds__(512) k_batched(const DevReplica *__restrict__ 
         atomicMax(&B.sint[32], it);
         if (R.shake == 1) shake_velocity_lds(B.shk, sidx, S.natoms, X, XO, V, R.dt);
       };
+      // the same for constraint group g of G of a vapor molecule (one lane
+      // per group): the group's own SHAKE velocities; the first lane counts
+      // and, on a failure, undoes the whole molecule
+      auto shake_grp = [&](const DevSpecies &S, int base, int g, int G) {
+        XRec<Rl> *x = B.xr + base;
+        double *v = B.v + 3 * base;
+        const double *xo = B.xo + 3 * base;
+        auto X = [&](int i, int q) -> double & { return x[i].x[q]; };
+        auto XO = [&](int i, int q) -> double { return xo[3 * i + q]; };
+        auto V = [&](int i, int q) -> double & { return v[3 * i + q]; };
+        int it = shake_group_lds(R, B.shk, 1, g, G, X, XO, R.shake_max_iter);
+        if (it < 0) {
+          if (g == 0) {
+            atomicAdd(&B.sint[33], 1);
+            atomicMax(&B.sint[32], -it);
+            shake_undo(S, X, XO, V, R.dt);
+          }
+          return;
+        }
+        if (g == 0) atomicMax(&B.sint[32], it);
+        if (R.shake == 1) {
+          const double inv_dt = 1.0 / R.dt;
+          const unsigned long long gm = B.shk.gm[4 + g];
+          for (int a = 0; a < S.natoms; a++)
+            if ((gm >> a) & 1ull)
+              for (int q = 0; q < 3; q++) V(a, q) = (X(a, q) - XO(a, q)) * inv_dt;
+        }
+      };
       const DevSpecies &Vs = R.sp[2];
       const int nwv = nt >> 6;
       // A step with constrained vapor, an unconstrained ion and no rebuild
@@ -1060,13 +1092,21 @@ __global__ void __launch_bounds__(512) k_batched(const DevReplica *__restrict__ 
       const bool overlap =
           shake && Vs.nrigid && !I.nrigid && L.nv > 0 && nwv >= 2 && lp + 1 <= lu && thermo == THERMO_NVE;
       if (overlap) {
-        const int nS = min(nwv - 1, (L.nv + 63) >> 6), nA = nwv - nS;
+        // a vapor molecule whose constraints fall into G groups that share
+        // no atom takes G adjacent lanes, one group each
+        const int G = __builtin_amdgcn_readfirstlane(B.shk.ng[1]);
+        const int gsh = G >= 4 ? 2 : G >= 2 ? 1 : 0;
+        const int nS = min(nwv - 1, ((L.nv << gsh) + 63) >> 6), nA = nwv - nS;
         const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
         if (wave >= nA) {
           // the SHAKE chain is a long run of dependent instructions: let it
           // win instruction issue on its SIMD over the waves of force work
           __builtin_amdgcn_s_setprio(2);
-          for (int m = tid - 64 * nA; m < L.nv; m += 64 * nS) shake_one(Vs, L.vbase + m * L.na_v);
+          if (gsh == 0)
+            for (int m = tid - 64 * nA; m < L.nv; m += 64 * nS) shake_one(Vs, L.vbase + m * L.na_v);
+          else
+            for (int slot = tid - 64 * nA; slot < (L.nv << gsh); slot += 64 * nS)
+              shake_grp(Vs, L.vbase + (slot >> gsh) * L.na_v, slot & ((1 << gsh) - 1), 1 << gsh);
           __builtin_amdgcn_s_setprio(0);
         } else {
           // ion COM (verlet.cpp:90-107), every phase-A wave for itself
@@ -1344,6 +1384,50 @@ static void drop_graphs(mdn_ctx *c) {
         if (h.graph[p][e]) { (void)hipGraphExecDestroy(h.graph[p][e]); h.graph[p][e] = nullptr; }
 }
 
+// The rigid triples of a species ordered by constraint group (connected
+// components of the constraint graph, in order of first appearance, merged
+// into at most 4), each group keeping the file order; with D, also the
+// group bounds and atom masks.  Constraints of different groups move
+// different atoms, so a sweep in this order gives the same positions, to
+// the bit, as one in the file order (shake.cpp:51-90).
+static std::vector<int> shake_groups(const Species &S, DevSpecies *D) {
+  const int nr = (int)S.rigid.size() / 3;
+  std::vector<int> comp(S.natoms);
+  for (int a = 0; a < S.natoms; a++) comp[a] = a;
+  std::function<int(int)> find = [&](int a) { return comp[a] == a ? a : comp[a] = find(comp[a]); };
+  for (int k = 0; k < nr; k++) {
+    const int a = find(S.rigid[3 * k]), b = find(S.rigid[3 * k + 1]);
+    if (a != b) comp[std::max(a, b)] = std::min(a, b);
+  }
+  std::vector<int> gid(nr), roots;
+  for (int k = 0; k < nr; k++) {
+    const int root = find(S.rigid[3 * k]);
+    int g = (int)(std::find(roots.begin(), roots.end(), root) - roots.begin());
+    if (g == (int)roots.size()) roots.push_back(root);
+    gid[k] = S.natoms <= 64 ? std::min(g, 3) : 0;
+  }
+  const int ng = nr == 0 ? 1 : *std::max_element(gid.begin(), gid.end()) + 1;
+  std::vector<int> out;
+  int gs[5] = {0, 0, 0, 0, 0};
+  unsigned long long gm[4] = {0, 0, 0, 0};
+  for (int g = 0; g < ng; g++) {
+    gs[g] = (int)out.size() / 3;
+    for (int k = 0; k < nr; k++)
+      if (gid[k] == g) {
+        out.insert(out.end(), S.rigid.begin() + 3 * k, S.rigid.begin() + 3 * k + 3);
+        for (int e = 0; e < 2; e++)
+          if (S.rigid[3 * k + e] < 64) gm[g] |= 1ull << S.rigid[3 * k + e];
+      }
+  }
+  gs[ng] = nr;
+  if (D) {
+    D->shake_ng = ng;
+    for (int g = 0; g < 5; g++) D->shake_gs[g] = g <= ng ? gs[g] : nr;
+    for (int g = 0; g < 4; g++) D->shake_gm[g] = gm[g];
+  }
+  return out;
+}
+
 static int upload_all(mdn_ctx *c) {
   drop_graphs(c);
   Layout lay;
@@ -1372,7 +1456,7 @@ static int upload_all(mdn_ctx *c) {
       o.sp[s][5] = lay.vec(S.bond, true);
       o.sp[s][6] = lay.vec(S.angle, true);
       o.sp[s][7] = lay.vec(S.dih, true);
-      o.sp[s][8] = lay.vec(S.rigid, true);
+      o.sp[s][8] = lay.vec(shake_groups(S, nullptr), true);
       std::vector<double> se(S.type.size()), sg(S.type.size());
       for (size_t k = 0; k < S.type.size(); k++) {
         se[k] = sqrt(d.atypes[S.type[k]].eps);
@@ -1472,6 +1556,7 @@ static int upload_all(mdn_ctx *c) {
       D.charge = at<double>(B, o.sp[s][2]); D.tmpl = at<double>(B, o.sp[s][3]);
       D.type = at<int>(B, o.sp[s][4]); D.bond = at<int>(B, o.sp[s][5]); D.angle = at<int>(B, o.sp[s][6]);
       D.dih = at<int>(B, o.sp[s][7]); D.rigid = at<int>(B, o.sp[s][8]);
+      shake_groups(S, &D);
       D.se = at<double>(B, o.sp[s][9]); D.sg = at<double>(B, o.sp[s][10]);
     }
     R.n_gas = d.n_gas; R.n_vap = d.n_vap; R.n_mol = d.n_mol; R.n_atoms = d.n_atoms;

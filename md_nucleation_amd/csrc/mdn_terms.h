@@ -645,6 +645,8 @@ struct ShakeLds {
   int *ij;       // [2][cap][2]
   double *c;     // [2][cap][4]: d^2, 1/m_i, 1/m_j, factor
   unsigned long long *cons;  // [2]
+  int *ng, *gs;              // [2], [2][5]: constraint groups (DevSpecies)
+  unsigned long long *gm;    // [2][4]
   int cap;
 };
 
@@ -665,6 +667,9 @@ __device__ inline void shake_lds_fill(const DevReplica &R, const DevSpecies &Sp,
   }
   T.n[s] = Sp.nrigid;
   T.cons[s] = cons;
+  T.ng[s] = Sp.shake_ng;
+  for (int g = 0; g < 5; g++) T.gs[5 * s + g] = Sp.shake_gs[g];
+  for (int g = 0; g < 4; g++) T.gm[4 * s + g] = Sp.shake_gm[g];
 }
 
 // shake_molecule with the constraint data (and the old bond vectors) read
@@ -713,6 +718,66 @@ __device__ int shake_molecule_lds(const DevReplica &R, const ShakeLds &T, int s,
       if (!(dr * dr <= 1e30)) return -it;
       if (R.shake == 2 && dr * dr > tolerance) converge = false;
     }
+  }
+  return it;
+}
+
+// shake_molecule_lds for one constraint group g of a molecule whose G
+// groups (1, 2 or 4) sit on adjacent lanes of one quad: each lane sweeps its
+// own group, and after every sweep the molecule's lanes agree on whether all
+// groups converged (one more sweep for all while any has not, as in the
+// single loop over every constraint) or one ran away.  The groups move
+// disjoint atoms, so the positions are those of shake_molecule_lds, to the
+// bit; a sweep takes as long as the largest group.
+__device__ inline int quad_and(int v, int G) {
+  if (G >= 2) v &= __builtin_amdgcn_update_dpp(v, v, 0xB1, 0xf, 0xf, false);  // lanes 1 0 3 2
+  if (G >= 4) v &= __builtin_amdgcn_update_dpp(v, v, 0x4E, 0xf, 0xf, false);  // lanes 2 3 0 1
+  return v;
+}
+template <class XA, class RA>
+__device__ int shake_group_lds(const DevReplica &R, const ShakeLds &T, int s, int g, int G, XA x, RA r,
+                               int max_iter) {
+  const int k0 = T.gs[5 * s + g], k1 = T.gs[5 * s + g + 1];
+  const int *ij = T.ij + 2 * (size_t)s * T.cap;
+  const double *cc = T.c + 4 * (size_t)s * T.cap;
+  const double tolerance = 1e-20, rho_zero = 0.01;
+  int it = 0;
+  bool converge = false;
+  while (!converge) {
+    if (it >= max_iter) return -it;
+    it++;
+    int conv = 1, ok = 1;
+    for (int k = k0; k < k1; k++) {
+      const int i = ij[2 * k], j = ij[2 * k + 1];
+      const double dk2 = cc[4 * k], mi = cc[4 * k + 1], mj = cc[4 * k + 2], hm = cc[4 * k + 3];
+      double rk[3], rn[3];
+      for (int q = 0; q < 3; q++) { rk[q] = r(i, q) - r(j, q); rn[q] = x(i, q) - x(j, q); }
+      double rn2 = rn[0] * rn[0] + rn[1] * rn[1] + rn[2] * rn[2];
+      double dot = rk[0] * rn[0] + rk[1] * rn[1] + rk[2] * rn[2];
+      if (R.shake != 2) {
+        double d0 = rn2 - dk2;
+        if (d0 * d0 > tolerance) conv = 0;
+      }
+      double ramk = (dk2 - rn2) * hm;
+      if (dot > 1e-10) ramk *= rcp_nr(dot);
+      else {
+        double rk2 = rk[0] * rk[0] + rk[1] * rk[1] + rk[2] * rk[2];
+        ramk = 0.5 * rho_zero + ramk / (rk2 * rho_zero) * (mi + mj);
+      }
+      double cfi = ramk * mi, cfj = ramk * mj;
+      for (int q = 0; q < 3; q++) {
+        double ni = x(i, q) + rk[q] * cfi, nj = x(j, q) - rk[q] * cfj;
+        x(i, q) = ni;
+        x(j, q) = nj;
+        rn[q] = ni - nj;
+      }
+      rn2 = rn[0] * rn[0] + rn[1] * rn[1] + rn[2] * rn[2];
+      double dr = rn2 - dk2;
+      if (!(dr * dr <= 1e30)) { ok = 0; break; }
+      if (R.shake == 2 && dr * dr > tolerance) conv = 0;
+    }
+    if (!quad_and(ok, G)) return -it;
+    converge = quad_and(conv, G) != 0;
   }
   return it;
 }
