@@ -39,7 +39,7 @@ PEAK_FP32_TFLOPS = 157.3
 PEAK_FP64_TFLOPS = 78.6
 # HBM bytes per step of the heavy launch (156 replicas with a CU each), from
 # the FETCH_SIZE / WRITE_SIZE passes in profiles/k_batched_mixed_512replicas.md
-# (copy-calibrated: FETCH_SIZE x 2.0, WRITE_SIZE x 1.0; 31.2 MB per 200 steps)
+# (copy-calibrated: FETCH_SIZE x 2.0, WRITE_SIZE x 1.0; 31.5 MB per 200 steps)
 HEAVY_HBM_BYTES_PER_STEP = 0.158e6
 # algorithmic FLOPs per unique pair / bonded term, as written in the reference
 FLOP = {"ljcoul_w": 31, "ljcoul": 30, "lj": 24, "bond": 22, "angle": 70, "dihedral": 150}
