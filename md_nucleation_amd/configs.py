@@ -91,6 +91,50 @@ def write_methanol(path):
     return write_molecule(path, at, bt, ct, dt, atoms, bonds, angles, dih)
 
 
+def _bonded_site(a, b, r, theta, normal):
+    """a point at distance r from atom a whose bond makes angle theta
+    (radians) with the direction a -> b, in the plane of b - a and normal"""
+    u = (b - a) / np.linalg.norm(b - a)
+    w = np.cross(normal, u)
+    w /= np.linalg.norm(w)
+    return a + r * (math.cos(theta) * u + math.sin(theta) * w)
+
+
+def write_glycol(path):
+    """Ethylene glycol HO-CH2-CH2-OH with GAFF-like parameters (methanol's
+    types plus c3-c3): under 'SHAKE H' its six X-H bonds fall into four
+    groups that share no atom (O-H, the two C-H of each carbon, O-H)."""
+    at = [("#c3", 12.01, 0.1094, 3.39967), ("#oh", 16.00, 0.2104, 3.06647),
+          ("#ho", 1.008, 0.01, 1.0), ("#h1", 1.008, 0.0157, 2.47135)]
+    bt = [("c3-oh", 314.1, 1.4233), ("oh-ho", 369.6, 0.9724), ("c3-h1", 335.9, 1.0969), ("c3-c3", 303.1, 1.5375)]
+    ct = [("h1-c3-h1", 39.18, 109.55), ("h1-c3-oh", 51.03, 109.88), ("c3-oh-ho", 47.09, 108.16),
+          ("c3-c3-oh", 67.7, 109.43), ("h1-c3-c3", 46.37, 110.07)]
+    tet = math.radians(109.47)
+    z, y = np.array([0.0, 0.0, 1.0]), np.array([0.0, 1.0, 0.0])
+    C1, C2 = np.zeros(3), np.array([1.5375, 0.0, 0.0])
+    O1 = _bonded_site(C1, C2, 1.4233, tet, z)
+    O2 = _bonded_site(C2, C1, 1.4233, tet, z)
+    HO1 = _bonded_site(O1, C1, 0.9724, math.radians(108.16), y)
+    HO2 = _bonded_site(O2, C2, 0.9724, math.radians(108.16), y)
+    def hydrogens(c, n1, n2):
+        # the two tetrahedral sites of carbon c opposite its bonds to n1, n2
+        u1, u2 = (n1 - c) / np.linalg.norm(n1 - c), (n2 - c) / np.linalg.norm(n2 - c)
+        bis = -(u1 + u2) / np.linalg.norm(u1 + u2)
+        out = np.cross(u1, u2) / np.linalg.norm(np.cross(u1, u2))
+        h = math.radians(54.735)
+        return [c + 1.0969 * (math.cos(h) * bis + sg * math.sin(h) * out) for sg in (1.0, -1.0)]
+    H1a, H1b = hydrogens(C1, C2, O1)
+    H2a, H2b = hydrogens(C2, C1, O2)
+    pos = [C1, C2, O1, HO1, O2, HO2, H1a, H1b, H2a, H2b]
+    types = [1, 1, 2, 3, 2, 3, 4, 4, 4, 4]
+    q = [0.185, 0.185, -0.683, 0.418, -0.683, 0.418, 0.04, 0.04, 0.04, 0.04]
+    atoms = _centre([(types[i], q[i], *pos[i]) for i in range(10)], [at[t - 1][1] for t in types])
+    bonds = [(1, 2, 4), (1, 3, 1), (3, 4, 2), (2, 5, 1), (5, 6, 2), (1, 7, 3), (1, 8, 3), (2, 9, 3), (2, 10, 3)]
+    angles = [(7, 1, 8, 1), (3, 1, 7, 2), (3, 1, 8, 2), (1, 3, 4, 3), (2, 5, 6, 3), (2, 1, 3, 4), (1, 2, 5, 4),
+              (7, 1, 2, 5), (8, 1, 2, 5), (9, 2, 10, 1), (5, 2, 9, 2), (5, 2, 10, 2), (9, 2, 1, 5), (10, 2, 1, 5)]
+    return write_molecule(path, at, bt, ct, [], atoms, bonds, angles, [])
+
+
 def write_water(path):
     at = [("#ow", 15.999, 0.1553, 3.166), ("#hw", 1.008, 0.01, 1.0)]
     bt = [("ow-hw", 553.0, 1.0)]
@@ -262,6 +306,7 @@ def molecule_files(d):
         "cl": write_cl(os.path.join(d, "cl.txt")),
         "methanol": write_methanol(os.path.join(d, "methanol.txt")),
         "water": write_water(os.path.join(d, "water.txt")),
+        "glycol": write_glycol(os.path.join(d, "glycol.txt")),
     }
 
 
