@@ -196,7 +196,7 @@ def test_shake_groups_vs_oracle(tmp_path):
     groups that share no atom (O-H, C-H x2, C-H x2, O-H), which the batched
     kernel's overlapped SHAKE runs on four lanes per molecule.  200 FP64 steps
     against the oracle's single sequential loop in file order: the positions
-    agree like those of the one-group methanol case."""
+    agree like those of methanol (two groups: C-H x3, O-H)."""
     eng, sds, orcs, Ss = _trajectory_case(tmp_path, "batched", 200, M.FP64, seeds=(3, 4), vap="glycol")
     rig = sds[0].view["species"][2]["rigid"].reshape(-1, 3)
     assert rig.shape[0] == 6
