@@ -491,9 +491,10 @@ struct SStoreT {
 // lane), k = 0..31.  Ion-ion pairs are selected by the AMBER exclusion
 // bitmask, ion-vapor pairs by vi, vapor-vapor pairs of distinct molecules by
 // vv.  A work item reads its lane's 64 selection bits for the partner block
-// once (real partner atom, and for ion-ion pairs not excluded) and rotates
-// them one slot per step; F_i and the F_j accumulators take s (x_i - x_j)
-// by FP64 fused multiply-adds.  For an ion of a few atoms the ion-vapor
+// once (real partner atom, and for ion-ion pairs not excluded), rotated so
+// that bit t belongs to the item's step t; F_i and the F_j accumulators take
+// s (x_i - x_j) by FP64 fused multiply-adds, and each step's partner data is
+// read from LDS one step ahead.  For an ion of a few atoms the ion-vapor
 // units are instead one per vapor block: each lane holds a vapor atom and
 // loops over the ion atoms, whose forces are wave sums.
 //
