@@ -350,7 +350,8 @@ def main():
         tuned = None
         if args.warmup:
             from md_nucleation_amd import autotune
-            per = max(10, args.warmup // len(autotune.DEFAULT_SHAPES))
+            # one untimed run and one trial per shape, `per` steps each
+            per = max(10, args.warmup // (len(autotune.DEFAULT_SHAPES) + 1))
             best, res = autotune.tune_batched(eng, n, steps=per, precision=M.MIXED)
             tuned = {"shape": list(best), "trials": {str(list(k)): v for k, v in res.items()},
                      "steps_per_trial": per}

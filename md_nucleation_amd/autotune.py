@@ -5,8 +5,10 @@ replicas and keeps the fastest (mdn_set_batch_shape).  The trial steps are
 real steps: they advance every replica, so tuning doubles as warmup.
 
 Candidates: split launches (the heaviest replicas in 512-thread workgroups
-with a CU each, the rest in 128-thread workgroups, four per CU) or one
+with a CU each, the rest in 128-thread workgroups that share CUs) or one
 launch of 256- or 128-thread workgroups; LDS re-sizing every `chunk` steps.
+An untimed run precedes the trials, so that none pays for the first
+launches.
 """
 from __future__ import annotations
 
@@ -28,9 +30,15 @@ def _atom_steps(eng, n):
 
 def tune_batched(eng, n_replicas, steps=100, precision=M.MIXED, shapes=DEFAULT_SHAPES, step0=0):
     """Returns (best_shape, {shape: atom-steps/s}); leaves the best applied.
-    `step0` is the step index of the first trial step (energy cadence)."""
+    `step0` is the step index of the first step it runs (energy cadence); it
+    runs (len(shapes) + 1) * steps steps."""
     results = {}
     s = step0
+    # an untimed run first, so that no trial pays for the first launches
+    eng.set_batch_shape(*shapes[0])
+    eng.run_batched(steps, step0=s, precision=precision)
+    eng.sync()
+    s += steps
     for shape in shapes:
         eng.set_batch_shape(*shape)
         eng.sync()
