@@ -515,7 +515,9 @@ struct DenseI {
 
 // Unit kinds: the pair selection that cannot change within a unit is
 // compiled out.
-enum { kPairIonIon = 1, kPairIonVap = 2, kPairVapVap = 3 };
+// kPairIonVapFull: ion-vapor units of two whole blocks, where every partner
+// is real and none is excluded, so no selection at all.
+enum { kPairIonIon = 1, kPairIonVap = 2, kPairVapVap = 3, kPairIonVapFull = 4 };
 
 // one pair (i, j) with its selection bit from the unit's partner mask:
 // returns s with the pair force on i = s (x_i - x_j) and on j the opposite,
@@ -549,7 +551,7 @@ __device__ inline double dense_pair(const DenseJ<Rl> &q, const DenseI<Rl> &c, ui
   const Rl wj = q.w;
   // vapor-vapor pairs of one molecule are intramolecular (integer test, no
   // short-circuit control flow)
-  const uint32_t on = CLS == kPairVapVap ? pass & (pj.w != c.mi ? 1u : 0u) : pass;
+  const uint32_t on = CLS == kPairVapVap ? pass & (pj.w != c.mi ? 1u : 0u) : CLS == kPairIonVapFull ? 1u : pass;
   // r^2 from the FP64 displacement, rounded once to pair precision
   const Rl r2 = (Rl)(dx * dx + dy * dy + dz * dz);
   const Rl rsq = on ? r2 : (Rl)1;
@@ -567,7 +569,7 @@ __device__ inline double dense_pair(const DenseJ<Rl> &q, const DenseI<Rl> &c, ui
   const Rl fc = c.qi * pj.x * rinv;
   if (EF) {
     const double ep = on ? (double)(e24 * (Rl)(1.0 / 6.0) * u3 * (u3 - (Rl)1) + fc) : 0.0;
-    (CLS == kPairIonIon ? e_ion : CLS == kPairIonVap ? e_vi : e_vv) += ep;
+    (CLS == kPairIonIon ? e_ion : CLS == kPairVapVap ? e_vv : e_vi) += ep;
   }
   // the pair-precision scalar force over r, applied to the FP64
   // displacement by the caller
@@ -760,6 +762,9 @@ __device__ inline int dense_n3(const DevReplica &R, const BatchLds<Rl> &B, int n
       if (cls == kPairIonIon)
         dense_run_off<Rl, EF, kPairIonIon>(B, c, lane, jb, kb, hL, jm, co, sa, sb, fx, fy, fz, a0, a1, e_ion, e_vi,
                                            e_vv);
+      else if (cls == kPairIonVapFull)
+        dense_run_off<Rl, EF, kPairIonVapFull>(B, c, lane, jb, kb, hL, jm, co, sa, sb, fx, fy, fz, a0, a1, e_ion,
+                                               e_vi, e_vv);
       else if (cls == kPairIonVap)
         dense_run_off<Rl, EF, kPairIonVap>(B, c, lane, jb, kb, hL, jm, co, sa, sb, fx, fy, fz, a0, a1, e_ion, e_vi,
                                            e_vv);
@@ -791,6 +796,8 @@ __device__ inline int dense_n3(const DevReplica &R, const BatchLds<Rl> &B, int n
         for (int part = 0; part < np; part++, item++)
           if (item % nw == wave) {
             if (swap) run(kPairIonVap, false, true, ni + 64 * bj, nh, 64 * bi, ni, part);
+            else if (!vh && 64 * bi + 64 <= ni && ni + 64 * bj + 64 <= nh)
+              run(kPairIonVapFull, false, false, 64 * bi, ni, ni + 64 * bj, nh, part);
             else run(kPairIonVap, false, vh, 64 * bi, ni, ni + 64 * bj, nh, part);
           }
       }
