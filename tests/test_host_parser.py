@@ -89,3 +89,42 @@ def test_parse_nvtion_values(tmp_path):
     c = configs.Condition(m["na"], m["methanol"], 2, "N2", 10, nvt_ion="warm").write(str(tmp_path / "n.cond"))
     with pytest.raises(M.MdnError, match="NVTion"):
         M.SysDef(c)
+
+
+def test_glycol_shake_constraints(tmp_path):
+    """The ethylene glycol writer: under SHAKE H its six X-H bonds are the
+    rigid pairs (four groups that share no atom), the template meets every
+    constraint length, and the oracle's SHAKE keeps them through 20 steps."""
+    from md_nucleation_amd import configs
+    from tests import engine_util as E
+    m = configs.molecule_files(str(tmp_path))
+    cond = configs.Condition(m["na"], m["glycol"], 6, "N2", 100, shake_h=True).write(str(tmp_path / "g.cond"))
+    sd = M.SysDef(cond)
+    v = sd.view
+    sp = v["species"][2]
+    rig = sp["rigid"].reshape(-1, 3)
+    assert sp["mass"].size == 10 and rig.shape[0] == 6
+    # atom-disjoint groups of the constraint graph
+    groups = []
+    for i, j, _ in rig:
+        hit = [g for g in groups if i in g or j in g]
+        for g in hit:
+            groups.remove(g)
+        groups.append(set().union({int(i), int(j)}, *hit))
+    assert len(groups) == 4
+    tmpl = sp["tmpl_x"].reshape(-1, 3)
+    for i, j, t in rig:
+        assert abs(np.linalg.norm(tmpl[i] - tmpl[j]) - v["btype"][2 * t + 1]) < 1e-12
+    st = sd.init_state(5)
+    orc = E.oracle_for(sd, seed=5)
+    S = orc.new_state(st.as_dict())
+    orc.make_pair(S)
+    rc, _ = orc.verlet(S, 20, eflag_every=10)
+    assert rc == 0
+    X = S.atom_x.reshape(-1, 3)
+    off = v["species"][0]["mass"].size + v["n_gas"] * v["species"][1]["mass"].size
+    for mol in np.flatnonzero(S.region[1 + v["n_gas"]:]):
+        o = off + mol * 10
+        for i, j, t in rig:
+            d = np.linalg.norm(X[o + i] - X[o + j])
+            assert abs(d * d - v["btype"][2 * t + 1] ** 2) < 1e-9
