@@ -308,8 +308,8 @@ def main():
                     help="also run a 256 MiB device-to-device copy (calibrates HBM byte counters under rocprofv3 --pmc)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build measures one GPU; replicas would shard across GPUs with no exchange "
-              "(see DESIGN.md), but no multi-GPU run is part of it", file=sys.stderr)
+        print("bench.py: only one GPU is measured; replicas would shard across GPUs as whole replicas "
+              "with no exchange between them, but no multi-GPU run is implemented", file=sys.stderr)
         sys.exit(2)
 
     from md_nucleation_amd import _lib as M
