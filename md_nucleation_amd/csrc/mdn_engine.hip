@@ -277,23 +277,33 @@ __global__ void k_terms(const DevReplica *__restrict__ Rp, uint32_t bits, int ef
 // vv for large in-domain vapor sets (e.g. a 2,000-methanol cluster):
 // i-centric, j tiles staged through LDS, no atomics on j; grid.y splits the j
 // range so that a launch fills the chip.  Each unordered pair is evaluated
-// from both sides, so its energy is halved.
+// from both sides, so its energy is halved.  The partner loop has no
+// branches: pairs of one molecule (the atom itself included) take r^2 = 1
+// and weight 0, and a tile is padded to a multiple of four partners with
+// far-away, zero-weight, zero-charge entries, so four partners' LDS reads
+// issue together; a padding partner's energy is masked off.  Per type pair
+// the table holds c0, c1, c0 / 12 and c1 / 6 in pair precision (the same
+// values as dividing per pair).
+template <typename Rl> struct Pack4;
+template <> struct Pack4<float> { typedef float4 T; };
+template <> struct Pack4<double> { typedef double4 T; };
+
 constexpr int kVVTile = 256;
-template <typename Rl>
-__global__ void __launch_bounds__(kVVTile) k_vv_tiled(const DevReplica *__restrict__ Rp, int eflag, int tv0, int ntv) {
+template <typename Rl, bool EF>
+__global__ void __launch_bounds__(kVVTile) k_vv_tiled(const DevReplica *__restrict__ Rp, int tv0, int ntv) {
   const DevReplica &R = *Rp;
   __shared__ double sx[kVVTile * 3];
   __shared__ double sq[kVVTile], sw[kVVTile];
   __shared__ int st[kVVTile], sm[kVVTile];
-  __shared__ double sc[2 * 64];
+  __shared__ typename Pack4<Rl>::T sc[64];
   LView L = view_from_dyn(R);
   const int nva = L.nv * L.na_v;
   // vapor-vapor coefficient table (vapor types are contiguous)
   for (int k = threadIdx.x; k < ntv * ntv; k += blockDim.x) {
     int a = k / ntv, b = k - a * ntv;
     const double *p = R.pair_coeff + 2 * ((size_t)(tv0 + a) * R.n_types + tv0 + b);
-    sc[2 * k] = p[0];
-    sc[2 * k + 1] = p[1];
+    const Rl c0 = (Rl)p[0], c1 = (Rl)p[1];
+    sc[k] = {c0, c1, c0 / Rl(12), c1 / Rl(6)};
   }
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const bool act = i < nva;
@@ -309,35 +319,56 @@ __global__ void __launch_bounds__(kVVTile) k_vv_tiled(const DevReplica *__restri
     ti = V.type[ii] - tv0;
     mi = k;
   }
+  const int trow = ti * ntv;
   int per = (nva + gridDim.y - 1) / gridDim.y;
   int j0 = blockIdx.y * per, j1 = min(nva, j0 + per);
   double fx = 0, fy = 0, fz = 0, e = 0;
+  // one partner b; valid: b is a real entry of the tile (uniform)
+  auto pair = [&](int b, bool valid) {
+    const double dx = xi[0] - sx[3 * b], dy = xi[1] - sx[3 * b + 1], dz = xi[2] - sx[3 * b + 2];
+    const bool on = sm[b] != mi;
+    const Rl r2 = (Rl)(dx * dx + dy * dy + dz * dz);
+    const typename Pack4<Rl>::T cc = sc[trow + st[b]];
+    const Rl qq = (Rl)(qi * sq[b]), w = (Rl)(wi * sw[b]);
+    Rl r2inv, rinv;
+    inv_r<Rl>(on ? r2 : (Rl)1, r2inv, rinv);
+    const Rl r6inv = r2inv * r2inv * r2inv;
+    const Rl flj = r6inv * (cc.x * r6inv - cc.y);
+    const Rl fc = qq * rinv;
+    if (EF) {
+      const double ep = (double)(r6inv * (cc.z * r6inv - cc.w)) + (double)fc;
+      e += on && valid ? ep : 0.0;
+    }
+    // the pair-precision scalar times the FP64 displacement
+    const double fpd = (double)((flj + fc) * r2inv * (on ? w : (Rl)0));
+    fx = fma(fpd, dx, fx); fy = fma(fpd, dy, fy); fz = fma(fpd, dz, fz);
+  };
   for (int jt = j0; jt < j1; jt += kVVTile) {
     __syncthreads();
-    int j = jt + threadIdx.x;
-    if (j < j1) {
+    const int nj = min(kVVTile, j1 - jt), nj4 = (nj + 3) & ~3;
+    const int b = threadIdx.x;
+    if (b < nj) {
+      const int j = jt + b;
       int k = j / L.na_v, jj = j - k * L.na_v;
       int g = R.off_vap + (R.in_vap[k] - 1 - R.n_gas) * L.na_v + jj;
-      sx[3 * threadIdx.x] = R.atom_x[3 * (size_t)g];
-      sx[3 * threadIdx.x + 1] = R.atom_x[3 * (size_t)g + 1];
-      sx[3 * threadIdx.x + 2] = R.atom_x[3 * (size_t)g + 2];
-      sq[threadIdx.x] = V.charge[jj];
-      sw[threadIdx.x] = R.atom_w[g];
-      st[threadIdx.x] = V.type[jj] - tv0;
-      sm[threadIdx.x] = k;
+      sx[3 * b] = R.atom_x[3 * (size_t)g];
+      sx[3 * b + 1] = R.atom_x[3 * (size_t)g + 1];
+      sx[3 * b + 2] = R.atom_x[3 * (size_t)g + 2];
+      sq[b] = V.charge[jj];
+      sw[b] = R.atom_w[g];
+      st[b] = V.type[jj] - tv0;
+      sm[b] = k;
+    } else if (b < nj4) {
+      sx[3 * b] = 1e6; sx[3 * b + 1] = 1e6; sx[3 * b + 2] = 1e6;
+      sq[b] = 0; sw[b] = 0; st[b] = 0; sm[b] = -2;
     }
     __syncthreads();
-    int nj = min(kVVTile, j1 - jt);
     if (act) {
-      // pair arithmetic in Rl, every pair's force accumulated in FP64
-      for (int b = 0; b < nj; b++) {
-        if (sm[b] == mi) continue;
-        double dx = xi[0] - sx[3 * b], dy = xi[1] - sx[3 * b + 1], dz = xi[2] - sx[3 * b + 2];
-        Rl rsq = (Rl)(dx * dx + dy * dy + dz * dz);
-        int c = 2 * (ti * ntv + st[b]);
-        Rl fp = ljcoul_fpair<Rl>(rsq, (Rl)sc[c], (Rl)sc[c + 1], (Rl)(qi * sq[b]), (Rl)(wi * sw[b]), eflag != 0, e);
-        const double fpd = (double)fp;  // the FP32 scalar times the FP64 displacement
-        fx = fma(fpd, dx, fx); fy = fma(fpd, dy, fy); fz = fma(fpd, dz, fz);
+      for (int b4 = 0; b4 < nj4; b4 += 4) {
+        pair(b4, true);
+        pair(b4 + 1, b4 + 1 < nj);
+        pair(b4 + 2, b4 + 2 < nj);
+        pair(b4 + 3, b4 + 3 < nj);
       }
     }
   }
@@ -346,7 +377,7 @@ __global__ void __launch_bounds__(kVVTile) k_vv_tiled(const DevReplica *__restri
     atomicAdd(&R.atom_f[3 * (size_t)gi + 1], fy);
     atomicAdd(&R.atom_f[3 * (size_t)gi + 2], fz);
   }
-  if (eflag) {
+  if (EF) {
     double s = wave_sum(0.5 * e);
     if ((threadIdx.x & 63) == 0 && s != 0.0) atomicAdd(&R.dyn->U[U_VV], s);
   }
@@ -367,9 +398,6 @@ __device__ inline double wave_rol1(double v) {
   return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
-template <typename Rl> struct Pack4;
-template <> struct Pack4<float> { typedef float4 T; };
-template <> struct Pack4<double> { typedef double4 T; };
 
 // per-atom LDS record: FP64 position and the pair-precision blend weight in
 // one 32-byte slot (the dense pass finds both from one address)
@@ -1636,7 +1664,8 @@ static int launch_terms(mdn_ctx *c, const HostReplica &h, const DevReplica *R, u
     int nva = d.n_vap * d.sp[2].natoms;  // bound on the in-domain vapor atoms
     int gx = std::max(1, (nva + kVVTile - 1) / kVVTile);
     int gy = std::max(1, std::min(64, (4 * c->n_cu + gx - 1) / gx));
-    hipLaunchKernelGGL(k_vv_tiled<Rl>, dim3(gx, gy), dim3(kVVTile), 0, c->stream, R, eflag ? 1 : 0, tv0, ntv);
+    if (eflag) hipLaunchKernelGGL((k_vv_tiled<Rl, true>), dim3(gx, gy), dim3(kVVTile), 0, c->stream, R, tv0, ntv);
+    else hipLaunchKernelGGL((k_vv_tiled<Rl, false>), dim3(gx, gy), dim3(kVVTile), 0, c->stream, R, tv0, ntv);
   }
   HIPCHK(hipGetLastError());
   return MDN_OK;
