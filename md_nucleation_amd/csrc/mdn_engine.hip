@@ -293,7 +293,7 @@ template <typename Rl, bool EF>
 __global__ void __launch_bounds__(kVVTile) k_vv_tiled(const DevReplica *__restrict__ Rp, int tv0, int ntv) {
   const DevReplica &R = *Rp;
   __shared__ double sx[kVVTile * 3];
-  __shared__ double sq[kVVTile], sw[kVVTile];
+  __shared__ Rl sq[kVVTile], sw[kVVTile];
   __shared__ int st[kVVTile], sm[kVVTile];
   __shared__ typename Pack4<Rl>::T sc[64];
   LView L = view_from_dyn(R);
@@ -320,6 +320,8 @@ __global__ void __launch_bounds__(kVVTile) k_vv_tiled(const DevReplica *__restri
     mi = k;
   }
   const int trow = ti * ntv;
+  // charge (times qqrd2e) and weight products in pair precision
+  const Rl qr = (Rl)qi, wr = (Rl)wi;
   int per = (nva + gridDim.y - 1) / gridDim.y;
   int j0 = blockIdx.y * per, j1 = min(nva, j0 + per);
   double fx = 0, fy = 0, fz = 0, e = 0;
@@ -329,7 +331,7 @@ __global__ void __launch_bounds__(kVVTile) k_vv_tiled(const DevReplica *__restri
     const bool on = sm[b] != mi;
     const Rl r2 = (Rl)(dx * dx + dy * dy + dz * dz);
     const typename Pack4<Rl>::T cc = sc[trow + st[b]];
-    const Rl qq = (Rl)(qi * sq[b]), w = (Rl)(wi * sw[b]);
+    const Rl qq = qr * sq[b], w = wr * sw[b];
     Rl r2inv, rinv;
     inv_r<Rl>(on ? r2 : (Rl)1, r2inv, rinv);
     const Rl r6inv = r2inv * r2inv * r2inv;
@@ -354,8 +356,8 @@ __global__ void __launch_bounds__(kVVTile) k_vv_tiled(const DevReplica *__restri
       sx[3 * b] = R.atom_x[3 * (size_t)g];
       sx[3 * b + 1] = R.atom_x[3 * (size_t)g + 1];
       sx[3 * b + 2] = R.atom_x[3 * (size_t)g + 2];
-      sq[b] = V.charge[jj];
-      sw[b] = R.atom_w[g];
+      sq[b] = (Rl)V.charge[jj];
+      sw[b] = (Rl)R.atom_w[g];
       st[b] = V.type[jj] - tv0;
       sm[b] = k;
     } else if (b < nj4) {
