@@ -126,10 +126,19 @@ __global__ void k_drift(const DevReplica *__restrict__ Rp, int save_old) {
 }
 
 // SHAKE per in-domain molecule with rigid pairs, shake.cpp:25-37
-__global__ void k_shake(const DevReplica *__restrict__ Rp) {
+// one thread per in-domain molecule; a molecule of up to kShakeLdsAtoms
+// atoms iterates on a copy of its positions and old positions in LDS (a
+// slot per thread, interleaved over the block's threads), so the sweeps'
+// dependent reads and writes do not go through the cache hierarchy;
+// larger molecules iterate in place in global memory.  Either way the
+// arithmetic is shake_molecule's, so the positions are the same.
+constexpr int kShakeBlk = 64, kShakeLdsAtoms = 12;
+__global__ void __launch_bounds__(kShakeBlk) k_shake(const DevReplica *__restrict__ Rp) {
   const DevReplica &R = *Rp;
+  __shared__ double s_x[kShakeLdsAtoms * 3 * kShakeBlk], s_xo[kShakeLdsAtoms * 3 * kShakeBlk];
   LView L = view_from_dyn(R);
-  int t = blockIdx.x * blockDim.x + threadIdx.x, nt = gridDim.x * blockDim.x;
+  const int tid = threadIdx.x;
+  int t = blockIdx.x * blockDim.x + tid, nt = gridDim.x * blockDim.x;
   int nm = 1 + L.nv + L.ng;
   for (int m = t; m < nm; m += nt) {
     int sp, o;
@@ -140,18 +149,31 @@ __global__ void k_shake(const DevReplica *__restrict__ Rp) {
     if (!S.nrigid) continue;
     double *x = R.atom_x + 3 * (size_t)o, *v = R.atom_v + 3 * (size_t)o;
     const double *xo = R.atom_xo + 3 * (size_t)o;
-    auto X = [&](int i, int q) -> double & { return x[3 * i + q]; };
-    auto XO = [&](int i, int q) -> double { return xo[3 * i + q]; };
     auto V = [&](int i, int q) -> double & { return v[3 * i + q]; };
-    int it = shake_molecule(R, S, X, XO, R.shake_max_iter);
-    if (it < 0) {
-      atomicAdd(&R.dyn->shake_fail, 1ull);
-      atomicMax(&R.dyn->shake_iters_max, -it);
-      shake_undo(S, X, XO, V, R.dt);
-      continue;
+    auto finish = [&](auto X, auto XO, int it) {
+      if (it < 0) {
+        atomicAdd(&R.dyn->shake_fail, 1ull);
+        atomicMax(&R.dyn->shake_iters_max, -it);
+        shake_undo(S, X, XO, V, R.dt);
+        return;
+      }
+      atomicMax(&R.dyn->shake_iters_max, it);
+      if (R.shake == 1) shake_velocity(S, X, XO, V, R.dt);
+    };
+    if (S.natoms <= kShakeLdsAtoms) {
+      auto XL = [&](int i, int q) -> double & { return s_x[(3 * i + q) * kShakeBlk + tid]; };
+      auto XOL = [&](int i, int q) -> double { return s_xo[(3 * i + q) * kShakeBlk + tid]; };
+      for (int k = 0; k < 3 * S.natoms; k++) {
+        s_x[k * kShakeBlk + tid] = x[k];
+        s_xo[k * kShakeBlk + tid] = xo[k];
+      }
+      finish(XL, XOL, shake_molecule(R, S, XL, XOL, R.shake_max_iter));
+      for (int k = 0; k < 3 * S.natoms; k++) x[k] = s_x[k * kShakeBlk + tid];
+    } else {
+      auto X = [&](int i, int q) -> double & { return x[3 * i + q]; };
+      auto XO = [&](int i, int q) -> double { return xo[3 * i + q]; };
+      finish(X, XO, shake_molecule(R, S, X, XO, R.shake_max_iter));
     }
-    atomicMax(&R.dyn->shake_iters_max, it);
-    if (R.shake == 1) shake_velocity(S, X, XO, V, R.dt);
   }
 }
 
@@ -1684,7 +1706,7 @@ static int launch_step(mdn_ctx *c, const HostReplica &h, const DevReplica *R, bo
   if (h.thermo == THERMO_SCALE) hipLaunchKernelGGL(k_ion_scale, dim3(1), dim3(1024), 0, c->stream, R);
   bool sh = d.shake && (d.sp[0].rigid.size() || d.sp[1].rigid.size() || d.sp[2].rigid.size());
   hipLaunchKernelGGL(k_drift, dim3(g), dim3(kBlk), 0, c->stream, R, sh ? 1 : 0);
-  if (sh) hipLaunchKernelGGL(k_shake, dim3(grid_for(d.n_mol, 64)), dim3(64), 0, c->stream, R);
+  if (sh) hipLaunchKernelGGL(k_shake, dim3(grid_for(d.n_mol, kShakeBlk)), dim3(kShakeBlk), 0, c->stream, R);
   hipLaunchKernelGGL(k_com_weights, dim3(1), dim3(1024), 0, c->stream, R, nh);
   hipLaunchKernelGGL(k_rebuild, dim3(1), dim3(1024), 0, c->stream, R, 0);
   int rc = launch_terms<Rl>(c, h, R, d.flags, eflag);
@@ -2107,7 +2129,7 @@ int mdn_update_position(mdn_ctx *c, int32_t r) {
   bool sh = h.def.shake && (h.def.sp[0].rigid.size() || h.def.sp[1].rigid.size() || h.def.sp[2].rigid.size());
   const DevReplica *R = c->d_reps + r;
   hipLaunchKernelGGL(k_drift, dim3(grid_for(h.def.n_atoms)), dim3(kBlk), 0, c->stream, R, sh ? 1 : 0);
-  if (sh) hipLaunchKernelGGL(k_shake, dim3(grid_for(h.def.n_mol, 64)), dim3(64), 0, c->stream, R);
+  if (sh) hipLaunchKernelGGL(k_shake, dim3(grid_for(h.def.n_mol, kShakeBlk)), dim3(kShakeBlk), 0, c->stream, R);
   hipLaunchKernelGGL(k_com_weights, dim3(1), dim3(1024), 0, c->stream, R, 0);
   HIPCHK(hipGetLastError());
   return MDN_OK;
