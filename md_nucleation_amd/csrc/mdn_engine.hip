@@ -140,6 +140,10 @@ __global__ void __launch_bounds__(kShakeBlk) k_shake(const DevReplica *__restric
   const int tid = threadIdx.x;
   int t = blockIdx.x * blockDim.x + tid, nt = gridDim.x * blockDim.x;
   int nm = 1 + L.nv + L.ng;
+  // the largest iteration count of the thread's molecules, merged over the
+  // wave before the one atomic per wave (thousands of same-address atomics
+  // would serialise at the cache)
+  int itmax = 0;
   for (int m = t; m < nm; m += nt) {
     int sp, o;
     if (m == 0) { sp = 0; o = 0; }
@@ -153,11 +157,11 @@ __global__ void __launch_bounds__(kShakeBlk) k_shake(const DevReplica *__restric
     auto finish = [&](auto X, auto XO, int it) {
       if (it < 0) {
         atomicAdd(&R.dyn->shake_fail, 1ull);
-        atomicMax(&R.dyn->shake_iters_max, -it);
+        itmax = max(itmax, -it);
         shake_undo(S, X, XO, V, R.dt);
         return;
       }
-      atomicMax(&R.dyn->shake_iters_max, it);
+      itmax = max(itmax, it);
       if (R.shake == 1) shake_velocity(S, X, XO, V, R.dt);
     };
     if (S.natoms <= kShakeLdsAtoms) {
@@ -175,6 +179,9 @@ __global__ void __launch_bounds__(kShakeBlk) k_shake(const DevReplica *__restric
       finish(X, XO, shake_molecule(R, S, X, XO, R.shake_max_iter));
     }
   }
+  // every lane of the block reaches this point (no early return above)
+  for (int k = 32; k > 0; k >>= 1) itmax = max(itmax, __shfl_xor(itmax, k));
+  if ((tid & 63) == 0 && itmax > 0) atomicMax(&R.dyn->shake_iters_max, itmax);
 }
 
 // Observer::computeProps (observer.cpp:5-36), one workgroup per replica:
@@ -214,9 +221,11 @@ __global__ void __launch_bounds__(256) k_observe(const DevReplica *__restrict__ 
   }
 }
 
-// ion COM + blend weights, verlet.cpp:90-134 (one workgroup); with nh set,
-// also the Nose-Hoover friction update that follows update_position
-// (MD::nosehoover_zeta, verlet.cpp:27)
+// ion COM + blend weights, verlet.cpp:90-134; with nh set, also the
+// Nose-Hoover friction update that follows update_position
+// (MD::nosehoover_zeta, verlet.cpp:27).  Every workgroup sums the ion COM
+// itself (the same sums in the same order, so the same value) and weighs
+// its share of the in-domain atoms; workgroup 0 writes the COM and zeta.
 __global__ void __launch_bounds__(1024) k_com_weights(const DevReplica *__restrict__ Rp, int nh) {
   const DevReplica &R = *Rp;
   __shared__ double s_red[7][16];
@@ -240,7 +249,9 @@ __global__ void __launch_bounds__(1024) k_com_weights(const DevReplica *__restri
   if (tid < 7) {
     double s = 0;
     for (int w = 0; w < (nt + 63) / 64; w++) s += s_red[tid][w];
-    if (tid == 6) {
+    if (blockIdx.x != 0) {
+      if (tid < 6) s_com[tid] = s * (1 / I.molmass);
+    } else if (tid == 6) {
       if (nh) R.dyn->zeta_ion = nh_zeta_step(R.dyn->zeta_ion, ion_temperature(s, I.natoms), R.dyn->Tnh_ion,
                                              I.natoms, R.dt);
     } else {
@@ -250,7 +261,7 @@ __global__ void __launch_bounds__(1024) k_com_weights(const DevReplica *__restri
     }
   }
   __syncthreads();
-  for (int a = L.n_ion + tid; a < L.n_in; a += nt) {
+  for (int a = L.n_ion + blockIdx.x * nt + tid; a < L.n_in; a += nt * gridDim.x) {
     int g = gatom(R, L, a);
     const double *x = R.atom_x + 3 * (size_t)g;
     double dx = x[0] - s_com[0], dy = x[1] - s_com[1], dz = x[2] - s_com[2];
@@ -1673,6 +1684,9 @@ static int grid_for(int n, int blk = kBlk, int cap = 2048) {
   int g = (n + blk - 1) / blk;
   return std::max(1, std::min(g, cap));
 }
+// k_com_weights: a 1024-thread workgroup per 1024 atoms of the replica (an
+// upper bound of the in-domain count), at most 32
+static int com_blocks(int natoms) { return std::max(1, std::min(32, (natoms + 1023) / 1024)); }
 
 // every selected potential of one replica through the per-call kernels; vv
 // goes to the tiled all-pairs kernel when the vapor's types fit its table
@@ -1707,7 +1721,7 @@ static int launch_step(mdn_ctx *c, const HostReplica &h, const DevReplica *R, bo
   bool sh = d.shake && (d.sp[0].rigid.size() || d.sp[1].rigid.size() || d.sp[2].rigid.size());
   hipLaunchKernelGGL(k_drift, dim3(g), dim3(kBlk), 0, c->stream, R, sh ? 1 : 0);
   if (sh) hipLaunchKernelGGL(k_shake, dim3(grid_for(d.n_mol, kShakeBlk)), dim3(kShakeBlk), 0, c->stream, R);
-  hipLaunchKernelGGL(k_com_weights, dim3(1), dim3(1024), 0, c->stream, R, nh);
+  hipLaunchKernelGGL(k_com_weights, dim3(com_blocks(d.n_atoms)), dim3(1024), 0, c->stream, R, nh);
   hipLaunchKernelGGL(k_rebuild, dim3(1), dim3(1024), 0, c->stream, R, 0);
   int rc = launch_terms<Rl>(c, h, R, d.flags, eflag);
   if (rc) return rc;
@@ -2130,7 +2144,7 @@ int mdn_update_position(mdn_ctx *c, int32_t r) {
   const DevReplica *R = c->d_reps + r;
   hipLaunchKernelGGL(k_drift, dim3(grid_for(h.def.n_atoms)), dim3(kBlk), 0, c->stream, R, sh ? 1 : 0);
   if (sh) hipLaunchKernelGGL(k_shake, dim3(grid_for(h.def.n_mol, kShakeBlk)), dim3(kShakeBlk), 0, c->stream, R);
-  hipLaunchKernelGGL(k_com_weights, dim3(1), dim3(1024), 0, c->stream, R, 0);
+  hipLaunchKernelGGL(k_com_weights, dim3(com_blocks(h.def.n_atoms)), dim3(1024), 0, c->stream, R, 0);
   HIPCHK(hipGetLastError());
   return MDN_OK;
 }
