@@ -249,6 +249,15 @@ int mdn_set_batch_shape(mdn_ctx *c, int32_t threads, int32_t lds_cap_atoms, int3
  * launch.  No reference counterpart. */
 int mdn_get_batch_info(mdn_ctx *c, int32_t *lds_cap_atoms, int32_t *lds_bytes, int32_t *heavy);
 
+/* launch plan of the last chunk of the last mdn_run_batched call:
+ * counts[4] = replicas in the own-CU (512-thread), three-per-CU and
+ * four-per-CU (128-thread) classes of a split launch, and in a uniform
+ * launch (mdn_set_batch_shape threads != 0, or no more replicas than CUs);
+ * order (may be NULL; at least as many entries as replicas) receives the
+ * replica ids in launch order: own-CU class first, then the three-per-CU
+ * and the four-per-CU classes.  No reference counterpart. */
+int mdn_get_batch_plan(mdn_ctx *c, int32_t *counts, int32_t *order, int32_t cap);
+
 /* ---- differentiable vi term (fitting ion-vapor parameters) ----
  * PotentialVaporIon::compute (src/potential/VaporIon.cpp:5-43) over S
  * snapshots with the ion's per-atom sigma, eps, q as inputs (mixing of

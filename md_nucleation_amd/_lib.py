@@ -95,6 +95,7 @@ _SIGS = {
     "mdn_get_counters": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_int64), C.POINTER(C.c_int64), _ip,
                                    C.POINTER(C.c_int64)]),
     "mdn_get_batch_info": (C.c_int, [C.c_void_p, _ip, _ip, _ip]),
+    "mdn_get_batch_plan": (C.c_int, [C.c_void_p, _ip, _ip, C.c_int32]),
     "mdn_vi_forward": (C.c_int, [_dp, _dp, C.c_int32, C.c_int32, C.c_int32] + [_dp] * 6 + [C.c_void_p, _dp, _dp]),
     "mdn_vi_adjoint": (C.c_int, [_dp, _dp, C.c_int32, C.c_int32, C.c_int32] + [_dp] * 6 + [C.c_void_p] + [_dp] * 5),
     "mdn_qqrd2e": (C.c_double, []),
@@ -380,6 +381,18 @@ class Engine:
         a, b, h = C.c_int32(), C.c_int32(), C.c_int32()
         check(lib().mdn_get_batch_info(self.h, C.byref(a), C.byref(b), C.byref(h)), "mdn_get_batch_info")
         return a.value, b.value, h.value
+
+    def batch_plan(self):
+        """Launch plan of the last batched chunk: ({"own_cu", "three_per_cu",
+        "four_per_cu", "uniform"} -> replica count, replica ids in launch
+        order)."""
+        counts = np.zeros(4, np.int32)
+        order = np.zeros(max(len(self.sys), 1), np.int32)
+        check(lib().mdn_get_batch_plan(self.h, counts.ctypes.data_as(_ip), order.ctypes.data_as(_ip), order.size),
+              "mdn_get_batch_plan")
+        n = int(counts.sum())
+        keys = ("own_cu", "three_per_cu", "four_per_cu", "uniform")
+        return {k: int(v) for k, v in zip(keys, counts)}, order[:n].copy()
 
     def sync(self):
         check(lib().mdn_sync(self.h), "mdn_sync")

@@ -1361,6 +1361,11 @@ struct mdn_ctx {
   int *d_order = nullptr;
   double last_ms = 0;
   int last_heavy = 0;
+  // launch plan of the last chunk of mdn_run_batched: replicas in the
+  // own-CU, three-per-CU and four-per-CU classes of a split launch, and in a
+  // uniform launch; the replica ids in launch order
+  int last_classes[4] = {0, 0, 0, 0};
+  std::vector<int> last_order;
   int any_shake = 0;
   int log_cap = 0;          // vapor events per replica between drains (0: off)
   double *d_obs = nullptr;  // [8 * n_rep] observables
@@ -2456,6 +2461,11 @@ int mdn_run_batched(mdn_ctx *c, int32_t nsteps, int32_t eflag_every, int64_t ste
       c->last_lds = (int)b.lds;
     }
     c->last_heavy = H;
+    c->last_classes[0] = H;
+    c->last_classes[1] = NB_;
+    c->last_classes[2] = NA;
+    c->last_classes[3] = H + NB_ + NA > 0 ? 0 : n;
+    c->last_order = order;
     HIPCHK(hipEventRecord(c->ev1, c->stream));
     if (done + chunk < nsteps)
       HIPCHK(hipMemcpyAsync(c->h_dyn, c->d_dyn, sizeof(Dyn) * n, hipMemcpyDeviceToHost, c->stream));
@@ -2498,6 +2508,17 @@ int mdn_get_batch_info(mdn_ctx *c, int32_t *lds_cap_atoms, int32_t *lds_bytes, i
   if (heavy) *heavy = c->last_heavy;
   if (lds_cap_atoms) *lds_cap_atoms = c->last_cap;
   if (lds_bytes) *lds_bytes = c->last_lds;
+  return MDN_OK;
+}
+
+int mdn_get_batch_plan(mdn_ctx *c, int32_t *counts, int32_t *order, int32_t cap) {
+  if (!c || !counts) return set_error(MDN_E_ARG, "null argument");
+  for (int k = 0; k < 4; k++) counts[k] = c->last_classes[k];
+  if (order) {
+    if (cap < (int)c->last_order.size())
+      return set_error(MDN_E_CAPACITY, "mdn_get_batch_plan: order holds fewer entries than replicas");
+    for (size_t k = 0; k < c->last_order.size(); k++) order[k] = c->last_order[k];
+  }
   return MDN_OK;
 }
 
