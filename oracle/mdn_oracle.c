@@ -158,6 +158,16 @@ void orc_rng_normal2(const orc_system *s, uint32_t mol, int64_t rebuild, uint32_
   z[1] = r * sin(th);
 }
 
+/* n draws of one purpose for molecules mol0 .. mol0 + n - 1 (the moment
+ * tests): two uniforms or two normals per molecule into out[2n] */
+void orc_rng_many(const orc_system *s, uint32_t mol0, int n, int64_t rebuild, uint32_t purpose, int normal,
+                  double *out) {
+  for (int k = 0; k < n; k++) {
+    if (normal) orc_rng_normal2(s, mol0 + (uint32_t)k, rebuild, purpose, out + 2 * k);
+    else orc_rng_uniform2(s, mol0 + (uint32_t)k, rebuild, purpose, out + 2 * k);
+  }
+}
+
 /* variables.hpp:94-98 */
 void orc_rotation(double *X, double *Y, double *Z, double A, double B, double C, double x, double y, double z) {
   *X = cos(C) * (x * cos(B) + y * sin(A) * sin(B) - z * cos(A) * sin(B)) + sin(C) * (y * cos(A) + z * sin(A));

@@ -17,10 +17,17 @@
  *     reference never calls shake(); the harness calls it directly).
  *   - Tersoff / Stillinger-Weber: golden vectors from the reference's own
  *     compute() with its neighbour list built by its own make_pair().
- *   - RNG stream (Philox4x32-10): published known-answer vectors; the normal
- *     transform by moments; the draws themselves have no reference
- *     counterpart (the reference seeds std::random_device) -> the stream is
- *     this build's own specification, shared by oracle and GPU path.
+ *   - RNG stream (Philox4x32-10): the published Random123 known-answer
+ *     vectors; uniforms and Box-Muller normals by moments and a
+ *     Kolmogorov-Smirnov test over 10^6 draws; the resampled Maxwell
+ *     velocities and N2 spins by their standard deviation sqrt(kB T / m)
+ *     (updateGasIn.cpp:18, 43-45, 68-69); the draws themselves have no
+ *     reference counterpart (the reference seeds std::random_device), so the
+ *     stream is this build's own specification, shared by oracle and GPU.
+ *   - gas-gas: the reference's PotentialGasGas::compute run on the intended
+ *     in-domain pair list (golden gg.npz), the list by brute force.
+ *   - SHAKE mode 1 (textbook update + velocity correction): closed-form
+ *     solution of a diatomic constraint.
  */
 #ifndef MDN_ORACLE_H
 #define MDN_ORACLE_H
@@ -126,6 +133,8 @@ void orc_rng_normal2(const orc_system *s, uint32_t mol, int64_t rebuild,
                      uint32_t purpose, double z[2]);
 void orc_rng_uniform2(const orc_system *s, uint32_t mol, int64_t rebuild,
                       uint32_t purpose, double u[2]);
+void orc_rng_many(const orc_system *s, uint32_t mol0, int n, int64_t rebuild,
+                  uint32_t purpose, int normal, double *out);
 /* variables.hpp:94-98 */
 void orc_rotation(double *X, double *Y, double *Z, double A, double B, double C,
                   double x, double y, double z);

@@ -98,6 +98,7 @@ def lib():
         L.orc_n_long.restype = C.c_int
         L.orc_rng_normal2.argtypes = [C.POINTER(OrcSystem), C.c_uint32, C.c_int64, C.c_uint32, _dp]
         L.orc_rng_uniform2.argtypes = [C.POINTER(OrcSystem), C.c_uint32, C.c_int64, C.c_uint32, _dp]
+        L.orc_rng_many.argtypes = [C.POINTER(OrcSystem), C.c_uint32, C.c_int, C.c_int64, C.c_uint32, C.c_int, _dp]
         _lib = L
     return _lib
 
@@ -202,6 +203,13 @@ class Oracle:
         rc = lib().orc_verlet(self.h, C.byref(S.c), nsteps, eflag_every, step0, C.byref(a))
         S.sync()
         return rc, a.value
+
+    def rng(self, n, rebuild=0, purpose=0, normal=True, mol0=0):
+        """n draws (pairs) of the replica's stream for molecules mol0.. of one
+        rebuild and purpose: array [n, 2]"""
+        out = np.zeros(2 * n)
+        lib().orc_rng_many(C.byref(self.s), mol0, n, rebuild, purpose, 1 if normal else 0, out.ctypes.data_as(_dp))
+        return out.reshape(n, 2)
 
     def ion_temperature(self, S):
         return lib().orc_ion_temperature(self.h, C.byref(S.c))

@@ -339,6 +339,32 @@ int main(int argc, char **argv) {
     std::vector<double> res;
     for (auto &p : v->positionAtom[mol]) res.insert(res.end(), p.begin(), p.end());
     put(out, "x_shake", res);
+  } else if (o == "gg") {
+    // PotentialGasGas::compute alone on the pair list given in gg_list
+    // (molecule ids); every force cleared first.  Its own makePair pairs
+    // loop counters (GasGas.cpp:50), so the list comes from the caller;
+    // the list the reference's makePair builds is dumped as list_gg_ref.
+    Variables *v = md->vars;
+    PotentialGasGas g(md->ML2);
+    g.makePair(v);
+    std::vector<double> own;
+    for (auto &q : g.pairList) own.push_back(q.i), own.push_back(q.j);
+    g.pairList.clear();
+    const std::vector<double> &l = need(a, "gg_list");
+    for (size_t k = 0; k + 1 < l.size(); k += 2) {
+      Pair p;
+      p.i = (int)l[k];
+      p.j = (int)l[k + 1];
+      g.pairList.push_back(p);
+    }
+    for (auto &mol : v->forceAtom)
+      for (auto &f : mol) f[0] = f[1] = f[2] = 0;
+    v->Uzero();
+    md->flags->eflag = 1;
+    g.compute(v, md->flags);
+    md->flags->eflag = 0;
+    dump_state(out, md);
+    put(out, "list_gg_ref", own);
   } else if (o == "born") {
     Variables *v = md->vars;
     for (auto &f : v->forceAtom[0]) f[0] = f[1] = f[2] = 0;

@@ -42,4 +42,13 @@ def write_case(name, d):
         pot = "Tersoff" if name == "tersoff" else "Stilinger-Weber"
         return configs.Condition(si, m["water"], 1, "He", 1, ion_potential=pot, interactions={},
                                  shake_h=False).write(os.path.join(d, "si_%s.cond" % name))
+    if name == "gg":
+        # gas-gas LJ on: 200 N2 in a 94 A box (every molecule within 81 A of
+        # the ion, so every blend weight is 1) around a 30-atom chain (the
+        # reference's gg weight reads the ion's weights, GasGas.cpp:25, which
+        # then hold at least the gas molecule's atom count)
+        pep = configs.write_peptide(os.path.join(d, "pep30.txt"), seed=5, n=30)
+        return configs.Condition(pep, m["methanol"], 5, "N2", 200, pressure=1e6, ion_potential="AMBER",
+                                 interactions={"gg": "LJ", "gi": "LJ", "gv": "LJ", "vi": "LJcoul",
+                                               "vv": "LJcoul"}).write(os.path.join(d, "gg.cond"))
     raise KeyError(name)

@@ -20,7 +20,8 @@ selected term, a no-rebuild trajectory with energies, one MD::make_pair):
           of every output file (K, U, ion, dump, vapor_in, vapor_out)
 plus SHAKE on perturbed methanol / water molecules and ROTATION samples.
 
-    python tests/golden/make_golden.py nvt    # rewrites only nvt.npz (or run.npz)
+  gg      PotentialGasGas::compute on the intended in-domain gas pair list
+    python tests/golden/make_golden.py nvt    # rewrites only nvt.npz (or run.npz, gg.npz)
 """
 import os
 import subprocess
@@ -132,16 +133,52 @@ def run_case(d):
     print("run", {k: v.size for k, v in res.items() if k.startswith("file_")})
 
 
+def gg_case(d):
+    """PotentialGasGas::compute (GasGas.cpp:4-41) on the intended list: every
+    pair of in-domain gas molecules whose centres lie within ML2, found by
+    brute force from the state's molecule centres (the reference's own
+    makePair pairs loop counters, GasGas.cpp:44-63; its list is stored too)."""
+    cond = write_case("gg", d)
+    sysa = harness(cond, None, "sys", cwd=d)
+    init = harness(cond, None, "init", cwd=d)
+    st = {k: init[k] for k in STATE_KEYS}
+    # weights are first computed by update_position; inside 100 A they are 1
+    st["atom_w"] = np.ones_like(st["atom_w"])
+    ng = int(sysa["scalars"][10])
+    ml2 = float(sysa["scalars"][6])
+    mol = st["mol_x"].reshape(-1, 3)
+    ids = [i for i in range(1, 1 + ng) if st["region"][i]]
+    lst = []
+    for a, i in enumerate(ids):
+        for j in ids[a + 1:]:
+            dd = mol[i] - mol[j]
+            if dd[0] * dd[0] + dd[1] * dd[1] + dd[2] * dd[2] < ml2:
+                lst += [i, j]
+    sp = os.path.join(d, "state.txt")
+    O.write_arrays(sp, dict(st, loopPair=[0], loop_update=[10 ** 8], margin_length=[10.0], gg_list=lst))
+    r = harness(cond, sp, "gg", cwd=d)
+    res = pack("sys_", sysa)
+    res.update(pack("init_", st))
+    res["gg_list"] = np.asarray(lst, np.float64)
+    res["gg_atom_f"] = r["atom_f"]
+    res["gg_U"] = r["U"]
+    res["gg_list_ref"] = r["list_gg_ref"]
+    np.savez_compressed(os.path.join(OUT, "gg.npz"), **res)
+    print("gg", {"pairs": len(lst) // 2, "in-domain gas": len(ids), "reference list": r["list_gg_ref"].size // 2})
+
+
 def main():
     if not os.path.exists(O.REF_HARNESS):
         subprocess.check_call(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "ref"])
-    if sys.argv[1:] in (["nvt"], ["run"]):
+    one = {"nvt": nvt_cases, "run": run_case, "gg": gg_case}
+    if len(sys.argv) == 2 and sys.argv[1] in one:
         with tempfile.TemporaryDirectory() as d:
-            (nvt_cases if sys.argv[1] == "nvt" else run_case)(d)
+            one[sys.argv[1]](d)
         return
     with tempfile.TemporaryDirectory() as d:
         nvt_cases(d)
         run_case(d)
+        gg_case(d)
         case("c1", write_case("c1", d), d, shake_mols=(601, 605))
         case("pep", write_case("pep", d), d, shake_mols=(201, 210))
         # the reference needs a gas line; its BMH table is indexed by ion-local
