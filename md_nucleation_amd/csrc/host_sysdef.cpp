@@ -446,14 +446,14 @@ void init_state(const SysDef &d, uint64_t seed, mdn_state *st) {
     std::vector<double> placed;  // atoms already placed (cluster)
     for (int i = 0; i < I.natoms; i++) placed.insert(placed.end(), &st->atom_x[3 * i], &st->atom_x[3 * i] + 3);
     double mk = d.m_vap_kg, sgm = sqrt(kKb * d.T / mk);
+    const int nv = V.natoms;
+    std::vector<double> bx(3 * (size_t)nv), tx(3 * (size_t)nv);
     for (int s = 0; s < ncl; s++) {
       int mol = first_vap + s;
       int o = d.mol_offset(mol);
-      double best = -1, bx[3 * 64];
-      int nv = std::min(V.natoms, 64);
+      double best = -1;
       for (int tr = 0; tr < 200; tr++) {
         double a1 = U01(rng) * 2 * M_PI, b1 = U01(rng) * 2 * M_PI, c1 = U01(rng) * 2 * M_PI;
-        double tx[3 * 64];
         for (int k = 0; k < nv; k++) {
           rotation(tx[3 * k], tx[3 * k + 1], tx[3 * k + 2], a1, b1, c1, V.tmpl_x[3 * k], V.tmpl_x[3 * k + 1], V.tmpl_x[3 * k + 2]);
           for (int q = 0; q < 3; q++) tx[3 * k + q] += sites[s][q];
@@ -468,7 +468,7 @@ void init_state(const SysDef &d, uint64_t seed, mdn_state *st) {
             dmin2 = std::min(dmin2, ex * ex + ey * ey + ez * ez);
           }
         }
-        if (dmin2 > best) { best = dmin2; memcpy(bx, tx, sizeof(double) * 3 * nv); }
+        if (dmin2 > best) { best = dmin2; bx = tx; }
         if (dmin2 >= 1.8 * 1.8) break;
       }
       double vc[3] = {sgm * N01(rng) * 1e-5, sgm * N01(rng) * 1e-5, sgm * N01(rng) * 1e-5};
@@ -483,13 +483,19 @@ void init_state(const SysDef &d, uint64_t seed, mdn_state *st) {
     }
   }
   // MD::initialization_gas (initializationGas.cpp:24-90): uniform COMs with
-  // minimum distance `dis` (periodic), Maxwell COM velocities.
+  // minimum distance `dis` (periodic), Maxwell COM velocities.  The
+  // reference retries without end; here a molecule that finds no free place
+  // in 10^5 tries is an error (the box is too crowded for the 10 A rule).
   auto place = [&](int mol, double m_kg) {
     double sg = sqrt(kKb * d.T / m_kg);
     for (int tries = 0;; tries++) {
+      if (tries == 100000)
+        throw std::runtime_error("molecule " + std::to_string(mol) + " found no place at least " +
+                                 std::to_string(dis) + " A from the others in 100000 tries (box " +
+                                 std::to_string(d.d_size) + " A too small for the molecule count)");
       double X[3];
       for (int k = 0; k < 3; k++) X[k] = (U01(rng) - 0.5) * d.d_size;
-      if (grid.clear_of(X, tries < 100000 ? dis : 0.0)) {
+      if (grid.clear_of(X, dis)) {
         for (int k = 0; k < 3; k++) st->mol_x[3 * mol + k] = X[k];
         for (int k = 0; k < 3; k++) st->mol_v[3 * mol + k] = sg * N01(rng) * 1e-5;
         grid.add(X);
