@@ -150,8 +150,12 @@ int mdn_add_replica(mdn_ctx *c, const mdn_sysdef *d, const mdn_state *st,
                     uint64_t seed, uint32_t replica_id, int32_t *handle);
 /* allocates device memory for all replicas and uploads them */
 int mdn_commit(mdn_ctx *c);
-/* replaces one replica's state; the other replicas keep theirs */
+/* replaces one replica's state (which starts at a step boundary, with no
+ * capacity flags); the other replicas keep theirs */
 int mdn_set_state(mdn_ctx *c, int32_t replica, const mdn_state *st);
+/* copies the replica's state out; MDN_E_CAPACITY (after copying) when a
+ * pair list overflowed its capacity on the device (the lists are then
+ * truncated) */
 int mdn_get_state(mdn_ctx *c, int32_t replica, mdn_state *st);
 /* ion thermostat of MD::verlet (verlet.cpp:22, 27, 36; nvt.cpp:5-35) for one
  * replica or all (replica = -1).  MDN_NVE is MD::setNVE (nvt.cpp:38-43, called
@@ -231,7 +235,19 @@ int mdn_verlet(mdn_ctx *c, int32_t replica, int32_t nsteps,
 int mdn_verlet_graph(mdn_ctx *c, int32_t replica, int32_t nsteps,
                      int32_t eflag_every, int64_t step0, int32_t precision);
 
-/* batched path: all replicas, nsteps in one persistent launch */
+/* batched path: all replicas, nsteps in one persistent launch.
+ * A replica whose in-domain atoms exceed the launch's LDS capacity is
+ * flagged (mdn_get_counters returns MDN_E_CAPACITY) and skipped from then
+ * on (until mdn_set_state gives it a new state).  When that happens at a
+ * launch start its state is that of the last completed step; when it
+ * happens at a rebuild inside a step, the stored state is that step in
+ * progress: positions, blend weights and pair lists of the step after its
+ * rebuild, velocities after the first half-kick, forces cleared.
+ * mdn_verlet and mdn_verlet_graph then finish that step first (forces,
+ * second half-kick), counting it as the first of their nsteps;
+ * mdn_make_pair and mdn_update_position return MDN_E_STATE until it is
+ * finished (mdn_compute followed by mdn_velocity_calculation also finishes
+ * it). */
 int mdn_run_batched(mdn_ctx *c, int32_t nsteps, int32_t eflag_every,
                     int64_t step0, int32_t precision);
 

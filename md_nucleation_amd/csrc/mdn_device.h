@@ -63,7 +63,12 @@ struct Dyn {
   int n_gi, n_gv, n_gg;
   int n_in_gas, n_in_vap;
   int status, shake_iters_max;
-  int thermo, pad;           // THERMO_* of the ion
+  int thermo;                // THERMO_* of the ion
+  // 1: the batched kernel stopped this replica inside a step (its in-domain
+  // atoms outgrew the launch's LDS at a rebuild): positions, weights and
+  // lists are those of the step, velocities have their first half-kick, the
+  // forces and the second half-kick are still to come
+  int pending;
   unsigned long long shake_fail;
   double zeta_ion, Tnh_ion;  // Nose-Hoover friction and target (nvt.cpp:19-35)
   long long step;            // MD::itime: steps since the caller last set it

@@ -1258,7 +1258,9 @@ __global__ void __launch_bounds__(512) k_batched(const DevReplica *__restrict__ 
           lp = D->loopPair;
           lu = D->loop_update;
           if (L.n_in > cap || L.gbase + 64 > cap_h) {
-            if (tid == 0) atomicOr(&D->status, ST_LDS_CAP);
+            // the stored state is step s in progress (after the rebuild):
+            // the per-call path finishes it (mdn_verlet)
+            if (tid == 0) { atomicOr(&D->status, ST_LDS_CAP); D->pending = 1; }
             dead = true;
             break;
           }
@@ -1340,6 +1342,8 @@ struct HostReplica {
   long long atom_steps = 0, step = 0;
   int status = 0, shake_iters_max = 0;
   unsigned long long shake_fail = 0;
+  // a step left unfinished by the batched kernel (Dyn::pending)
+  int pending = 0;
 };
 
 }  // namespace mdn
@@ -1588,6 +1592,7 @@ static int upload_all(mdn_ctx *c) {
     dyn.status = h.status;
     dyn.shake_iters_max = h.shake_iters_max;
     dyn.shake_fail = h.shake_fail;
+    dyn.pending = h.pending;
     for (int k = 0; k < U_N; k++) dyn.U[k] = h.U[k];
     dyn.rebuild_count = h.rebuild_count;
     dyn.loopPair = h.loopPair;
@@ -1714,6 +1719,32 @@ static int launch_terms(mdn_ctx *c, const HostReplica &h, const DevReplica *R, u
   return MDN_OK;
 }
 
+// start of the rest of a step the batched kernel left unfinished: clears the
+// flag and, on an energy step, the energy buckets
+__global__ void k_resume(const DevReplica *__restrict__ Rp, int zeroU) {
+  const DevReplica &R = *Rp;
+  if (threadIdx.x == 0) {
+    if (zeroU)
+      for (int k = 0; k < U_N; k++) R.dyn->U[k] = 0;
+    R.dyn->pending = 0;
+  }
+}
+
+// the rest of that step: the forces (verlet.cpp:31-32) and the second
+// half-kick (verlet.cpp:34, 36)
+template <typename Rl>
+static int launch_resume(mdn_ctx *c, HostReplica &h, const DevReplica *R, bool eflag) {
+  const SysDef &d = h.def;
+  hipLaunchKernelGGL(k_resume, dim3(1), dim3(64), 0, c->stream, R, eflag ? 1 : 0);
+  int rc = launch_terms<Rl>(c, h, R, d.flags, eflag);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_kick, dim3(grid_for(d.n_atoms)), dim3(kBlk), 0, c->stream, R, 0, 1,
+                     h.thermo == THERMO_NH ? 1 : 0);
+  HIPCHK(hipGetLastError());
+  h.pending = 0;
+  return MDN_OK;
+}
+
 // one MD::verlet step of one replica through the per-call kernels
 template <typename Rl>
 static int launch_step(mdn_ctx *c, const HostReplica &h, const DevReplica *R, bool eflag) {
@@ -1754,6 +1785,7 @@ static int pull_replica(mdn_ctx *c, int32_t r, bool state) {
   h.status = dyn.status;
   h.shake_iters_max = dyn.shake_iters_max;
   h.shake_fail = dyn.shake_fail;
+  h.pending = dyn.pending;
   if (!state) return MDN_OK;
   std::vector<int> gi(2 * (size_t)std::max(h.cap_gi, 1)), gv(2 * (size_t)std::max(h.cap_gv, 1)),
       gg(2 * (size_t)std::max(h.cap_gg, 1));
@@ -1899,6 +1931,9 @@ int mdn_set_state(mdn_ctx *c, int32_t r, const mdn_state *st) {
       if ((rc = pull_replica(c, q, q != r))) return rc;
   rc = copy_state_in(c->reps[r], st);
   if (rc) return rc;
+  // a new state starts at a step boundary with no capacity flags
+  c->reps[r].pending = 0;
+  c->reps[r].status = 0;
   return mdn_commit(c);
 }
 
@@ -2076,7 +2111,11 @@ int mdn_get_state(mdn_ctx *c, int32_t r, mdn_state *st) {
   if ((rc = fetch(R.gi, dyn.n_gi, st->cap_gi, st->gi, &st->n_gi, 0))) return rc;
   if ((rc = fetch(R.gv, dyn.n_gv, st->cap_gv, st->gv, &st->n_gv, 1))) return rc;
   if ((rc = fetch(R.gg, dyn.n_gg, st->cap_gg, st->gg, &st->n_gg, 2))) return rc;
-  return status_to_code(dyn.status) ? set_error(status_to_code(dyn.status), "replica status flags set") : MDN_OK;
+  // a pair list that overflowed on the device was truncated: the state is
+  // incomplete.  The LDS-capacity flag alone leaves a complete state.
+  if (dyn.status & (ST_CAP_GI | ST_CAP_GV | ST_CAP_GG))
+    return set_error(MDN_E_CAPACITY, "replica status flags set: a pair list overflowed (lists truncated)");
+  return MDN_OK;
 }
 
 int mdn_set_positions(mdn_ctx *c, int32_t r, const double *atom_x) {
@@ -2101,9 +2140,17 @@ int mdn_get_forces(mdn_ctx *c, int32_t r, double *atom_f) {
   return MDN_OK;
 }
 
+static int refuse_pending(mdn_ctx *c, int32_t r, const char *fn) {
+  if (!c->reps[r].pending) return MDN_OK;
+  return set_error(MDN_E_STATE, std::string(fn) + ": replica " + std::to_string(r) +
+                                    " holds a step left unfinished by mdn_run_batched; finish it with mdn_verlet "
+                                    "(or mdn_compute followed by mdn_velocity_calculation)");
+}
+
 int mdn_make_pair(mdn_ctx *c, int32_t r) {
   int rc = check_replica(c, r);
   if (rc) return rc;
+  if ((rc = refuse_pending(c, r, "mdn_make_pair"))) return rc;
   HIPCHK(hipSetDevice(c->device));
   hipLaunchKernelGGL(k_rebuild, dim3(1), dim3(1024), 0, c->stream, c->d_reps + r, 1);
   HIPCHK(hipGetLastError());
@@ -2135,6 +2182,10 @@ int mdn_compute(mdn_ctx *c, int32_t r, uint32_t bits, int32_t eflag, int32_t pre
 int mdn_velocity_calculation(mdn_ctx *c, int32_t r) {
   int rc = check_replica(c, r);
   if (rc) return rc;
+  if (c->reps[r].pending) {  // this kick ends the step left unfinished
+    hipLaunchKernelGGL(k_resume, dim3(1), dim3(64), 0, c->stream, c->d_reps + r, 0);
+    c->reps[r].pending = 0;
+  }
   hipLaunchKernelGGL(k_kick, dim3(grid_for(c->reps[r].def.n_atoms)), dim3(kBlk), 0, c->stream, c->d_reps + r, 0, 0,
                      0);
   HIPCHK(hipGetLastError());
@@ -2144,6 +2195,7 @@ int mdn_velocity_calculation(mdn_ctx *c, int32_t r) {
 int mdn_update_position(mdn_ctx *c, int32_t r) {
   int rc = check_replica(c, r);
   if (rc) return rc;
+  if ((rc = refuse_pending(c, r, "mdn_update_position"))) return rc;
   const HostReplica &h = c->reps[r];
   bool sh = h.def.shake && (h.def.sp[0].rigid.size() || h.def.sp[1].rigid.size() || h.def.sp[2].rigid.size());
   const DevReplica *R = c->d_reps + r;
@@ -2158,11 +2210,16 @@ int mdn_verlet(mdn_ctx *c, int32_t r, int32_t nsteps, int32_t eflag_every, int64
   int rc = check_replica(c, r);
   if (rc) return rc;
   HIPCHK(hipSetDevice(c->device));
-  const HostReplica &h = c->reps[r];
+  HostReplica &h = c->reps[r];
   HIPCHK(hipEventRecord(c->ev0, c->stream));
   for (int s = 0; s < nsteps; s++) {
     bool ef = eflag_every > 0 && ((step0 + s + 1) % eflag_every) == 0;
-    rc = precision == MDN_MIXED ? launch_step<float>(c, h, c->d_reps + r, ef) : launch_step<double>(c, h, c->d_reps + r, ef);
+    if (s == 0 && h.pending)  // the step the batched kernel left unfinished
+      rc = precision == MDN_MIXED ? launch_resume<float>(c, h, c->d_reps + r, ef)
+                                  : launch_resume<double>(c, h, c->d_reps + r, ef);
+    else
+      rc = precision == MDN_MIXED ? launch_step<float>(c, h, c->d_reps + r, ef)
+                                  : launch_step<double>(c, h, c->d_reps + r, ef);
     if (rc) return rc;
   }
   HIPCHK(hipEventRecord(c->ev1, c->stream));
@@ -2181,6 +2238,11 @@ int mdn_verlet_graph(mdn_ctx *c, int32_t r, int32_t nsteps, int32_t eflag_every,
   HIPCHK(hipEventRecord(c->ev0, c->stream));
   for (int s = 0; s < nsteps; s++) {
     const int ef = eflag_every > 0 && ((step0 + s + 1) % eflag_every) == 0;
+    if (s == 0 && h.pending) {  // the step the batched kernel left unfinished
+      rc = p ? launch_resume<float>(c, h, c->d_reps + r, ef != 0) : launch_resume<double>(c, h, c->d_reps + r, ef != 0);
+      if (rc) return rc;
+      continue;
+    }
     if (!h.graph[p][ef]) {
       hipGraph_t g = nullptr;
       HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
@@ -2467,13 +2529,15 @@ int mdn_run_batched(mdn_ctx *c, int32_t nsteps, int32_t eflag_every, int64_t ste
     c->last_classes[3] = H + NB_ + NA > 0 ? 0 : n;
     c->last_order = order;
     HIPCHK(hipEventRecord(c->ev1, c->stream));
-    if (done + chunk < nsteps)
-      HIPCHK(hipMemcpyAsync(c->h_dyn, c->d_dyn, sizeof(Dyn) * n, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(c->h_dyn, c->d_dyn, sizeof(Dyn) * n, hipMemcpyDeviceToHost, c->stream));
     float ms = 0;
     HIPCHK(hipStreamSynchronize(c->stream));
     HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
     c->last_ms += ms;
   }
+  // replicas stopped inside a step (LDS capacity outgrown at a rebuild): the
+  // per-call path finishes that step first
+  for (int r = 0; r < n; r++) c->reps[r].pending = c->h_dyn[r].pending;
   return MDN_OK;
 }
 
