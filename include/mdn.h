@@ -299,6 +299,25 @@ int mdn_vi_adjoint(const double *x, const double *w, int32_t S, int32_t n_ion,
                    const double *vap_q, const uint8_t *vap_in,
                    const double *gE, const double *gF,
                    double *g_sigma, double *g_eps, double *g_q);
+/* mdn_vi_adjoint for callers that cannot allocate (code under HIP graph
+ * capture, tight fitting loops, harnesses that forbid hipMalloc).  No
+ * counterpart in the reference.  Every pointer is a DEVICE pointer on the
+ * current device (same layouts as mdn_vi_adjoint); the kernels are queued on
+ * `stream` (a hipStream_t; NULL = the null stream) and the call returns
+ * without waiting for them.  It allocates nothing: the partial gradients go
+ * to a fixed 4 MiB device array of the library, which bounds
+ * S * n_ion <= 174762 (larger batches return MDN_E_ARG; use mdn_vi_adjoint).
+ * Each GPU holds one such array for the whole process and every call on that
+ * GPU uses it, so calls that overlap on one GPU (concurrent streams or host
+ * threads) race; calls in one stream are ordered and safe. */
+int mdn_vi_adjoint_noalloc(const double *x, const double *w, int32_t S,
+                           int32_t n_ion, int32_t n_vap,
+                           const double *ion_sigma, const double *ion_eps,
+                           const double *ion_q, const double *vap_sigma,
+                           const double *vap_eps, const double *vap_q,
+                           const uint8_t *vap_in, const double *gE,
+                           const double *gF, double *g_sigma, double *g_eps,
+                           double *g_q, void *stream);
 /* Coulomb constant of the reference (constants.hpp:50), kcal A/(mol e^2) */
 double mdn_qqrd2e(void);
 

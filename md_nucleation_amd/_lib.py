@@ -98,6 +98,7 @@ _SIGS = {
     "mdn_get_batch_plan": (C.c_int, [C.c_void_p, _ip, _ip, C.c_int32]),
     "mdn_vi_forward": (C.c_int, [_dp, _dp, C.c_int32, C.c_int32, C.c_int32] + [_dp] * 6 + [C.c_void_p, _dp, _dp]),
     "mdn_vi_adjoint": (C.c_int, [_dp, _dp, C.c_int32, C.c_int32, C.c_int32] + [_dp] * 6 + [C.c_void_p] + [_dp] * 5),
+    "mdn_vi_adjoint_noalloc": (C.c_int, [C.c_void_p] * 2 + [C.c_int32] * 3 + [C.c_void_p] * 12 + [C.c_void_p]),
     "mdn_qqrd2e": (C.c_double, []),
     "mdn_sync": (C.c_int, [C.c_void_p]),
     "mdn_last_kernel_ms": (C.c_int, [C.c_void_p, _dp]),

@@ -17,6 +17,7 @@
 // snapshots), so results do not depend on scheduling.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -272,6 +273,76 @@ int mdn_vi_adjoint(const double *x, const double *w, int32_t S, int32_t n_ion, i
       hipMemcpy(g_eps, ge, sizeof(double) * n_ion, hipMemcpyDeviceToHost) != hipSuccess ||
       hipMemcpy(g_q, gq, sizeof(double) * n_ion, hipMemcpyDeviceToHost) != hipSuccess)
     return set_error(MDN_E_HIP, "mdn_vi_adjoint: copy back failed");
+  return MDN_OK;
+}
+
+}  // extern "C"
+
+namespace mdn {
+
+// Scratch of mdn_vi_adjoint_noalloc: the per-(snapshot, ion atom) partial
+// gradients, 3 doubles each, in a fixed device array (4 MiB per GPU, held
+// until the process ends).  In host code `scratch` names a host-side
+// stand-in; its device address is taken once per device (scratch_on) and
+// only that address is passed to kernels.
+constexpr size_t kViScratchDoubles = (size_t)1 << 19;
+static __device__ __align__(256) double scratch[kViScratchDoubles];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static double *g_scratch[kMaxDevices] = {nullptr};
+
+// the current device's copy of `scratch` (its address resolved on the first
+// call on that device, with the device current)
+static int scratch_on(double **out) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return set_error(MDN_E_HIP, "mdn_vi_adjoint_noalloc: hipGetDevice failed");
+  if (dev < 0 || dev >= kMaxDevices) return set_error(MDN_E_ARG, "mdn_vi_adjoint_noalloc: device index too large");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch[dev]) {
+    void *p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return set_error(MDN_E_HIP, std::string("mdn_vi_adjoint_noalloc: hipGetSymbolAddress: ") + hipGetErrorString(e));
+    g_scratch[dev] = static_cast<double *>(p);
+  }
+  *out = g_scratch[dev];
+  return MDN_OK;
+}
+
+}  // namespace mdn
+
+extern "C" {
+
+int mdn_vi_adjoint_noalloc(const double *x, const double *w, int32_t S, int32_t n_ion, int32_t n_vap,
+                           const double *ion_sigma, const double *ion_eps, const double *ion_q,
+                           const double *vap_sigma, const double *vap_eps, const double *vap_q,
+                           const uint8_t *vap_in, const double *gE, const double *gF, double *g_sigma, double *g_eps,
+                           double *g_q, void *stream) {
+  int rc = check_vi(x, w, S, n_ion, n_vap, ion_sigma, ion_eps, ion_q, vap_sigma, vap_eps, vap_q);
+  if (rc) return rc;
+  if (!g_sigma || !g_eps || !g_q) return set_error(MDN_E_ARG, "mdn_vi_adjoint_noalloc: null output");
+  const size_t need = (size_t)S * (size_t)n_ion * 3;
+  if (need > kViScratchDoubles)
+    return set_error(MDN_E_ARG, "mdn_vi_adjoint_noalloc: S * n_ion = " + std::to_string((size_t)S * n_ion) +
+                                    " exceeds the scratch capacity of " + std::to_string(kViScratchDoubles / 3) +
+                                    " (snapshot, ion atom) pairs; use mdn_vi_adjoint");
+  double *part = nullptr;
+  if ((rc = scratch_on(&part))) return rc;
+  ViBatch b;
+  b.x = x; b.w = w; b.S = S; b.n_ion = n_ion; b.n_vap = n_vap;
+  b.sig_i = ion_sigma; b.eps_i = ion_eps; b.q_i = ion_q;
+  b.sig_v = vap_sigma; b.eps_v = vap_eps; b.q_v = vap_q;
+  b.vin = vap_in;
+  b.K = mdn_qqrd2e();
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  // every slot of the partials that k_vi_sum_g reads is written by k_vi_adj
+  // first, so the scratch needs no clearing between calls
+  const int ni = S * n_ion;
+  hipLaunchKernelGGL(k_vi_adj, dim3((ni + 255) / 256), dim3(256), 0, st, b, gE, gF, part);
+  hipLaunchKernelGGL(k_vi_sum_g, dim3((n_ion + 255) / 256), dim3(256), 0, st, part, S, n_ion, g_sigma, g_eps, g_q);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return set_error(MDN_E_HIP, std::string("mdn_vi_adjoint_noalloc: launch: ") + hipGetErrorString(e));
   return MDN_OK;
 }
 
