@@ -53,6 +53,9 @@ struct DevSpecies {
   // shake_gm[g].  Within a group the constraints keep their file order.
   int shake_ng, shake_gs[5];
   unsigned long long shake_gm[4];
+  // 1 when every group joins one centre atom to distinct satellites, at
+  // most three (X-H, X-H2, X-H3): SHAKE then iterates it in registers
+  int shake_star;
 };
 
 // mutable per-replica scalars

@@ -486,6 +486,7 @@ __device__ inline BatchLds<Rl> carve(char *base, int cap, int cap_h, int kcap, i
   B.shk.ij = ip; ip += 4 * (size_t)scap;
   B.shk.ng = ip; ip += 2;
   B.shk.gs = ip; ip += 10;
+  B.shk.star = ip; ip += 2;
   B.allow = reinterpret_cast<uint32_t *>(ip); ip += acap;
   int16_t *hp = reinterpret_cast<int16_t *>(ip);
   B.kidx = hp; hp += cap;
@@ -495,7 +496,7 @@ __device__ inline BatchLds<Rl> carve(char *base, int cap, int cap_h, int kcap, i
 static size_t batch_lds_bytes(int cap, int cap_h, int kcap, int acap, int scap, size_t rl) {
   return sizeof(double) * ((size_t)10 * cap + 3 * (size_t)cap_h + 2 * (size_t)kcap + (U_N + 6) * 16 + 16 +
                            8 * (size_t)scap + 2 + 8) +
-         sizeof(int) * (2 + 4 * (size_t)scap + 12) +
+         sizeof(int) * (2 + 4 * (size_t)scap + 14) +
          ((4 * rl * (size_t)kcap + sizeof(double) - 1) / sizeof(double)) * sizeof(double) +
          rl * 4 * (size_t)cap_h + sizeof(int) * (64 + (size_t)acap) + sizeof(int16_t) * (size_t)cap;
 }
@@ -1500,7 +1501,25 @@ static std::vector<int> shake_groups(const Species &S, DevSpecies *D) {
       }
   }
   gs[ng] = nr;
+  // star groups: one centre atom joined to distinct satellites, at most
+  // kStarMax constraints (in the group order of `out`)
+  bool star = nr > 0;
+  for (int g = 0; g < ng && star; g++) {
+    const int k0 = gs[g], k1 = gs[g + 1], n = k1 - k0;
+    if (n < 1 || n > kStarMax) { star = false; break; }
+    const int i0 = out[3 * k0], j0 = out[3 * k0 + 1];
+    const int c = n >= 2 && !(i0 == out[3 * k0 + 3] || i0 == out[3 * k0 + 4]) ? j0 : i0;
+    std::vector<int> sats;
+    for (int k = k0; k < k1; k++) {
+      const int i = out[3 * k], j = out[3 * k + 1];
+      if (i == j || (i != c && j != c)) { star = false; break; }
+      const int sk = i == c ? j : i;
+      if (std::find(sats.begin(), sats.end(), sk) != sats.end()) { star = false; break; }
+      sats.push_back(sk);
+    }
+  }
   if (D) {
+    D->shake_star = star ? 1 : 0;
     D->shake_ng = ng;
     for (int g = 0; g < 5; g++) D->shake_gs[g] = g <= ng ? gs[g] : nr;
     for (int g = 0; g < 4; g++) D->shake_gm[g] = gm[g];

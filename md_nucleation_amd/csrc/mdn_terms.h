@@ -647,8 +647,13 @@ struct ShakeLds {
   unsigned long long *cons;  // [2]
   int *ng, *gs;              // [2], [2][5]: constraint groups (DevSpecies)
   unsigned long long *gm;    // [2][4]
+  int *star;                 // [2]: every group a star of <= kStarMax constraints
   int cap;
 };
+
+// every constraint group of the species joins one centre atom to distinct
+// satellites, at most kStarMax of them (X-H, X-H2, X-H3): DevSpecies::shake_star
+constexpr int kStarMax = 3;
 
 __device__ inline void shake_lds_fill(const DevReplica &R, const DevSpecies &Sp, const ShakeLds &T, int s) {
   const int n = Sp.nrigid < T.cap ? Sp.nrigid : T.cap;
@@ -668,6 +673,7 @@ __device__ inline void shake_lds_fill(const DevReplica &R, const DevSpecies &Sp,
   T.n[s] = Sp.nrigid;
   T.cons[s] = cons;
   T.ng[s] = Sp.shake_ng;
+  T.star[s] = Sp.nrigid <= T.cap && Sp.shake_star ? 1 : 0;
   for (int g = 0; g < 5; g++) T.gs[5 * s + g] = Sp.shake_gs[g];
   for (int g = 0; g < 4; g++) T.gm[4 * s + g] = Sp.shake_gm[g];
 }
@@ -734,10 +740,92 @@ __device__ inline int quad_and(int v, int G) {
   if (G >= 4) v &= __builtin_amdgcn_update_dpp(v, v, 0x4E, 0xf, 0xf, false);  // lanes 2 3 0 1
   return v;
 }
+// The sweep of a star group (every constraint joins one centre atom to its
+// own satellite, up to kStarMax: X-H, X-H2, X-H3) with the group's atoms
+// held in registers for the whole iteration and written back once.  Each
+// constraint is taken as (centre, satellite) whichever order the file gives:
+// that only negates the old and new bond vectors, exactly, so every update
+// rounds as in the (i, j) order.  Sweep count (negative on failure) as
+// shake_group_lds.  For species whose groups are all stars (ShakeLds::star),
+// so that all lanes of a molecule take the same path.
+template <class XA, class RA>
+__device__ int shake_group_star(const DevReplica &R, const ShakeLds &T, int s, int k0, int k1, int G, XA x, RA r,
+                                int max_iter) {
+  const int n = k1 - k0;
+  const int *ij = T.ij + 2 * (size_t)s * T.cap;
+  const double *cc = T.c + 4 * (size_t)s * T.cap;
+  const int i0 = ij[2 * k0], j0 = ij[2 * k0 + 1];
+  const int c = n >= 2 && !(i0 == ij[2 * k0 + 2] || i0 == ij[2 * k0 + 3]) ? j0 : i0;
+  const double tolerance = 1e-20, rho_zero = 0.01;
+  // only the positions, which carry the chain from one update to the next,
+  // stay in registers; bond data and old positions are re-read from LDS
+  double xc[3], xs[kStarMax][3];
+  for (int q = 0; q < 3; q++) xc[q] = x(c, q);
+#pragma unroll
+  for (int k = 0; k < kStarMax; k++) {
+    const int kk = k0 + (k < n ? k : 0);
+    const int sk = ij[2 * kk] == c ? ij[2 * kk + 1] : ij[2 * kk];
+    for (int q = 0; q < 3; q++) xs[k][q] = x(sk, q);
+  }
+  int it = 0;
+  bool converge = false;
+  while (!converge) {
+    if (it >= max_iter) return -it;
+    it++;
+    int conv = 1, ok = 1;
+#pragma unroll
+    for (int k = 0; k < kStarMax; k++) {
+      if (k >= n) break;
+      const int kk = k0 + k;
+      const bool ci = ij[2 * kk] == c;
+      const int sk = ci ? ij[2 * kk + 1] : ij[2 * kk];
+      const double dk2 = cc[4 * kk], hm = cc[4 * kk + 3];
+      const double mc = ci ? cc[4 * kk + 1] : cc[4 * kk + 2], ms = ci ? cc[4 * kk + 2] : cc[4 * kk + 1];
+      double rk[3], rn[3];
+      for (int q = 0; q < 3; q++) { rk[q] = r(c, q) - r(sk, q); rn[q] = xc[q] - xs[k][q]; }
+      double rn2 = rn[0] * rn[0] + rn[1] * rn[1] + rn[2] * rn[2];
+      double dot = rk[0] * rn[0] + rk[1] * rn[1] + rk[2] * rn[2];
+      if (R.shake != 2) {
+        double d0 = rn2 - dk2;
+        if (d0 * d0 > tolerance) conv = 0;
+      }
+      double ramk = (dk2 - rn2) * hm;
+      if (dot > 1e-10) ramk *= rcp_nr(dot);
+      else {
+        double rk2 = rk[0] * rk[0] + rk[1] * rk[1] + rk[2] * rk[2];
+        ramk = 0.5 * rho_zero + ramk / (rk2 * rho_zero) * (mc + ms);
+      }
+      double cfc = ramk * mc, cfs = ramk * ms;
+      for (int q = 0; q < 3; q++) {
+        double nc = xc[q] + rk[q] * cfc, ns = xs[k][q] - rk[q] * cfs;
+        xc[q] = nc;
+        xs[k][q] = ns;
+        rn[q] = nc - ns;
+      }
+      rn2 = rn[0] * rn[0] + rn[1] * rn[1] + rn[2] * rn[2];
+      double dr = rn2 - dk2;
+      if (!(dr * dr <= 1e30)) { ok = 0; break; }
+      if (R.shake == 2 && dr * dr > tolerance) conv = 0;
+    }
+    if (!quad_and(ok, G)) return -it;
+    converge = quad_and(conv, G) != 0;
+  }
+  for (int q = 0; q < 3; q++) x(c, q) = xc[q];
+#pragma unroll
+  for (int k = 0; k < kStarMax; k++) {
+    if (k >= n) break;
+    const int kk = k0 + k;
+    const int sk = ij[2 * kk] == c ? ij[2 * kk + 1] : ij[2 * kk];
+    for (int q = 0; q < 3; q++) x(sk, q) = xs[k][q];
+  }
+  return it;
+}
+
 template <class XA, class RA>
 __device__ int shake_group_lds(const DevReplica &R, const ShakeLds &T, int s, int g, int G, XA x, RA r,
                                int max_iter) {
   const int k0 = T.gs[5 * s + g], k1 = T.gs[5 * s + g + 1];
+  if (T.star[s]) return shake_group_star(R, T, s, k0, k1, G, x, r, max_iter);
   const int *ij = T.ij + 2 * (size_t)s * T.cap;
   const double *cc = T.c + 4 * (size_t)s * T.cap;
   const double tolerance = 1e-20, rho_zero = 0.01;

@@ -6,11 +6,13 @@ mdn_run_batched's split launch (more replicas than CUs: the costliest in
 sharing CUs three or four at a time, on three streams, re-planned between
 chunks).
 
-  * 1000 FP64 steps in 250-step chunks (three re-plans; every replica
-    rebuilds inside the window, see the assertion) against the oracle run
-    from the same state: identical classification and rebuild count for
-    every one of the 512 replicas, in-domain positions within 1e-8 A,
-    forces within 1e-7 relative, energies within 1e-6;
+  * 900 FP64 steps in 250-step chunks (re-planned between chunks) against
+    the oracle for every one of the 512 replicas: over steps 600-900 from
+    the device's own state at step 600 (a rebuild inside that window for
+    almost every replica) identical classification and rebuild counts,
+    in-domain positions within 1e-8 A, forces within 1e-7 relative, energies
+    within 1e-6; over all 900 steps from the common start, identical
+    classification and rebuild counts with positions within 1e-6 A;
   * one mixed-precision step of the same launch against the oracle: F(x1)
     for identical x1 (x1 depends only on the starting state), relative RMS
     force error <= 1e-5 per replica and over the whole workload, and the
@@ -29,7 +31,6 @@ from tests import engine_util as E
 pytestmark = pytest.mark.gpu
 
 N_REP = 512
-STEPS = 1000
 CHUNK = 250
 EVERY = 100
 THREADS = max(1, min(16, len(os.sched_getaffinity(0))))
@@ -62,26 +63,12 @@ def _oracles(reps, nsteps, every):
     return orcs, Ss
 
 
-def test_split_launch_fp64_vs_oracle(workload):
-    reps = workload
-    eng = W.build_engine(reps)
-    eng.set_batch_shape(0, 0, 0, CHUNK)
-    print("case: split launch, %d replicas, %d FP64 steps in %d-step chunks" % (N_REP, STEPS, CHUNK), flush=True)
-    eng.run_batched(STEPS, eflag_every=EVERY, precision=M.FP64)
-    eng.sync()
-    plan, order = eng.batch_plan()
-    print("last chunk's plan:", plan, flush=True)
-    assert plan["own_cu"] > 0 and plan["three_per_cu"] > 0 and plan["four_per_cu"] > 0 and plan["uniform"] == 0
-    assert sorted(order.tolist()) == list(range(N_REP))
-    orcs, Ss = _oracles(reps, STEPS, EVERY)
+def _compare(reps, eng_states, Ss, tol_x, tol_f, label):
     worst = {}
     for r, (sd, _, _, kind) in enumerate(reps):
-        a, rb, _, nfail = eng.counters(r)  # raises if any status flag (capacity) is set
-        st = eng.get_state(r)
-        S = Ss[r]
-        assert st.c.rebuild_count == S.c.rebuild_count >= 2, r
-        assert rb == S.c.rebuild_count
-        np.testing.assert_array_equal(st.region, S.region, err_msg="replica %d" % r)
+        st, S = eng_states[r], Ss[r]
+        assert st.c.rebuild_count == S.c.rebuild_count, (label, r)
+        np.testing.assert_array_equal(st.region, S.region, err_msg="%s replica %d" % (label, r))
         m = E.in_atom_mask(sd, st.region)
         dx = float(np.abs(st.atom_x.reshape(-1, 3)[m] - S.atom_x.reshape(-1, 3)[m]).max())
         fe = E.rel(st.atom_f.reshape(-1, 3)[m], S.atom_f.reshape(-1, 3)[m])
@@ -90,13 +77,58 @@ def test_split_launch_fp64_vs_oracle(workload):
         w[0] += 1
         w[1] = max(w[1], dx)
         w[2] = max(w[2], fe)
-        assert dx < 1e-8, (r, k, dx)
-        assert fe < 1e-7, (r, k, fe)
-        np.testing.assert_allclose(st.U, S.U, rtol=1e-6, atol=1e-6, err_msg="replica %d" % r)
-        assert nfail == S.c.shake_failures == 0
+        assert dx < tol_x, (label, r, k, dx)
+        assert fe < tol_f, (label, r, k, fe)
+        np.testing.assert_allclose(st.U, S.U, rtol=1e-6, atol=1e-6, err_msg="%s replica %d" % (label, r))
     for k, (cnt, dx, fe) in sorted(worst.items()):
-        print("%-16s replicas %3d  max |dx| %.2e A  max rel F %.2e" % (k, cnt, dx, fe))
+        print("%s %-16s replicas %3d  max |dx| %.2e A  max rel F %.2e" % (label, k, cnt, dx, fe))
     assert len(worst) == 6
+
+
+def test_split_launch_fp64_vs_oracle(workload):
+    """600 + 300 FP64 steps of the split launch.  Window check: the oracle
+    restarts from the device's own state after step 600 and runs the next 300
+    steps, which hold a rebuild for almost every replica (their first
+    loop_update lies between 449 and 883 steps): in-domain positions within
+    1e-8 A, forces within 1e-7, energies within 1e-6.  Whole-run check: the
+    oracle from the common start over all 900 steps gives the same
+    classification and rebuild counts, positions within 1e-6 A (a rounding
+    difference grows about 1e4-fold over a thousand steps of these systems,
+    so a rounding-level difference of the two paths is not held to 1e-8 A
+    that far)."""
+    reps = workload
+    eng = W.build_engine(reps)
+    eng.set_batch_shape(0, 0, 0, CHUNK)
+    print("case: split launch, %d replicas, 600 + 300 FP64 steps in %d-step chunks" % (N_REP, CHUNK), flush=True)
+    eng.run_batched(600, eflag_every=EVERY, precision=M.FP64)
+    eng.sync()
+    mid = [eng.get_state(r, cap=20000) for r in range(N_REP)]
+    eng.run_batched(300, eflag_every=EVERY, step0=600, precision=M.FP64)
+    eng.sync()
+    plan, order = eng.batch_plan()
+    print("last chunk's plan:", plan, flush=True)
+    assert plan["own_cu"] > 0 and plan["three_per_cu"] > 0 and plan["four_per_cu"] > 0 and plan["uniform"] == 0
+    assert sorted(order.tolist()) == list(range(N_REP))
+    end = []
+    for r in range(N_REP):
+        a, rb, _, nfail = eng.counters(r)  # raises if any status flag (capacity) is set
+        assert nfail == 0
+        end.append(eng.get_state(r, cap=20000))
+    # window: the oracle from the device's state at step 600
+    orcs, Sw = [], []
+    for r, (sd, _, seed, _) in enumerate(reps):
+        orc = E.oracle_for(sd, seed=seed, replica=r)
+        orcs.append(orc)
+        Sw.append(orc.new_state(mid[r].as_dict(), cap=20000))
+    assert O.batch_verlet(orcs, Sw, 300, EVERY, THREADS)[0] == 0
+    in_window = sum(Sw[r].c.rebuild_count > mid[r].c.rebuild_count for r in range(N_REP))
+    print("replicas rebuilding inside the window: %d of %d" % (in_window, N_REP))
+    assert in_window >= 0.9 * N_REP
+    _compare(reps, end, Sw, 1e-8, 1e-7, "window 600-900")
+    # whole run from the common start
+    _, Ss = _oracles(reps, 900, EVERY)
+    assert all(S.c.rebuild_count >= 2 for S in Ss)
+    _compare(reps, end, Ss, 1e-6, 1e-5, "steps 0-900")
     eng.close()
 
 
