@@ -6,13 +6,18 @@ vapor molecules, 5,000 N2, 300 K, dt 1 fs), FP32 pair arithmetic with FP64
 accumulation and integration.  Beside it, on the same workload:
   * the plain FP64 per-call GPU path (one replica after another, one launch
     per reference call, no graph), timed on a sample of replicas;
-  * the CPU baseline: the reference's own stepping loop, built from its
-    sources (oracle/_ref), one single-threaded process per replica as the
-    reference's main runs one replica per OpenMP thread, on a bounded sample
-    (the FP64 oracle, a C restatement of the same loops, is timed beside it
-    and stands in when the reference build is absent);
-  * the relative RMS force error of the mixed path against the FP64 path on
-    identical states.
+  * the CPU baseline: the FP64 oracle (a C restatement of the reference's
+    loops, doing the same work as the GPU path, SHAKE included), one replica
+    per OpenMP thread as the reference's main runs them, on the host cores
+    given to this process, on a bounded sample; the reference's own stepping
+    loop (built from its sources into oracle/_ref, where present) and the
+    oracle without SHAKE on the same sample give the port/reference time
+    ratio;
+  * the force and energy error of the mixed path against the FP64 oracle on
+    the timed launch (every replica) and on configs 1-3;
+  * configs 1 and 2 (one large replica each): plain FP64 per-call GPU,
+    batched and CPU-oracle throughput;
+  * the 12,001-atom cluster and the differentiable vi term.
 
     python bench.py [--gpus 1] [--steps 10000] [--warmup 200]
 """
@@ -90,55 +95,99 @@ def total_atom_steps(eng, n):
     return sum(eng.counters(r)[0] for r in range(n))
 
 
-def _oracle_for(sd, seed):
-    """The FP64 oracle on the same parsed system (CPU-baseline leg only)."""
+def _oracle_for(sd, seed, replica=0, shake=None):
+    """The FP64 oracle on the same parsed system (the checker and the CPU
+    baseline; never the measured path)."""
     from oracle import oracle as O
     v = sd.view
     sysd = {k: v[k] for k in ("pair_coeff", "btype", "ctype", "dmulti", "dcoeff", "species", "n_gas", "n_vap",
                               "ion_type_base", "dt", "T", "d_size", "margin", "ML2", "m_gas_kg", "m_vap_kg",
                               "gas_spin", "flags", "efield", "shake", "shake_max_iter")}
-    sysd.update(seed=seed, replica=0)
+    sysd.update(seed=seed, replica=replica)
+    if shake is not None:
+        sysd["shake"] = shake
     return O.Oracle(sysd)
 
 
-def cpu_baseline(reps, seconds=12.0):
-    """FP64 oracle on the host cores: one replica per OpenMP thread."""
-    from oracle import oracle as O
-    threads = max(1, min(16, os.cpu_count() or 1))
-    sample = reps[:threads]
+def cpu_info():
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"model": model, "nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0))}
+
+
+def cpu_threads():
+    """The host threads this process may use: its CPU affinity, at most the
+    16 cores a one-GPU share of the machine holds."""
+    return max(1, min(16, len(os.sched_getaffinity(0))))
+
+
+def _oracle_sample(reps, k, shake=None):
     orcs, states = [], []
-    for sd, st, seed, _ in sample:
-        orc = _oracle_for(sd, seed)
-        S = orc.new_state(st.as_dict())
+    for r in range(k):
+        sd, st, seed, _ = reps[r]
+        orc = _oracle_for(sd, seed, replica=r, shake=shake)
+        S = orc.new_state(st.as_dict(), cap=20000)
         orc.make_pair(S)
         orcs.append(orc)
         states.append(S)
-    n_cal = 50
+    return orcs, states
+
+
+def _time_oracle(reps, k, seconds, shake=None, nsteps=None):
+    from oracle import oracle as O
+    orcs, states = _oracle_sample(reps, k, shake)
+    if nsteps is None:
+        n_cal = 50
+        t0 = time.perf_counter()
+        O.batch_verlet(orcs, states, n_cal, 0, k)
+        dt_cal = time.perf_counter() - t0
+        nsteps = int(max(50, min(20000, n_cal * seconds / max(dt_cal, 1e-6))))
+    secs = []
     t0 = time.perf_counter()
-    O.batch_verlet(orcs, states, n_cal, 0, threads)
-    dt_cal = time.perf_counter() - t0
-    nsteps = int(max(50, min(20000, n_cal * seconds / max(dt_cal, 1e-6))))
-    t0 = time.perf_counter()
-    rc, acc = O.batch_verlet(orcs, states, nsteps, 0, threads)
+    rc, acc = O.batch_verlet(orcs, states, nsteps, 0, k, seconds=secs)
     el = time.perf_counter() - t0
-    return {"value": sum(acc) / el, "unit": "atom-steps/s", "cores": threads, "kind": "port",
-            "sample": "%d replicas (first %d of the workload) x %d steps, FP64 oracle, %.1f s wall, OpenMP %d threads"
-                      % (len(sample), len(sample), nsteps, el, threads)}
+    # k cores kept busy (the 512-replica job over k cores): k x total
+    # atom-steps / total replica-seconds, the measure of the reference loop
+    return k * sum(acc) / sum(secs), nsteps, el
+
+
+def cpu_baseline(reps, seconds=12.0):
+    """The FP64 oracle on the host cores doing the GPU path's work (the same
+    replicas, SHAKE as the GPU runs it, their own rebuilds): one replica per
+    OpenMP thread, as the reference's main runs one replica per thread
+    (main.cpp:21-29)."""
+    k = cpu_threads()
+    val, nsteps, el = _time_oracle(reps, k, seconds)
+    info = cpu_info()
+    return {"value": val, "unit": "atom-steps/s", "cores": k, "kind": "port",
+            "cpu": info["model"], "nproc": info["nproc"], "affinity": info["affinity"],
+            "sample": "the first %d replicas of the workload x %d steps (their rebuilds, SHAKE on as on the GPU), "
+                      "FP64 oracle built with the reference's flags (-O3, OpenMP), one replica per thread, %.1f s "
+                      "wall; value = %d x total atom-steps / total replica-seconds (cores kept busy, as for the "
+                      "reference loop)" % (k, nsteps, el, k)}
 
 
 REF_HARNESS = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
 
 
-def cpu_baseline_reference(d, n_rep, steps=20000, procs=16):
+def cpu_reference_loop(d, n_rep, steps=20000):
     """The reference's own stepping loop (MD::verlet with its own rebuilds,
-    built from the reference sources by oracle/Makefile) on the first
+    built from the reference's sources by oracle/Makefile) on the first
     replicas of the workload: one single-threaded process per replica, as the
-    reference's main runs one replica per OpenMP thread.  Started before
-    anything touches the GPU.  None when the harness is not built."""
+    reference's main runs one replica per OpenMP thread.  The reference runs
+    no SHAKE (verlet.cpp:24).  Started before anything touches the GPU.
+    None when the build is absent."""
     import subprocess
     if not os.path.exists(REF_HARNESS):
         return None
-    k = min(procs, n_rep)
+    k = min(cpu_threads(), n_rep)
     env = dict(os.environ, OMP_NUM_THREADS="1")
     ps = [subprocess.Popen([REF_HARNESS, os.path.join(d, "c4_%d.cond" % r), "-", "time", str(steps)], cwd=d,
                            env=env, stdout=subprocess.PIPE, stderr=subprocess.DEVNULL, text=True) for r in range(k)]
@@ -151,12 +200,150 @@ def cpu_baseline_reference(d, n_rep, steps=20000, procs=16):
         acc += v["atom_steps"]
         secs += v["seconds"]
         wall = max(wall, v["seconds"])
-    # the whole mix on k cores kept busy: k x (atom-steps / process-seconds)
-    return {"value": k * acc / secs, "unit": "atom-steps/s", "cores": k, "kind": "reference",
-            "sample": "%d replicas (first %d of the workload) x %d steps each: the reference's own MD::verlet loop "
-                      "(its rebuilds; SHAKE is not called there), g++ -O3 -fopenmp build of its sources, one "
-                      "single-threaded process per replica at once, %.1f s longest loop; value = %d x total "
-                      "atom-steps / total process-seconds" % (k, k, steps, wall, k)}
+    return {"value": k * acc / secs, "unit": "atom-steps/s", "cores": k, "kind": "reference", "steps": steps,
+            "sample": "the first %d replicas x %d steps each: the reference's own MD::verlet loop (its rebuilds, no "
+                      "SHAKE), one single-threaded process per replica, %.1f s longest loop; value = %d x total "
+                      "atom-steps / total process-seconds" % (k, steps, wall, k)}
+
+
+def port_vs_reference(reps, ref):
+    """The oracle with SHAKE off (the reference loop's work) on the sample the
+    reference loop ran, for the port / reference time ratio on these cores."""
+    k = ref["cores"]
+    val, nsteps, el = _time_oracle(reps, k, 0.0, shake=0, nsteps=ref["steps"])
+    return {"value": val, "unit": "atom-steps/s", "cores": k, "steps": nsteps,
+            "port_over_reference_time": ref["value"] / val}
+
+
+def _in_mask(sd, region):
+    v = sd.view
+    na = [sp["mass"].size for sp in v["species"]]
+    return np.repeat(np.asarray(region, bool), [na[0]] + [na[1]] * v["n_gas"] + [na[2]] * v["n_vap"])
+
+
+class ErrAcc:
+    """Force and per-label energy error of the mixed path against the FP64
+    oracle over one or more replicas: relative RMS over in-domain atoms, max
+    abs, and per label the largest |dU| with the largest |U|."""
+
+    def __init__(self):
+        self.num = self.den = self.fmax = 0.0
+        self.du = np.zeros(8)
+        self.uref = np.zeros(8)
+        self.n = 0
+
+    def add(self, sd, st, S):
+        m = _in_mask(sd, S.region)
+        fa, fb = st.atom_f.reshape(-1, 3)[m], S.atom_f.reshape(-1, 3)[m]
+        self.num += float(((fa - fb) ** 2).sum())
+        self.den += float((fb ** 2).sum())
+        self.fmax = max(self.fmax, float(np.abs(fa - fb).max())) if fa.size else self.fmax
+        self.du = np.maximum(self.du, np.abs(st.U - S.U))
+        self.uref = np.maximum(self.uref, np.abs(S.U))
+        self.n += 1
+
+    def result(self, what):
+        from md_nucleation_amd import _lib as M
+        return {"rel_rms": (self.num / max(self.den, 1e-300)) ** 0.5, "max_abs": self.fmax, "unit": "kcal/mol/A",
+                "energy_max_abs_err": {lab: float(self.du[k]) for k, lab in enumerate(M.U_LABELS)},
+                "energy_max_abs": {lab: float(self.uref[k]) for k, lab in enumerate(M.U_LABELS)},
+                "replicas": self.n, "vs": what}
+
+
+def batched_force_error(reps):
+    """One mixed step of the timed launch shape (all replicas, split launch)
+    against the FP64 oracle from the same states: both hold F(x1) for the
+    same x1 (x1 depends only on the starting state), energies on that step."""
+    from md_nucleation_amd import _lib as M
+    from md_nucleation_amd import workloads as W
+    from oracle import oracle as O
+    eng = W.build_engine(reps)
+    eng.run_batched(1, eflag_every=1, precision=M.MIXED)
+    eng.sync()
+    plan, _ = eng.batch_plan()
+    orcs, states = _oracle_sample(reps, len(reps))
+    O.batch_verlet(orcs, states, 1, 1, cpu_threads())
+    acc = ErrAcc()
+    for r, (sd, _, _, _) in enumerate(reps):
+        acc.add(sd, eng.get_state(r), states[r])
+    eng.close()
+    out = acc.result("FP64 oracle, one step from identical states, every replica, the split launch")
+    out["launch"] = plan
+    return out
+
+
+def single_configs(d, with_cpu=True):
+    """Configs 1 and 2 (one replica each): the mixed force / energy error
+    against the FP64 oracle, and throughput of the plain FP64 per-call GPU
+    path, the batched path (mixed and FP64) and the FP64 oracle on one
+    thread.  Config 3 (the cluster) adds its error here; its throughput is
+    in cluster_bench."""
+    from md_nucleation_amd import _lib as M
+    from md_nucleation_amd import configs
+    cases = (("config1", "Na+ + 20 methanol + 10,000 N2 (SHAKE H)", lambda dd: configs.config1(dd), 2000, 2000, 300),
+             ("config2", "150-atom AMBER chain + 200 water + 100,000 N2 (SHAKE H)", lambda dd: configs.config2(dd),
+              1000, 1000, 100),
+             ("config3", "Na+ + 2,000 methanol cluster (12,001 atoms, all-pairs) + 100,000 Ar",
+              lambda dd: configs.config3(dd), 0, 0, 0))
+    out = {}
+    for name, what, fn, n_pc, n_b, n_cpu in cases:
+        dd = os.path.join(d, name)
+        os.makedirs(dd, exist_ok=True)
+        sd = M.SysDef(fn(dd))
+        seed = 17
+        st0 = sd.init_state(seed)
+        if name == "config3":
+            st0.atom_w[:] = 1.0
+        res = {"workload": what}
+        # error: one mixed step against the oracle
+        e = M.Engine(0)
+        e.add(sd, st0, seed=seed)
+        e.commit()
+        e.make_pair(0)
+        if name == "config3":
+            e.verlet(0, 1, eflag_every=1, precision=M.MIXED)
+        else:
+            e.run_batched(1, eflag_every=1, precision=M.MIXED)
+        e.sync()
+        orc = _oracle_for(sd, seed)
+        S = orc.new_state(st0.as_dict())
+        orc.make_pair(S)
+        orc.verlet(S, 1, eflag_every=1)
+        acc = ErrAcc()
+        acc.add(sd, e.get_state(0), S)
+        res["force_err"] = acc.result("FP64 oracle, one step from an identical state")
+        e.close()
+        if n_pc:
+            for label, path, prec, n in (("plain_fp64_gpu", "percall", M.FP64, n_pc),
+                                         ("batched_fp64", "batched", M.FP64, n_b),
+                                         ("batched_mixed", "batched", M.MIXED, n_b)):
+                e = M.Engine(0)
+                e.add(sd, st0, seed=seed)
+                e.commit()
+                e.make_pair(0)
+                run = (lambda k, p=prec: e.verlet(0, k, precision=p)) if path == "percall" else \
+                      (lambda k, p=prec: e.run_batched(k, precision=p))
+                run(20)
+                e.sync()
+                a0 = e.counters(0)[0]
+                t0 = time.perf_counter()
+                run(n)
+                e.sync()
+                el = time.perf_counter() - t0
+                res[label] = {"value": (e.counters(0)[0] - a0) / el, "unit": "atom-steps/s",
+                              "ms_per_step": el * 1e3 / n, "steps": n}
+                e.close()
+            if with_cpu:
+                S = orc.new_state(st0.as_dict())
+                orc.make_pair(S)
+                t0 = time.perf_counter()
+                rc, acc_steps = orc.verlet(S, n_cpu)
+                el = time.perf_counter() - t0
+                res["cpu_oracle"] = {"value": acc_steps / el, "unit": "atom-steps/s", "cores": 1, "kind": "port",
+                                     "sample": "%d steps, FP64 oracle, one thread (one replica)" % n_cpu}
+        out[name] = res
+        print("%s done" % name, file=sys.stderr, flush=True)
+    return out
 
 
 def copy_check(nbytes):
@@ -300,8 +487,10 @@ def main():
     ap.add_argument("--replicas", type=int, default=512)
     ap.add_argument("--plain-replicas", type=int, default=4)
     ap.add_argument("--plain-steps", type=int, default=300)
+    ap.add_argument("--tune-steps", type=int, default=100, help="steps per launch-shape trial (on a copy)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-configs", action="store_true", help="skip configs 1-3 (errors, single-replica rates)")
     ap.add_argument("--no-cluster", action="store_true", help="skip the 12,001-atom cluster measurement")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable vi term measurement")
     ap.add_argument("--copy-check", action="store_true",
@@ -318,43 +507,26 @@ def main():
     with tempfile.TemporaryDirectory() as d:
         t0 = time.perf_counter()
         reps = W.batched_replicas(d, n_rep=args.replicas)
-        cpu_ref = None if args.no_cpu else cpu_baseline_reference(d, len(reps))
-        eng = W.build_engine(reps)
-        print("workload ready: %d replicas in %.1f s" % (len(reps), time.perf_counter() - t0), file=sys.stderr,
-              flush=True)
+        # the reference's own loop runs in child processes started before
+        # anything touches the GPU
+        cpu_ref = None if args.no_cpu else cpu_reference_loop(d, len(reps))
         n = len(reps)
-        # mixed vs FP64 force error on identical states (one step from the
-        # same start: both hold F(x1) for the same x1)
-        k_err = min(n, 24)
-        e64 = W.build_engine(reps[:k_err])
-        e32 = W.build_engine(reps[:k_err])
-        e64.run_batched(1, precision=M.FP64)
-        e32.run_batched(1, precision=M.MIXED)
-        num = den = 0.0
-        fmax = 0.0
-        for r in range(k_err):
-            a, b = e32.get_state(r), e64.get_state(r)
-            m = np.repeat(b.region.astype(bool), [reps[r][0].view["species"][0]["mass"].size]
-                          + [reps[r][0].view["species"][1]["mass"].size] * reps[r][0].view["n_gas"]
-                          + [reps[r][0].view["species"][2]["mass"].size] * reps[r][0].view["n_vap"])
-            fa, fb = a.atom_f.reshape(-1, 3)[m], b.atom_f.reshape(-1, 3)[m]
-            num += float(((fa - fb) ** 2).sum())
-            den += float((fb ** 2).sum())
-            fmax = max(fmax, float(np.abs(fa - fb).max()))
-        force_err = (num / den) ** 0.5
-        e64.close()
-        e32.close()
-
-        # batched mixed precision: warmup (the launch-shape autotuner's trials
-        # are the warmup steps), then the timed region
+        # launch-shape autotuning on a copy of the workload: the timed engine
+        # itself takes exactly the warmup steps before the timed region
         tuned = None
-        if args.warmup:
+        if args.tune_steps > 0:
             from md_nucleation_amd import autotune
-            # one untimed run and one trial per shape, `per` steps each
-            per = max(10, args.warmup // (len(autotune.DEFAULT_SHAPES) + 1))
-            best, res = autotune.tune_batched(eng, n, steps=per, precision=M.MIXED)
+            et = W.build_engine(reps)
+            best, res = autotune.tune_batched(et, n, steps=args.tune_steps, precision=M.MIXED)
             tuned = {"shape": list(best), "trials": {str(list(k)): v for k, v in res.items()},
-                     "steps_per_trial": per}
+                     "steps_per_trial": args.tune_steps, "on": "a copy of the workload"}
+            et.close()
+        eng = W.build_engine(reps)
+        if tuned:
+            eng.set_batch_shape(*tuned["shape"])
+        print("workload ready: %d replicas in %.1f s" % (n, time.perf_counter() - t0), file=sys.stderr, flush=True)
+        if args.warmup:
+            eng.run_batched(args.warmup, eflag_every=0, precision=M.MIXED)
         eng.sync()
         a0 = total_atom_steps(eng, n)
         t0 = time.perf_counter()
@@ -362,8 +534,10 @@ def main():
         eng.sync()
         el = time.perf_counter() - t0
         kern_ms = eng.last_kernel_ms()
-        steps_per_launch = min(args.steps, tuned["shape"][3] if tuned and tuned["shape"][3] > 0 else args.steps)
+        chunk = tuned["shape"][3] if tuned and tuned["shape"][3] > 0 else args.steps
+        steps_per_launch = min(args.steps, chunk)
         lds_cap, lds_bytes, n_heavy = eng.batch_info()
+        plan, _ = eng.batch_plan()
         a1 = total_atom_steps(eng, n)
         value = (a1 - a0) / el
         flops = 0
@@ -371,6 +545,12 @@ def main():
             flops += step_flops(reps[r][0], eng.get_state(r))
         achieved = flops * args.steps / (kern_ms * 1e-3) / 1e12
         print("batched: %.3f s for %d steps, kernel %.3f ms" % (el, args.steps, kern_ms), file=sys.stderr, flush=True)
+        eng.close()
+
+        # mixed-precision force / energy error of the timed launch shape
+        # against the FP64 oracle, every replica
+        force_err = batched_force_error(reps)
+        print("force error done", file=sys.stderr, flush=True)
 
         # plain FP64 per-call path on a sample of the same replicas
         pr = min(args.plain_replicas, n)
@@ -386,8 +566,9 @@ def main():
         plain = (total_atom_steps(ep, pr) - p0) / pel
         ep.close()
 
-        cpu_port = None if args.no_cpu else cpu_baseline(reps, args.cpu_seconds)
-        cpu = cpu_ref or cpu_port
+        cpu = None if args.no_cpu else cpu_baseline(reps, args.cpu_seconds)
+        calib = port_vs_reference(reps, cpu_ref) if cpu_ref else None
+        cfgs = None if args.no_configs else single_configs(os.path.join(d, "cfg"), with_cpu=not args.no_cpu)
         cl = None if args.no_cluster else cluster_bench(os.path.join(d, "cluster"), with_cpu=not args.no_cpu)
         adj = None if args.no_adjoint else adjoint_bench(os.path.join(d, "fm"))
         if args.copy_check:
@@ -406,18 +587,20 @@ def main():
                            "sample": "%d replicas x %d steps, per-call kernels, one replica after another" %
                                      (pr, args.plain_steps)},
         "batched_vs_plain": value / plain,
-        "force_err": {"rel_rms": force_err, "max_abs": fmax, "vs": "FP64 path, identical states, %d replicas" % k_err},
+        "force_err": force_err,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS,
                      "traffic": HEAVY_HBM_BYTES_PER_STEP * steps_per_launch,
-                     "traffic_note": "HBM bytes per heavy launch (%d steps), counter passes in "
+                     "traffic_note": "HBM bytes per launch of the own-CU class (%d steps), counter passes in "
                                      "profiles/k_batched_mixed_512replicas.md" % steps_per_launch,
-                     "kernel": "k_batched<float>", "kernel_ms": kern_ms, "lds_bytes_per_wg": lds_bytes, "replicas_with_own_cu": n_heavy,
-                     "lds_cap_atoms": lds_cap,
+                     "kernel": "k_batched<float>", "kernel_ms": kern_ms, "lds_bytes_per_wg": lds_bytes,
+                     "replicas_with_own_cu": n_heavy, "lds_cap_atoms": lds_cap, "launch_plan": plan,
                      "flops_per_step": flops},
         "cpu_baseline": cpu,
-        "cpu_oracle": cpu_port if cpu_ref else None,
+        "cpu_reference_loop": cpu_ref,
+        "port_vs_reference": calib,
         "autotune": tuned,
+        "configs": cfgs,
         "cluster": cl,
         "adjoint": adj,
     }

@@ -162,6 +162,11 @@ int orc_verlet(orc_ctx *c, orc_state *st, int nsteps, int eflag_every,
                int64_t step0, int64_t *atom_steps);
 int orc_shake_molecule(orc_ctx *c, double *x, const double *old,
                        int species);
+/* one replica per OpenMP thread; seconds (may be NULL) gets each replica's
+ * loop time */
+int orc_verlet_batch(orc_ctx **ctx, orc_state **st, int n, int nsteps,
+                     int eflag_every, int64_t step0, int64_t *atom_steps,
+                     int nthreads, double *seconds);
 
 #ifdef __cplusplus
 }

@@ -84,7 +84,7 @@ def lib():
         L.orc_verlet.argtypes = [C.c_void_p, C.POINTER(OrcState), C.c_int, C.c_int, C.c_int64,
                                  C.POINTER(C.c_int64)]
         L.orc_verlet_batch.argtypes = [C.POINTER(C.c_void_p), C.POINTER(C.POINTER(OrcState)), C.c_int, C.c_int,
-                                       C.c_int, C.c_int64, C.POINTER(C.c_int64), C.c_int]
+                                       C.c_int, C.c_int64, C.POINTER(C.c_int64), C.c_int, C.POINTER(C.c_double)]
         L.orc_shake_molecule.argtypes = [C.c_void_p, _dp, _dp, C.c_int]
         L.orc_shake_molecule.restype = C.c_int
         L.orc_philox4x32_10.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
@@ -280,13 +280,17 @@ class OracleState:
         return d
 
 
-def batch_verlet(oracles, states, nsteps, eflag_every=0, nthreads=0):
-    """One replica per OpenMP thread (cpu baseline)."""
+def batch_verlet(oracles, states, nsteps, eflag_every=0, nthreads=0, seconds=None):
+    """One replica per OpenMP thread (cpu baseline).  With a list for
+    `seconds`, each replica's loop time is appended to it."""
     n = len(oracles)
     hs = (C.c_void_p * n)(*[o.h for o in oracles])
     ss = (C.POINTER(OrcState) * n)(*[C.pointer(s.c) for s in states])
     acc = (C.c_int64 * n)()
-    rc = lib().orc_verlet_batch(hs, ss, n, nsteps, eflag_every, 0, acc, nthreads)
+    sec = (C.c_double * n)()
+    rc = lib().orc_verlet_batch(hs, ss, n, nsteps, eflag_every, 0, acc, nthreads, sec)
+    if seconds is not None:
+        seconds.extend(sec[:])
     return rc, list(acc)
 
 
