@@ -393,7 +393,7 @@ def cluster_bench(d, steps=40, cpu_steps=3, with_cpu=True):
         t = float(np.median(ms)) * 1e-3
         peak = PEAK_FP32_TFLOPS if prec == M.MIXED else PEAK_FP64_TFLOPS
         ach = pairs * FLOP["ljcoul_w"] / t / 1e12
-        out[name]["vv_kernel"] = {"kernel": "k_vv_tiled<%s>" % ("float" if prec == M.MIXED else "double"),
+        out[name]["vv_kernel"] = {"kernel": "k_vv_n3<%s>" % ("float" if prec == M.MIXED else "double"),
                                   "unique_pairs": pairs, "ms": t * 1e3, "achieved_tflops": ach, "peak_tflops": peak,
                                   "frac": ach / peak}
         eng.close()

@@ -307,114 +307,227 @@ __global__ void k_terms(const DevReplica *__restrict__ Rp, uint32_t bits, int ef
   if (eflag) flush_energy(R, e);
 }
 
-// vv for large in-domain vapor sets (e.g. a 2,000-methanol cluster):
-// i-centric, j tiles staged through LDS, no atomics on j; grid.y splits the j
-// range so that a launch fills the chip.  Each unordered pair is evaluated
-// from both sides, so its energy is halved.  The partner loop has no
-// branches: pairs of one molecule (the atom itself included) take r^2 = 1
-// and weight 0, and a tile is padded to a multiple of four partners with
-// far-away, zero-weight, zero-charge entries, so four partners' LDS reads
-// issue together; a padding partner's energy is masked off.  Per type pair
-// the table holds c0, c1, c0 / 12 and c1 / 6 in pair precision (the same
-// values as dividing per pair).
+// every lane takes the value of lane (l + 1) mod 64 (DPP wave_rol:1; all 64
+// lanes must be active)
+__device__ inline double wave_rol1(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_mov_dpp((int)b, 0x134, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), 0x134, 0xf, 0xf, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
 template <typename Rl> struct Pack4;
 template <> struct Pack4<float> { typedef float4 T; };
 template <> struct Pack4<double> { typedef double4 T; };
 
-constexpr int kVVTile = 256;
+// vv (VaporVapor.cpp:5-45) for large in-domain vapor sets (e.g. a
+// 2,000-methanol cluster), each unordered pair evaluated once (Newton's
+// third law).  The in-domain vapor atoms are cut into 64-atom blocks and
+// those into super-blocks of four; a 256-thread workgroup takes one
+// super-block pair (SI <= SJ).  Its four waves own the four i-blocks of SI
+// (lane l of wave w: atom 256 SI + 64 w + l, held in registers) and visit
+// the j-blocks of SJ in the order w, w + 1, w + 2, w + 3 (mod 4), so that
+// at any stage the waves work on different j-blocks.  Within a block pair
+// lane l meets, at step k, partner slot (l + k) mod 64: F_i accumulates in
+// FP64 registers, F_j in FP64 registers that travel with their partner (a
+// wave rotate after every step hands lane l the accumulator of lane l + 1,
+// whose partner it meets next) and are added to the workgroup's LDS F_j
+// array when the block pair ends.  At the end every F_i and F_j reaches
+// atom_f by one FP64 atomic per component.  An off-diagonal block pair runs
+// steps k and k + 32 together (two chains), k = 0..31; a block with itself
+// runs k = 1..32, k = 32 on lanes 0..31 only.  In a diagonal super-block
+// pair (SI = SJ) wave w takes the block pairs (w, w), (w, w + 1 mod 4) and,
+// for w < 2, (w, w + 2), which covers each unordered pair once.  A vapor
+// molecule's atoms are contiguous and at most 64, so they lie in at most
+// two adjacent blocks: the same-molecule test is compiled only into block
+// pairs of adjacent or equal blocks.  Padding atoms past the vapor range sit
+// 1e6 A away at distinct places, with zero charge and weight: their forces
+// are exactly zero and their energies are masked.  Per type pair the table
+// holds c0, c1, c0 / 12 and c1 / 6 in pair precision.
+constexpr int kVVSuper = 256;
+
+// an atom's pair data: the i side carries qqrd2e q, the j side q
+template <typename Rl>
+struct alignas(16) VVp {
+  Rl q, w;
+  int t, m;  // type - first vapor type; in-domain molecule (-1: padding)
+};
+
+template <typename Rl>
+struct VVI {
+  double x0, x1, x2;
+  VVp<Rl> p;
+};
+
+template <typename Rl>
+struct VVLds {
+  double *x, *y, *z;         // [256] partner positions
+  VVp<Rl> *p;                // [256]
+  double *fx, *fy, *fz;      // [256] partner force sums
+  typename Pack4<Rl>::T *sc; // [ntv * ntv]
+};
+
+// atom a (< 256 nsb) of the in-domain vapor range, or padding past nva
+template <typename Rl>
+__device__ inline void vv_atom(const DevReplica &R, const LView &L, int nva, int tv0, int a, bool iside, double &x0,
+                               double &x1, double &x2, VVp<Rl> &p) {
+  if (a < nva) {
+    const DevSpecies &V = R.sp[2];
+    const int k = a / L.na_v, ii = a - k * L.na_v;
+    const size_t g = R.off_vap + (size_t)(R.in_vap[k] - 1 - R.n_gas) * L.na_v + ii;
+    x0 = R.atom_x[3 * g]; x1 = R.atom_x[3 * g + 1]; x2 = R.atom_x[3 * g + 2];
+    p.q = (Rl)(iside ? R.qqrd2e * V.charge[ii] : V.charge[ii]);
+    p.w = (Rl)R.atom_w[g];
+    p.t = V.type[ii] - tv0;
+    p.m = k;
+  } else {
+    x0 = 1e6 + 1e3 * (double)a; x1 = 1e6; x2 = 1e6;
+    p.q = 0; p.w = 0; p.t = 0; p.m = -1;
+  }
+}
+
+__device__ inline size_t vv_global(const DevReplica &R, const LView &L, int a) {
+  const int k = a / L.na_v, ii = a - k * L.na_v;
+  return R.off_vap + (size_t)(R.in_vap[k] - 1 - R.n_gas) * L.na_v + ii;
+}
+
+// one pair: returns s with the force on i = s (x_i - x_j), on j the
+// opposite; pass = 0 masks the pair
+template <typename Rl, bool EF, bool NEAR>
+__device__ inline double vv_n3_pair(const VVI<Rl> &c, const VVLds<Rl> &S, int j, int ntv, uint32_t pass, double &dx,
+                                    double &dy, double &dz, double &e) {
+  const VVp<Rl> pj = S.p[j];
+  dx = c.x0 - S.x[j];
+  dy = c.x1 - S.y[j];
+  dz = c.x2 - S.z[j];
+  const uint32_t on = NEAR ? pass & (pj.m != c.p.m ? 1u : 0u) : 1u;
+  const Rl r2 = (Rl)(dx * dx + dy * dy + dz * dz);
+  const Rl rsq = NEAR ? (on ? r2 : (Rl)1) : r2;
+  const typename Pack4<Rl>::T cc = S.sc[c.p.t * ntv + pj.t];
+  const Rl qq = c.p.q * pj.q, w = c.p.w * pj.w;
+  Rl r2inv, rinv;
+  inv_r<Rl>(rsq, r2inv, rinv);
+  const Rl r6inv = r2inv * r2inv * r2inv;
+  const Rl flj = r6inv * (cc.x * r6inv - cc.y);
+  const Rl fc = qq * rinv;
+  if (EF) {
+    const double ep = (double)(r6inv * (cc.z * r6inv - cc.w)) + (double)fc;
+    e += on && pj.m >= 0 && c.p.m >= 0 ? ep : 0.0;
+  }
+  return (double)((flj + fc) * r2inv * (NEAR ? (on ? w : (Rl)0) : w));
+}
+
+// i-block against a different j-block (slots jb..jb+63): steps k and
+// k + 32 together; F_j sums into the LDS array
+template <typename Rl, bool EF, bool NEAR>
+__device__ inline void vv_n3_off(const VVI<Rl> &c, const VVLds<Rl> &S, int jb, int lane, int ntv, double &fx,
+                                 double &fy, double &fz, double &e) {
+  double a0x = 0, a0y = 0, a0z = 0, a1x = 0, a1y = 0, a1z = 0;
+#pragma unroll 2
+  for (int t = 0; t < 32; t++) {
+    double dx, dy, dz, ex, ey, ez;
+    const double s = vv_n3_pair<Rl, EF, NEAR>(c, S, jb + ((lane + t) & 63), ntv, 1u, dx, dy, dz, e);
+    const double u = vv_n3_pair<Rl, EF, NEAR>(c, S, jb + ((lane + t + 32) & 63), ntv, 1u, ex, ey, ez, e);
+    fx = fma(s, dx, fx); fy = fma(s, dy, fy); fz = fma(s, dz, fz);
+    fx = fma(u, ex, fx); fy = fma(u, ey, fy); fz = fma(u, ez, fz);
+    a0x = wave_rol1(fma(-s, dx, a0x));
+    a0y = wave_rol1(fma(-s, dy, a0y));
+    a0z = wave_rol1(fma(-s, dz, a0z));
+    a1x = wave_rol1(fma(-u, ex, a1x));
+    a1y = wave_rol1(fma(-u, ey, a1y));
+    a1z = wave_rol1(fma(-u, ez, a1z));
+  }
+  // after 32 rotations lane l holds chain 0's sum of slot l + 32 and chain
+  // 1's of slot l
+  const int s0 = jb + ((lane + 32) & 63), s1 = jb + lane;
+  atomicAdd(&S.fx[s0], a0x); atomicAdd(&S.fy[s0], a0y); atomicAdd(&S.fz[s0], a0z);
+  atomicAdd(&S.fx[s1], a1x); atomicAdd(&S.fy[s1], a1y); atomicAdd(&S.fz[s1], a1z);
+}
+
+// a block with itself (its atoms also in slots jb..jb+63): k = 1..32
 template <typename Rl, bool EF>
-__global__ void __launch_bounds__(kVVTile) k_vv_tiled(const DevReplica *__restrict__ Rp, int tv0, int ntv) {
+__device__ inline void vv_n3_diag(const VVI<Rl> &c, const VVLds<Rl> &S, int jb, int lane, int ntv, double &fx,
+                                  double &fy, double &fz, double &e) {
+  double ax = 0, ay = 0, az = 0;
+#pragma unroll 2
+  for (int k = 1; k <= 32; k++) {
+    double dx, dy, dz;
+    const uint32_t pass = k < 32 || lane < 32 ? 1u : 0u;
+    const double s = vv_n3_pair<Rl, EF, true>(c, S, jb + ((lane + k) & 63), ntv, pass, dx, dy, dz, e);
+    fx = fma(s, dx, fx); fy = fma(s, dy, fy); fz = fma(s, dz, fz);
+    ax = wave_rol1(fma(-s, dx, ax));
+    ay = wave_rol1(fma(-s, dy, ay));
+    az = wave_rol1(fma(-s, dz, az));
+  }
+  // after 32 rotations lane l holds the sum of slot l + 33
+  const int s0 = jb + ((lane + 33) & 63);
+  atomicAdd(&S.fx[s0], ax); atomicAdd(&S.fy[s0], ay); atomicAdd(&S.fz[s0], az);
+}
+
+template <typename Rl, bool EF>
+__global__ void __launch_bounds__(kVVSuper) k_vv_n3(const DevReplica *__restrict__ Rp, int tv0, int ntv) {
   const DevReplica &R = *Rp;
-  __shared__ double sx[kVVTile * 3];
-  __shared__ Rl sq[kVVTile], sw[kVVTile];
-  __shared__ int st[kVVTile], sm[kVVTile];
+  __shared__ double sx[kVVSuper], sy[kVVSuper], sz[kVVSuper], sfx[kVVSuper], sfy[kVVSuper], sfz[kVVSuper];
+  __shared__ VVp<Rl> sp[kVVSuper];
   __shared__ typename Pack4<Rl>::T sc[64];
-  LView L = view_from_dyn(R);
+  const LView L = view_from_dyn(R);
   const int nva = L.nv * L.na_v;
-  // vapor-vapor coefficient table (vapor types are contiguous)
-  for (int k = threadIdx.x; k < ntv * ntv; k += blockDim.x) {
-    int a = k / ntv, b = k - a * ntv;
+  const int nsb = (nva + kVVSuper - 1) / kVVSuper;
+  // super-block pair of this workgroup: row-major over SI <= SJ
+  int t = blockIdx.x, SI = 0;
+  while (SI < nsb && t >= nsb - SI) { t -= nsb - SI; SI++; }
+  if (SI >= nsb) return;  // the grid is sized for every vapor molecule in the domain
+  const int SJ = SI + t;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  for (int k = tid; k < ntv * ntv; k += kVVSuper) {
+    const int a = k / ntv, b = k - a * ntv;
     const double *p = R.pair_coeff + 2 * ((size_t)(tv0 + a) * R.n_types + tv0 + b);
     const Rl c0 = (Rl)p[0], c1 = (Rl)p[1];
     sc[k] = {c0, c1, c0 / Rl(12), c1 / Rl(6)};
   }
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const bool act = i < nva;
-  double xi[3] = {0, 0, 0}, qi = 0, wi = 0;
-  int ti = 0, mi = -1, gi = 0;
-  const DevSpecies &V = R.sp[2];
-  if (act) {
-    int k = i / L.na_v, ii = i - k * L.na_v;
-    gi = R.off_vap + (R.in_vap[k] - 1 - R.n_gas) * L.na_v + ii;
-    xi[0] = R.atom_x[3 * (size_t)gi]; xi[1] = R.atom_x[3 * (size_t)gi + 1]; xi[2] = R.atom_x[3 * (size_t)gi + 2];
-    qi = R.qqrd2e * V.charge[ii];
-    wi = R.atom_w[gi];
-    ti = V.type[ii] - tv0;
-    mi = k;
+  {
+    VVp<Rl> p;
+    double x0, x1, x2;
+    vv_atom<Rl>(R, L, nva, tv0, kVVSuper * SJ + tid, false, x0, x1, x2, p);
+    sx[tid] = x0; sy[tid] = x1; sz[tid] = x2; sp[tid] = p;
+    sfx[tid] = 0; sfy[tid] = 0; sfz[tid] = 0;
   }
-  const int trow = ti * ntv;
-  // charge (times qqrd2e) and weight products in pair precision
-  const Rl qr = (Rl)qi, wr = (Rl)wi;
-  int per = (nva + gridDim.y - 1) / gridDim.y;
-  int j0 = blockIdx.y * per, j1 = min(nva, j0 + per);
+  VVI<Rl> c;
+  const int ia = kVVSuper * SI + 64 * w + lane;
+  vv_atom<Rl>(R, L, nva, tv0, ia, true, c.x0, c.x1, c.x2, c.p);
+  __syncthreads();
+  const VVLds<Rl> S{sx, sy, sz, sp, sfx, sfy, sfz, sc};
   double fx = 0, fy = 0, fz = 0, e = 0;
-  // one partner b; valid: b is a real entry of the tile (uniform)
-  auto pair = [&](int b, bool valid) {
-    const double dx = xi[0] - sx[3 * b], dy = xi[1] - sx[3 * b + 1], dz = xi[2] - sx[3 * b + 2];
-    const bool on = sm[b] != mi;
-    const Rl r2 = (Rl)(dx * dx + dy * dy + dz * dz);
-    const typename Pack4<Rl>::T cc = sc[trow + st[b]];
-    const Rl qq = qr * sq[b], w = wr * sw[b];
-    Rl r2inv, rinv;
-    inv_r<Rl>(on ? r2 : (Rl)1, r2inv, rinv);
-    const Rl r6inv = r2inv * r2inv * r2inv;
-    const Rl flj = r6inv * (cc.x * r6inv - cc.y);
-    const Rl fc = qq * rinv;
-    if (EF) {
-      const double ep = (double)(r6inv * (cc.z * r6inv - cc.w)) + (double)fc;
-      e += on && valid ? ep : 0.0;
-    }
-    // the pair-precision scalar times the FP64 displacement
-    const double fpd = (double)((flj + fc) * r2inv * (on ? w : (Rl)0));
-    fx = fma(fpd, dx, fx); fy = fma(fpd, dy, fy); fz = fma(fpd, dz, fz);
-  };
-  for (int jt = j0; jt < j1; jt += kVVTile) {
-    __syncthreads();
-    const int nj = min(kVVTile, j1 - jt), nj4 = (nj + 3) & ~3;
-    const int b = threadIdx.x;
-    if (b < nj) {
-      const int j = jt + b;
-      int k = j / L.na_v, jj = j - k * L.na_v;
-      int g = R.off_vap + (R.in_vap[k] - 1 - R.n_gas) * L.na_v + jj;
-      sx[3 * b] = R.atom_x[3 * (size_t)g];
-      sx[3 * b + 1] = R.atom_x[3 * (size_t)g + 1];
-      sx[3 * b + 2] = R.atom_x[3 * (size_t)g + 2];
-      sq[b] = (Rl)V.charge[jj];
-      sw[b] = (Rl)R.atom_w[g];
-      st[b] = V.type[jj] - tv0;
-      sm[b] = k;
-    } else if (b < nj4) {
-      sx[3 * b] = 1e6; sx[3 * b + 1] = 1e6; sx[3 * b + 2] = 1e6;
-      sq[b] = 0; sw[b] = 0; st[b] = 0; sm[b] = -2;
-    }
-    __syncthreads();
-    if (act) {
-      for (int b4 = 0; b4 < nj4; b4 += 4) {
-        pair(b4, true);
-        pair(b4 + 1, b4 + 1 < nj);
-        pair(b4 + 2, b4 + 2 < nj);
-        pair(b4 + 3, b4 + 3 < nj);
-      }
+  if (SI == SJ) {
+    vv_n3_diag<Rl, EF>(c, S, 64 * w, lane, ntv, fx, fy, fz, e);
+    vv_n3_off<Rl, EF, true>(c, S, 64 * ((w + 1) & 3), lane, ntv, fx, fy, fz, e);
+    if (w < 2) vv_n3_off<Rl, EF, true>(c, S, 64 * (w + 2), lane, ntv, fx, fy, fz, e);
+  } else {
+    for (int s = 0; s < 4; s++) {
+      const int jw = (w + s) & 3;
+      // blocks 4 SI + w and 4 SJ + jw are adjacent only for SJ = SI + 1,
+      // w = 3, jw = 0
+      if (SJ == SI + 1 && w == 3 && jw == 0) vv_n3_off<Rl, EF, true>(c, S, 64 * jw, lane, ntv, fx, fy, fz, e);
+      else vv_n3_off<Rl, EF, false>(c, S, 64 * jw, lane, ntv, fx, fy, fz, e);
     }
   }
-  if (act) {
-    atomicAdd(&R.atom_f[3 * (size_t)gi], fx);
-    atomicAdd(&R.atom_f[3 * (size_t)gi + 1], fy);
-    atomicAdd(&R.atom_f[3 * (size_t)gi + 2], fz);
+  if (ia < nva) {
+    const size_t g = vv_global(R, L, ia);
+    atomicAdd(&R.atom_f[3 * g], fx);
+    atomicAdd(&R.atom_f[3 * g + 1], fy);
+    atomicAdd(&R.atom_f[3 * g + 2], fz);
+  }
+  __syncthreads();
+  const int ja = kVVSuper * SJ + tid;
+  if (ja < nva) {
+    const size_t g = vv_global(R, L, ja);
+    atomicAdd(&R.atom_f[3 * g], sfx[tid]);
+    atomicAdd(&R.atom_f[3 * g + 1], sfy[tid]);
+    atomicAdd(&R.atom_f[3 * g + 2], sfz[tid]);
   }
   if (EF) {
-    double s = wave_sum(0.5 * e);
-    if ((threadIdx.x & 63) == 0 && s != 0.0) atomicAdd(&R.dyn->U[U_VV], s);
+    const double s = wave_sum(e);
+    if (lane == 0 && s != 0.0) atomicAdd(&R.dyn->U[U_VV], s);
   }
 }
 
@@ -424,14 +537,6 @@ __global__ void __launch_bounds__(kVVTile) k_vv_tiled(const DevReplica *__restri
 // positions only for the ion + vapor atoms (the only constrained ones).
 // per-atom pair parameters packed for one LDS read: charge, sqrt(eps),
 // sigma, local vapor molecule id (-1: ion or gas) as a value
-// every lane takes the value of lane (l + 1) mod 64 (DPP wave_rol:1; all 64
-// lanes must be active)
-__device__ inline double wave_rol1(double v) {
-  const long long b = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_mov_dpp((int)b, 0x134, 0xf, 0xf, false);
-  const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), 0x134, 0xf, 0xf, false);
-  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
-}
 
 
 // per-atom LDS record: FP64 position and the pair-precision blend weight in
@@ -1718,7 +1823,7 @@ static int grid_for(int n, int blk = kBlk, int cap = 2048) {
 static int com_blocks(int natoms) { return std::max(1, std::min(32, (natoms + 1023) / 1024)); }
 
 // every selected potential of one replica through the per-call kernels; vv
-// goes to the tiled all-pairs kernel when the vapor's types fit its table
+// goes to the Newton-3 all-pairs kernel when the vapor's types fit its table
 template <typename Rl>
 static int launch_terms(mdn_ctx *c, const HostReplica &h, const DevReplica *R, uint32_t bits, bool eflag) {
   const SysDef &d = h.def;
@@ -1728,11 +1833,12 @@ static int launch_terms(mdn_ctx *c, const HostReplica &h, const DevReplica *R, u
   uint32_t rest = tiled ? bits & ~(uint32_t)MDN_VV : bits;
   if (rest) hipLaunchKernelGGL(k_terms<Rl>, dim3(512), dim3(kBlk), 0, c->stream, R, rest, eflag ? 1 : 0);
   if ((bits & MDN_VV) && tiled) {
-    int nva = d.n_vap * d.sp[2].natoms;  // bound on the in-domain vapor atoms
-    int gx = std::max(1, (nva + kVVTile - 1) / kVVTile);
-    int gy = std::max(1, std::min(64, (4 * c->n_cu + gx - 1) / gx));
-    if (eflag) hipLaunchKernelGGL((k_vv_tiled<Rl, true>), dim3(gx, gy), dim3(kVVTile), 0, c->stream, R, tv0, ntv);
-    else hipLaunchKernelGGL((k_vv_tiled<Rl, false>), dim3(gx, gy), dim3(kVVTile), 0, c->stream, R, tv0, ntv);
+    // one workgroup per super-block pair, for the largest possible
+    // in-domain vapor set
+    const int nsb = std::max(1, (d.n_vap * d.sp[2].natoms + kVVSuper - 1) / kVVSuper);
+    const int gx = nsb * (nsb + 1) / 2;
+    if (eflag) hipLaunchKernelGGL((k_vv_n3<Rl, true>), dim3(gx), dim3(kVVSuper), 0, c->stream, R, tv0, ntv);
+    else hipLaunchKernelGGL((k_vv_n3<Rl, false>), dim3(gx), dim3(kVVSuper), 0, c->stream, R, tv0, ntv);
   }
   HIPCHK(hipGetLastError());
   return MDN_OK;
