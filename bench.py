@@ -250,14 +250,29 @@ class ErrAcc:
                 "replicas": self.n, "vs": what}
 
 
-def batched_force_error(reps):
-    """One mixed step of the timed launch shape (all replicas, split launch)
-    against the FP64 oracle from the same states: both hold F(x1) for the
-    same x1 (x1 depends only on the starting state), energies on that step."""
+def _class_shapes(cfg, plan):
+    """the chosen launch shape per class of the split launch"""
+    if plan["uniform"]:
+        return {"uniform": {"replicas": plan["uniform"], "threads": cfg["threads"]}}
+    return {"own_cu": {"replicas": plan["own_cu"], "threads": cfg["own_threads"]},
+            "three_per_cu": {"replicas": plan["three_per_cu"], "threads": cfg["shared_threads"]},
+            "four_per_cu": {"replicas": plan["four_per_cu"], "threads": cfg["shared_threads"]},
+            "dense_run": cfg["dense_run"] or "per replica", "heavy_factor": cfg["heavy_factor"],
+            "chunk": cfg["chunk"]}
+
+
+def batched_force_error(reps, cfg=None):
+    """One mixed step of the timed launch shape (all replicas, split launch,
+    the tuned configuration) against the FP64 oracle from the same states:
+    both hold F(x1) for the same x1 (x1 depends only on the starting state),
+    energies on that step."""
     from md_nucleation_amd import _lib as M
+    from md_nucleation_amd import autotune
     from md_nucleation_amd import workloads as W
     from oracle import oracle as O
     eng = W.build_engine(reps)
+    if cfg:
+        autotune.apply(eng, cfg)
     eng.run_batched(1, eflag_every=1, precision=M.MIXED)
     eng.sync()
     plan, _ = eng.batch_plan()
@@ -487,7 +502,7 @@ def main():
     ap.add_argument("--replicas", type=int, default=512)
     ap.add_argument("--plain-replicas", type=int, default=4)
     ap.add_argument("--plain-steps", type=int, default=300)
-    ap.add_argument("--tune-steps", type=int, default=100, help="steps per launch-shape trial (on a copy)")
+    ap.add_argument("--tune-steps", type=int, default=1000, help="steps per launch-shape trial (on a copy)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-configs", action="store_true", help="skip configs 1-3 (errors, single-replica rates)")
@@ -513,17 +528,23 @@ def main():
         n = len(reps)
         # launch-shape autotuning on a copy of the workload: the timed engine
         # itself takes exactly the warmup steps before the timed region
+        from md_nucleation_amd import autotune
         tuned = None
+        cfg = dict(autotune.DEFAULT)
         if args.tune_steps > 0:
-            from md_nucleation_amd import autotune
             et = W.build_engine(reps)
-            best, res = autotune.tune_batched(et, n, steps=args.tune_steps, precision=M.MIXED)
-            tuned = {"shape": list(best), "trials": {str(list(k)): v for k, v in res.items()},
-                     "steps_per_trial": args.tune_steps, "on": "a copy of the workload"}
+            tt = time.perf_counter()
+            cfg, trials, tplan = autotune.tune_batched(et, n, steps=args.tune_steps, precision=M.MIXED)
+            tuned = {"config": cfg, "classes": _class_shapes(cfg, tplan),
+                     "trials": [[{k: v for k, v in c.items() if c[k] != autotune.DEFAULT[k]}, round(r / 1e9, 4)]
+                                for c, r in trials],
+                     "trial_unit": "1e9 atom-steps per kernel-second (changes from the default shown)",
+                     "steps_per_trial": args.tune_steps, "seconds": time.perf_counter() - tt,
+                     "on": "a copy of the workload"}
             et.close()
+            print("autotune: %s" % cfg, file=sys.stderr, flush=True)
         eng = W.build_engine(reps)
-        if tuned:
-            eng.set_batch_shape(*tuned["shape"])
+        autotune.apply(eng, cfg)
         print("workload ready: %d replicas in %.1f s" % (n, time.perf_counter() - t0), file=sys.stderr, flush=True)
         if args.warmup:
             eng.run_batched(args.warmup, eflag_every=0, precision=M.MIXED)
@@ -534,7 +555,7 @@ def main():
         eng.sync()
         el = time.perf_counter() - t0
         kern_ms = eng.last_kernel_ms()
-        chunk = tuned["shape"][3] if tuned and tuned["shape"][3] > 0 else args.steps
+        chunk = cfg["chunk"] if cfg["chunk"] > 0 else args.steps
         steps_per_launch = min(args.steps, chunk)
         lds_cap, lds_bytes, n_heavy = eng.batch_info()
         plan, _ = eng.batch_plan()
@@ -549,7 +570,7 @@ def main():
 
         # mixed-precision force / energy error of the timed launch shape
         # against the FP64 oracle, every replica
-        force_err = batched_force_error(reps)
+        force_err = batched_force_error(reps, cfg)
         print("force error done", file=sys.stderr, flush=True)
 
         # plain FP64 per-call path on a sample of the same replicas

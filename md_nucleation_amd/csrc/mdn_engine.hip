@@ -563,6 +563,7 @@ struct BatchLds {
   double *kms;      // [kcap] species-atom masses
   ShakeLds shk;     // constraint tables of the ion and the vapor [2][scap]
   int *sint;        // [64]
+  int runl;         // dense work-item run length (8, 16, 32; 0: chosen per replica)
 };
 
 template <typename Rl>
@@ -850,6 +851,7 @@ __device__ inline int dense_n3(const DevReplica &R, const BatchLds<Rl> &B, int n
     const int items = n_diag * (32 / l) + n_off * (64 / l) + n_bc, c = (items + nw - 1) / nw * (l + 6);
     if (c < best) { best = c; L = l; }
   }
+  if (B.runl) L = B.runl;
   const int W = __builtin_amdgcn_readfirstlane(R.amber_words);
   double *__restrict__ pf = B.f;
   double e_ion = 0, e_vi = 0, e_vv = 0;
@@ -1129,9 +1131,10 @@ template <typename Rl>
 __global__ void __launch_bounds__(512) k_batched(const DevReplica *__restrict__ reps,
                                                  const int *__restrict__ order, int n_rep, int nsteps,
                                                  int eflag_every, long long step0, int cap, int cap_h, int kcap,
-                                                 int acap, int scap) {
+                                                 int acap, int scap, int runl) {
   extern __shared__ __align__(16) char smem[];
   BatchLds<Rl> B = carve<Rl>(smem, cap, cap_h, kcap, acap, scap);
+  B.runl = runl;
   const int tid = threadIdx.x, nt = blockDim.x;
   for (int rr = blockIdx.x; rr < n_rep; rr += gridDim.x) {
     const DevReplica &R = reps[order[rr]];
@@ -1466,6 +1469,11 @@ struct mdn_ctx {
   int batch_cap = 0, batch_threads = 256, batch_grid = 0;
   mdn::Dyn *d_dyn = nullptr;
   int last_cap = 0, last_lds = 0, batch_chunk = 1000, batch_split = 1, n_cu = 256;
+  // split-launch tuning: workgroup sizes of the own-CU and shared-CU
+  // classes, the dense run length (0: per replica), and the cost (a multiple
+  // of the median) above which a replica may get a CU of its own
+  int own_threads = 512, shared_threads = 128, dense_runl = 0;
+  double heavy_factor = 2.0;
   hipStream_t stream2 = nullptr, stream3 = nullptr;
   hipEvent_t evj = nullptr, evk = nullptr, evl = nullptr;
   int *d_order = nullptr;
@@ -2461,7 +2469,7 @@ static int launch_batched(mdn_ctx *c, hipStream_t st, const int *order, int n, i
   HIPCHK(hipFuncSetAttribute((const void *)k_batched<Rl>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
   int grid = c->batch_grid > 0 ? std::min(c->batch_grid, n) : n;
   hipLaunchKernelGGL(k_batched<Rl>, dim3(grid), dim3(threads), b.lds, st, c->d_reps, order, n, ns, eflag_every,
-                     step0, b.cap, b.cap_h, b.kcap, b.acap, b.scap);
+                     step0, b.cap, b.cap_h, b.kcap, b.acap, b.scap, c->dense_runl);
   HIPCHK(hipGetLastError());
   return MDN_OK;
 }
@@ -2534,15 +2542,17 @@ int mdn_run_batched(mdn_ctx *c, int32_t nsteps, int32_t eflag_every, int64_t ste
       }
       double med = cost[order[n / 2]];
       int Hmax = 0;
-      while (Hmax < n && Hmax < c->n_cu && cost[order[Hmax]] > 2.0 * med) Hmax++;
+      while (Hmax < n && Hmax < c->n_cu && cost[order[Hmax]] > c->heavy_factor * med) Hmax++;
       const int nx = xcds(c->n_cu), per = c->n_cu / nx;
       auto up = [&](int k) { return (k + nx - 1) / nx; };
       // workgroups per CU of a class: its LDS (the largest need of its
-      // replicas, at the class budget) and 8 waves per CU at 256 VGPRs
+      // replicas, at the class budget), 8 waves per CU at 256 VGPRs, at most
+      // four
+      const int wmax = std::min(4, 8 / (c->shared_threads / 64));
       auto per_cu = [&](int cnt, const CapNeed &q, size_t budget) {
-        if (cnt == 0) return 4;
+        if (cnt == 0) return wmax;
         const size_t l = (caps_of(c, q, rl, budget).lds + 1023) / 1024 * 1024;
-        return (int)std::min<size_t>(4, (160 * 1024) / l);
+        return (int)std::min<size_t>(wmax, (160 * 1024) / l);
       };
       // CUs of the fullest XCD
       auto cus = [&](int nh, int na, const CapNeed &qa, int nb, const CapNeed &qb) {
@@ -2619,9 +2629,9 @@ int mdn_run_batched(mdn_ctx *c, int32_t nsteps, int32_t eflag_every, int64_t ste
       size_t lds = 0;
       int cap = 0;
       struct Part { hipStream_t st; int off, cnt, threads; size_t budget; const std::vector<int> *set; };
-      const Part parts[3] = {{c->stream, 0, H, 512, 160 * 1024, &heavy},
-                             {c->stream2, H, NB_, 128, kB, &clsB},
-                             {c->stream3, H + NB_, NA, 128, kA, &clsA}};
+      const Part parts[3] = {{c->stream, 0, H, c->own_threads, 160 * 1024, &heavy},
+                             {c->stream2, H, NB_, c->shared_threads, kB, &clsB},
+                             {c->stream3, H + NB_, NA, c->shared_threads, kA, &clsA}};
       for (const Part &pt : parts) {
         if (pt.cnt == 0) continue;
         BatchCaps bc = batch_caps(c, dyn, *pt.set, rl, pt.budget);
@@ -2674,6 +2684,24 @@ int mdn_set_batch_shape(mdn_ctx *c, int32_t threads, int32_t lds_cap_atoms, int3
   c->batch_cap = lds_cap_atoms;
   c->batch_grid = grid;
   c->batch_chunk = chunk;
+  return MDN_OK;
+}
+
+int mdn_set_batch_tuning(mdn_ctx *c, int32_t own_threads, int32_t shared_threads, int32_t dense_run,
+                         double heavy_factor) {
+  if (!c) return set_error(MDN_E_ARG, "null context");
+  if (own_threads != 256 && own_threads != 512)
+    return set_error(MDN_E_ARG, "mdn_set_batch_tuning: own_threads must be 256 or 512");
+  if (shared_threads != 64 && shared_threads != 128 && shared_threads != 256)
+    return set_error(MDN_E_ARG, "mdn_set_batch_tuning: shared_threads must be 64, 128 or 256");
+  if (dense_run != 0 && dense_run != 8 && dense_run != 16 && dense_run != 32)
+    return set_error(MDN_E_ARG, "mdn_set_batch_tuning: dense_run must be 0, 8, 16 or 32");
+  if (!(heavy_factor >= 1.0 && heavy_factor <= 100.0))
+    return set_error(MDN_E_ARG, "mdn_set_batch_tuning: heavy_factor must lie in [1, 100]");
+  c->own_threads = own_threads;
+  c->shared_threads = shared_threads;
+  c->dense_runl = dense_run;
+  c->heavy_factor = heavy_factor;
   return MDN_OK;
 }
 

@@ -16,7 +16,10 @@ chunks).
   * one mixed-precision step of the same launch against the oracle: F(x1)
     for identical x1 (x1 depends only on the starting state), relative RMS
     force error <= 1e-5 per replica and over the whole workload, and the
-    per-label energy error.
+    per-label energy error;
+  * every value of each autotuner knob (workgroup sizes of the classes, dense
+    run length, own-CU threshold, chunk): 300 FP64 steps that agree with the
+    default launch's to rounding (positions within 1e-9 A).
 """
 import os
 
@@ -169,3 +172,41 @@ def test_split_launch_mixed_force_error_vs_oracle(workload):
     assert total < 1e-5
     assert np.all(du <= 1e-4 * np.maximum(uref, 1.0))
     eng.close()
+
+
+KNOB_VALUES = [
+    {"own_threads": 256}, {"shared_threads": 64}, {"shared_threads": 256}, {"dense_run": 8},
+    {"dense_run": 16}, {"dense_run": 32}, {"heavy_factor": 1.5}, {"heavy_factor": 3.0}, {"chunk": 100},
+]
+
+
+def test_tuning_knobs_same_trajectory(workload):
+    """The autotuner's knobs change only the launch shape and the order in
+    which forces accumulate: 300 FP64 steps (a rebuild for about half the
+    replicas) under each knob value agree with the default launch's."""
+    from md_nucleation_amd import autotune
+    reps = workload
+
+    def run(cfg):
+        eng = W.build_engine(reps)
+        autotune.apply(eng, cfg)
+        eng.run_batched(300, eflag_every=EVERY, precision=M.FP64)
+        eng.sync()
+        plan, _ = eng.batch_plan()
+        out = [eng.get_state(r, cap=20000) for r in range(N_REP)]
+        eng.close()
+        return plan, out
+
+    base_plan, base = run(dict(autotune.DEFAULT))
+    for change in KNOB_VALUES:
+        cfg = dict(autotune.DEFAULT)
+        cfg.update(change)
+        print("case: knob %s" % change, flush=True)
+        plan, got = run(cfg)
+        worst = 0.0
+        for r in range(N_REP):
+            assert got[r].c.rebuild_count == base[r].c.rebuild_count, (change, r)
+            np.testing.assert_array_equal(got[r].region, base[r].region)
+            worst = max(worst, float(np.abs(got[r].atom_x - base[r].atom_x).max()))
+        print("knob %s: plan %s, max |dx| vs default %.2e A" % (change, plan, worst))
+        assert worst < 1e-9
