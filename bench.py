@@ -484,12 +484,21 @@ def segment_bench(d, nsteps=100, every=10):
         obs = set(range(every, nsteps + 1, every))
         tgt = {n: xs[n] for n in obs}
         seg.n_force_calls = 0
+        seg.n_hvp_calls = 0
         t0 = time.perf_counter()
         SG.loss_and_grad(seg, theta, x0, v0, nsteps, tgt, obs, every=every)
         dt = time.perf_counter() - t0
+        nf, nh = seg.n_force_calls, seg.n_hvp_calls
+        # one exact Hessian-vector product alone (host copies included)
+        u = rng.normal(0, 1, x0.shape)
+        t1 = time.perf_counter()
+        for _ in range(10):
+            seg.hvp(theta, x0, u)
+        hvp_ms = (time.perf_counter() - t1) * 1e2
         return {"workload": "%d-step segment of the chain + 200 water (750 atoms), checkpoints every %d" % (nsteps, every),
-                "gradient_s": dt, "force_evaluations": seg.n_force_calls,
-                "ms_per_force_evaluation": dt * 1e3 / max(seg.n_force_calls, 1)}
+                "gradient_s": dt, "force_evaluations": nf, "hvp_calls": nh,
+                "hvp": "exact: mdn_vi_jvp (closed form) + mdn_force_jvp (dual-number force kernels), FP64",
+                "ms_per_hvp": hvp_ms}
     finally:
         seg.close()
 

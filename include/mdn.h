@@ -217,6 +217,15 @@ int mdn_run(mdn_ctx *c, const char *outdir, int32_t calc0, int32_t precision, ui
  * force evaluations (the trajectory-segment adjoint) */
 int mdn_set_positions(mdn_ctx *c, int32_t replica, const double *atom_x);
 int mdn_get_forces(mdn_ctx *c, int32_t replica, double *atom_f);
+/* derivative of the forces of the selected terms (bits as mdn_compute; gi,
+ * vi, gg, gv, vv, AMBER, vapor and gas intra; Efield adds nothing; BMH,
+ * Tersoff and SW: MDN_E_ARG) along the atom displacement atom_u ([n_atoms][3],
+ * reference order) at the replica's positions, in FP64: atom_hu = d/de
+ * F(x + e u) at e = 0, the Hessian of -U applied to u, by forward-mode
+ * differentiation of the force kernels (exact to rounding).  Pair lists and
+ * blend weights are held as they stand.  No reference counterpart: the
+ * second-order terms of the trajectory-segment adjoint. */
+int mdn_force_jvp(mdn_ctx *c, int32_t replica, uint32_t bits, const double *atom_u, double *atom_hu);
 
 /* drop-in path: one replica, FP64, one launch per reference call */
 int mdn_make_pair(mdn_ctx *c, int32_t replica);
@@ -309,6 +318,14 @@ int mdn_vi_adjoint(const double *x, const double *w, int32_t S, int32_t n_ion,
                    const double *vap_q, const uint8_t *vap_in,
                    const double *gE, const double *gF,
                    double *g_sigma, double *g_eps, double *g_q);
+/* derivative of mdn_vi_forward's forces along the displacement u
+ * ([S][n_ion + n_vap][3]) of the snapshots' atoms, into hu (same shape):
+ * the Hessian of -E applied to u, closed form.  No reference counterpart:
+ * the second-order terms of the trajectory-segment adjoint. */
+int mdn_vi_jvp(const double *x, const double *w, int32_t S, int32_t n_ion, int32_t n_vap,
+               const double *ion_sigma, const double *ion_eps, const double *ion_q,
+               const double *vap_sigma, const double *vap_eps, const double *vap_q,
+               const uint8_t *vap_in, const double *u, double *hu);
 /* mdn_vi_adjoint for callers that cannot allocate (code under HIP graph
  * capture, tight fitting loops, harnesses that forbid hipMalloc).  No
  * counterpart in the reference.  Every pointer is a DEVICE pointer on the

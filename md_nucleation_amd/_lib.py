@@ -83,6 +83,7 @@ _SIGS = {
     "mdn_run": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int32, C.c_int32, C.c_uint32]),
     "mdn_set_positions": (C.c_int, [C.c_void_p, C.c_int32, _dp]),
     "mdn_get_forces": (C.c_int, [C.c_void_p, C.c_int32, _dp]),
+    "mdn_force_jvp": (C.c_int, [C.c_void_p, C.c_int32, C.c_uint32, _dp, _dp]),
     "mdn_make_pair": (C.c_int, [C.c_void_p, C.c_int32]),
     "mdn_zero_forces": (C.c_int, [C.c_void_p, C.c_int32]),
     "mdn_compute": (C.c_int, [C.c_void_p, C.c_int32, C.c_uint32, C.c_int32, C.c_int32]),
@@ -99,6 +100,7 @@ _SIGS = {
     "mdn_get_batch_plan": (C.c_int, [C.c_void_p, _ip, _ip, C.c_int32]),
     "mdn_vi_forward": (C.c_int, [_dp, _dp, C.c_int32, C.c_int32, C.c_int32] + [_dp] * 6 + [C.c_void_p, _dp, _dp]),
     "mdn_vi_adjoint": (C.c_int, [_dp, _dp, C.c_int32, C.c_int32, C.c_int32] + [_dp] * 6 + [C.c_void_p] + [_dp] * 5),
+    "mdn_vi_jvp": (C.c_int, [_dp, _dp, C.c_int32, C.c_int32, C.c_int32] + [_dp] * 6 + [C.c_void_p] + [_dp] * 2),
     "mdn_vi_adjoint_noalloc": (C.c_int, [C.c_void_p] * 2 + [C.c_int32] * 3 + [C.c_void_p] * 12 + [C.c_void_p]),
     "mdn_qqrd2e": (C.c_double, []),
     "mdn_sync": (C.c_int, [C.c_void_p]),
@@ -338,6 +340,16 @@ class Engine:
         f = np.zeros(3 * self.sys[r].n_atoms)
         check(lib().mdn_get_forces(self.h, int(r), f.ctypes.data_as(_dp)), "mdn_get_forces")
         return f.reshape(-1, 3)
+
+    def force_jvp(self, r, bits, u):
+        """d/de F(x + e u) of the selected terms at the replica's positions
+        (the Hessian of -U applied to u), FP64, [n_atoms, 3]"""
+        uu = np.ascontiguousarray(np.asarray(u, np.float64).reshape(-1))
+        assert uu.size == 3 * self.sys[r].n_atoms
+        hu = np.zeros_like(uu)
+        check(lib().mdn_force_jvp(self.h, int(r), int(bits), uu.ctypes.data_as(_dp), hu.ctypes.data_as(_dp)),
+              "mdn_force_jvp")
+        return hu.reshape(-1, 3)
 
     def make_pair(self, r):
         check(lib().mdn_make_pair(self.h, int(r)), "mdn_make_pair")

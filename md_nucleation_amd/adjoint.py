@@ -156,6 +156,18 @@ def vi_adjoint(snap, sig, eps, q, gE=None, gF=None):
     return tuple(out)
 
 
+def vi_jvp(snap, sig, eps, q, U):
+    """derivative of the vi forces along the displacement U [S, n, 3] of the
+    snapshots' atoms (the Hessian of -E applied to U), [S, n, 3]"""
+    x, w, vin, ps, vp = _prep(snap, sig, eps, q)
+    S, n = x.shape[:2]
+    u = np.ascontiguousarray(np.asarray(U, np.float64).reshape(S, n, 3))
+    hu = np.zeros((S, n, 3))
+    M.check(M.lib().mdn_vi_jvp(_p(x), _p(w), S, snap.n_ion, n - snap.n_ion, *map(_p, ps), *map(_p, vp),
+                               vin.ctypes.data_as(C.c_void_p), _p(u), _p(hu)), "mdn_vi_jvp")
+    return hu
+
+
 def _host(t):
     return t.detach().to("cpu", torch.float64).numpy()
 

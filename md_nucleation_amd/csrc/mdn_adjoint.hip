@@ -67,6 +67,63 @@ __device__ inline PairD vi_pair(double r2, double s, double e, double qi, double
   return p;
 }
 
+// d f / d(r^2) of the force factor f = (c0 r^-14 - c1 r^-8 + K q_i q_j r^-3) w
+__device__ inline double vi_df_dr2(double r2, double s, double e, double qi, double qj, double K, double w) {
+  double r2inv = 1.0 / r2, r6inv = r2inv * r2inv * r2inv, rinv = sqrt(r2inv), r4inv = r2inv * r2inv;
+  double s2 = s * s, s6 = s2 * s2 * s2, s12 = s6 * s6;
+  double c0 = 48 * e * s12, c1 = 24 * e * s6;
+  return (-7.0 * c0 * r6inv * r6inv * r4inv + 4.0 * c1 * r6inv * r4inv - 1.5 * K * qi * qj * rinv * r4inv) * w;
+}
+
+// derivative of the pair forces along the displacement u of the atoms (the
+// Hessian of -E applied to u): with d = x_j - x_i and du = u_j - u_i,
+//   dF_j = f du + 2 (d . du) (df/dr^2) d,   dF_i = -dF_j
+// ion side: one thread per (snapshot, ion atom)
+__global__ void k_vi_jvp_ion(ViBatch b, const double *U, double *HU) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= b.S * b.n_ion) return;
+  int s = t / b.n_ion, i = t - s * b.n_ion, n = b.n_ion + b.n_vap;
+  const double *X = b.x + (size_t)s * n * 3, *W = b.w + (size_t)s * n, *Us = U + (size_t)s * n * 3;
+  double xi = X[3 * i], yi = X[3 * i + 1], zi = X[3 * i + 2];
+  double ux = Us[3 * i], uy = Us[3 * i + 1], uz = Us[3 * i + 2];
+  double hx = 0, hy = 0, hz = 0;
+  for (int j = 0; j < b.n_vap; j++) {
+    if (!vap_in(b, s, j)) continue;
+    int a = b.n_ion + j;
+    double dx = X[3 * a] - xi, dy = X[3 * a + 1] - yi, dz = X[3 * a + 2] - zi;
+    double ex = Us[3 * a] - ux, ey = Us[3 * a + 1] - uy, ez = Us[3 * a + 2] - uz;
+    double sg = (b.sig_i[i] + b.sig_v[j]) * 0.5, ep = sqrt(b.eps_i[i] * b.eps_v[j]);
+    double r2 = dx * dx + dy * dy + dz * dz, w = W[i] * W[a];
+    double f = vi_pair(r2, sg, ep, b.q_i[i], b.q_v[j], b.K, w).f;
+    double g = 2.0 * (dx * ex + dy * ey + dz * ez) * vi_df_dr2(r2, sg, ep, b.q_i[i], b.q_v[j], b.K, w);
+    hx -= f * ex + g * dx; hy -= f * ey + g * dy; hz -= f * ez + g * dz;
+  }
+  double *H = HU + ((size_t)s * n + i) * 3;
+  H[0] = hx; H[1] = hy; H[2] = hz;
+}
+
+// vapor side: one thread per (snapshot, vapor atom)
+__global__ void k_vi_jvp_vap(ViBatch b, const double *U, double *HU) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= b.S * b.n_vap) return;
+  int s = t / b.n_vap, j = t - s * b.n_vap, n = b.n_ion + b.n_vap, a = b.n_ion + j;
+  const double *X = b.x + (size_t)s * n * 3, *W = b.w + (size_t)s * n, *Us = U + (size_t)s * n * 3;
+  double xj = X[3 * a], yj = X[3 * a + 1], zj = X[3 * a + 2];
+  double ux = Us[3 * a], uy = Us[3 * a + 1], uz = Us[3 * a + 2];
+  double hx = 0, hy = 0, hz = 0;
+  for (int i = 0; i < b.n_ion && vap_in(b, s, j); i++) {
+    double dx = xj - X[3 * i], dy = yj - X[3 * i + 1], dz = zj - X[3 * i + 2];
+    double ex = ux - Us[3 * i], ey = uy - Us[3 * i + 1], ez = uz - Us[3 * i + 2];
+    double sg = (b.sig_i[i] + b.sig_v[j]) * 0.5, ep = sqrt(b.eps_i[i] * b.eps_v[j]);
+    double r2 = dx * dx + dy * dy + dz * dz, w = W[i] * W[a];
+    double f = vi_pair(r2, sg, ep, b.q_i[i], b.q_v[j], b.K, w).f;
+    double g = 2.0 * (dx * ex + dy * ey + dz * ez) * vi_df_dr2(r2, sg, ep, b.q_i[i], b.q_v[j], b.K, w);
+    hx += f * ex + g * dx; hy += f * ey + g * dy; hz += f * ez + g * dz;
+  }
+  double *H = HU + ((size_t)s * n + a) * 3;
+  H[0] = hx; H[1] = hy; H[2] = hz;
+}
+
 // forward, ion side: one thread per (snapshot, ion atom): F_i and the
 // snapshot's energy partial of atom i
 __global__ void k_vi_fwd_ion(ViBatch b, double *F, double *Epart) {
@@ -241,6 +298,31 @@ int mdn_vi_forward(const double *x, const double *w, int32_t S, int32_t n_ion, i
   if (hipMemcpy(E, dE, sizeof(double) * S, hipMemcpyDeviceToHost) != hipSuccess ||
       hipMemcpy(F, dF, sizeof(double) * S * n * 3, hipMemcpyDeviceToHost) != hipSuccess)
     return set_error(MDN_E_HIP, "mdn_vi_forward: copy back failed");
+  return MDN_OK;
+}
+
+int mdn_vi_jvp(const double *x, const double *w, int32_t S, int32_t n_ion, int32_t n_vap, const double *ion_sigma,
+               const double *ion_eps, const double *ion_q, const double *vap_sigma, const double *vap_eps,
+               const double *vap_q, const uint8_t *vap_in, const double *u, double *hu) {
+  int rc = check_vi(x, w, S, n_ion, n_vap, ion_sigma, ion_eps, ion_q, vap_sigma, vap_eps, vap_q);
+  if (rc) return rc;
+  if (!u || !hu) return set_error(MDN_E_ARG, "mdn_vi_jvp: null direction or output");
+  DevBatch db;
+  ViBatch b;
+  b.K = mdn_qqrd2e();
+  if ((rc = upload_batch(db, b, x, w, S, n_ion, n_vap, ion_sigma, ion_eps, ion_q, vap_sigma, vap_eps, vap_q, vap_in)))
+    return rc;
+  size_t n = (size_t)n_ion + n_vap;
+  const double *dU = nullptr;
+  double *dH = nullptr;
+  if ((rc = db.up(u, (size_t)S * n * 3, &dU)) || (rc = db.alloc((size_t)S * n * 3, &dH))) return rc;
+  int ni = S * n_ion, nv = S * n_vap;
+  hipLaunchKernelGGL(k_vi_jvp_ion, dim3((ni + 255) / 256), dim3(256), 0, 0, b, dU, dH);
+  if (nv) hipLaunchKernelGGL(k_vi_jvp_vap, dim3((nv + 255) / 256), dim3(256), 0, 0, b, dU, dH);
+  if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess)
+    return set_error(MDN_E_HIP, "mdn_vi_jvp: kernel failed");
+  if (hipMemcpy(hu, dH, sizeof(double) * S * n * 3, hipMemcpyDeviceToHost) != hipSuccess)
+    return set_error(MDN_E_HIP, "mdn_vi_jvp: copy back failed");
   return MDN_OK;
 }
 

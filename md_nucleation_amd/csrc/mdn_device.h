@@ -199,6 +199,54 @@ __device__ inline float blend_weight_r<float>(double r2) {
 }
 
 // ------------------------------------------------------------------------
+// Forward-mode dual number: a value and its derivative along one direction
+// of the positions.  Instantiating the force routines with it gives the
+// derivative of the forces along u, i.e. the Hessian of -U applied to u,
+// exactly (to FP64 rounding).  Comparisons look at the value only.
+struct Dual {
+  double v, d;
+  __host__ __device__ Dual() : v(0), d(0) {}
+  __host__ __device__ Dual(double x) : v(x), d(0) {}
+  __host__ __device__ Dual(double x, double dx) : v(x), d(dx) {}
+  __host__ __device__ explicit operator double() const { return v; }
+  __device__ Dual &operator+=(const Dual &b) { v += b.v; d += b.d; return *this; }
+  __device__ Dual &operator-=(const Dual &b) { v -= b.v; d -= b.d; return *this; }
+  __device__ Dual &operator*=(const Dual &b) { d = d * b.v + v * b.d; v *= b.v; return *this; }
+};
+__device__ inline Dual operator-(const Dual &a) { return Dual(-a.v, -a.d); }
+__device__ inline Dual operator+(const Dual &a, const Dual &b) { return Dual(a.v + b.v, a.d + b.d); }
+__device__ inline Dual operator-(const Dual &a, const Dual &b) { return Dual(a.v - b.v, a.d - b.d); }
+__device__ inline Dual operator*(const Dual &a, const Dual &b) { return Dual(a.v * b.v, a.d * b.v + a.v * b.d); }
+__device__ inline Dual operator/(const Dual &a, const Dual &b) {
+  const double q = a.v / b.v;
+  return Dual(q, (a.d - q * b.d) / b.v);
+}
+__device__ inline Dual operator+(const Dual &a, double b) { return Dual(a.v + b, a.d); }
+__device__ inline Dual operator+(double a, const Dual &b) { return Dual(a + b.v, b.d); }
+__device__ inline Dual operator-(const Dual &a, double b) { return Dual(a.v - b, a.d); }
+__device__ inline Dual operator-(double a, const Dual &b) { return Dual(a - b.v, -b.d); }
+__device__ inline Dual operator*(const Dual &a, double b) { return Dual(a.v * b, a.d * b); }
+__device__ inline Dual operator*(double a, const Dual &b) { return Dual(a * b.v, a * b.d); }
+__device__ inline Dual operator/(const Dual &a, double b) { return Dual(a.v / b, a.d / b); }
+__device__ inline Dual operator/(double a, const Dual &b) {
+  const double q = a / b.v;
+  return Dual(q, -q * b.d / b.v);
+}
+__device__ inline bool operator>(const Dual &a, double b) { return a.v > b; }
+__device__ inline bool operator<(const Dual &a, double b) { return a.v < b; }
+// the overloads below sit beside the usual double ones
+using ::acos;
+using ::sqrt;
+__device__ inline Dual sqrt(const Dual &a) {
+  const double r = ::sqrt(a.v);
+  return Dual(r, r > 0 ? 0.5 * a.d / r : 0.0);
+}
+// d acos(x) = -dx / sqrt(1 - x^2)
+__device__ inline Dual acos(const Dual &a) { return Dual(::acos(a.v), -a.d / ::sqrt(1.0 - a.v * a.v)); }
+__device__ inline double val(double a) { return a; }
+__device__ inline double val(const Dual &a) { return a.v; }
+
+// ------------------------------------------------------------------------
 // Pair arithmetic.  R = double (plain) or float (mixed: the displacement is
 // formed in FP64 and rounded, the rest is FP32; the caller accumulates the
 // force in FP64).
@@ -241,28 +289,31 @@ __device__ inline R lj_fpair(R rsq, R c0, R c1, R w, bool eflag, double &e) {
 }
 
 // harmonic bond, compute_AMBER.cpp:52-77: returns fb with F_i = fb*d, F_j = -fb*d
-__device__ inline double bond_fb(double rsq, double k, double r0, bool eflag, double &e) {
-  double r = sqrt(rsq);
-  double dr = r - r0;
-  double rk = k * dr;
-  if (eflag) e += rk * dr;
+// (T = double, or Dual for the derivative along a direction)
+template <typename T>
+__device__ inline T bond_fb(T rsq, double k, double r0, bool eflag, double &e) {
+  T r = sqrt(rsq);
+  T dr = r - r0;
+  T rk = k * dr;
+  if (eflag) e += val(rk * dr);
   return -2.0 * rk / r;
 }
 
 // harmonic angle, compute_AMBER.cpp:80-127
-__device__ inline void angle_f(const double d1[3], const double d2[3], double k, double th0, bool eflag, double &e,
-                               double f1[3], double f3[3]) {
-  double rsq1 = d1[0] * d1[0] + d1[1] * d1[1] + d1[2] * d1[2];
-  double rsq2 = d2[0] * d2[0] + d2[1] * d2[1] + d2[2] * d2[2];
-  double r1 = sqrt(rsq1), r2 = sqrt(rsq2);
-  double C = d1[0] * d2[0] + d1[1] * d2[1] + d1[2] * d2[2];
-  C /= r1 * r2;
-  double Cs = 1.0 / sqrt(1.0 - C * C);
-  double dtheta = acos(C) - th0;
-  double tk = k * dtheta;
-  if (eflag) e += tk * dtheta;
-  double a = -2.0 * tk * Cs;
-  double a11 = a * C / rsq1, a12 = -a / (r1 * r2), a22 = a * C / rsq2;
+template <typename T>
+__device__ inline void angle_f(const T d1[3], const T d2[3], double k, double th0, bool eflag, double &e, T f1[3],
+                               T f3[3]) {
+  T rsq1 = d1[0] * d1[0] + d1[1] * d1[1] + d1[2] * d1[2];
+  T rsq2 = d2[0] * d2[0] + d2[1] * d2[1] + d2[2] * d2[2];
+  T r1 = sqrt(rsq1), r2 = sqrt(rsq2);
+  T C = d1[0] * d2[0] + d1[1] * d2[1] + d1[2] * d2[2];
+  C = C / (r1 * r2);
+  T Cs = 1.0 / sqrt(1.0 - C * C);
+  T dtheta = acos(C) - th0;
+  T tk = k * dtheta;
+  if (eflag) e += val(tk * dtheta);
+  T a = -2.0 * tk * Cs;
+  T a11 = a * C / rsq1, a12 = -a / (r1 * r2), a22 = a * C / rsq2;
   for (int q = 0; q < 3; q++) {
     f1[q] = a11 * d1[q] + a12 * d2[q];
     f3[q] = a22 * d2[q] + a12 * d1[q];
@@ -272,30 +323,30 @@ __device__ inline void angle_f(const double d1[3], const double d2[3], double k,
 // multi-term dihedral, compute_AMBER.cpp:130-253 (LAMMPS charmm form, n-fold
 // recurrence for cos/sin(n phi); the n==0 special case checks the first
 // term's n for every term, compute_AMBER.cpp:199)
-__device__ inline void dihedral_f(const double vb1[3], const double vb2[3], const double vb3[3], const double *co,
-                                  int multi, bool eflag, double &e, double f1[3], double f2[3], double f3[3],
-                                  double f4[3]) {
-  double vb2m[3] = {-vb2[0], -vb2[1], -vb2[2]};
-  double ax = vb1[1] * vb2m[2] - vb1[2] * vb2m[1];
-  double ay = vb1[2] * vb2m[0] - vb1[0] * vb2m[2];
-  double az = vb1[0] * vb2m[1] - vb1[1] * vb2m[0];
-  double bx = vb3[1] * vb2m[2] - vb3[2] * vb2m[1];
-  double by = vb3[2] * vb2m[0] - vb3[0] * vb2m[2];
-  double bz = vb3[0] * vb2m[1] - vb3[1] * vb2m[0];
-  double rasq = ax * ax + ay * ay + az * az;
-  double rbsq = bx * bx + by * by + bz * bz;
-  double rgsq = vb2m[0] * vb2m[0] + vb2m[1] * vb2m[1] + vb2m[2] * vb2m[2];
-  double rg = sqrt(rgsq);
-  double rginv = rg > 0 ? 1.0 / rg : 0.0;
-  double ra2inv = rasq > 0 ? 1.0 / rasq : 0.0;
-  double rb2inv = rbsq > 0 ? 1.0 / rbsq : 0.0;
-  double rabinv = sqrt(ra2inv * rb2inv);
-  double c = (ax * bx + ay * by + az * bz) * rabinv;
-  double s = rg * rabinv * (ax * vb3[0] + ay * vb3[1] + az * vb3[2]);
-  double df = 0.0;
+template <typename T>
+__device__ inline void dihedral_f(const T vb1[3], const T vb2[3], const T vb3[3], const double *co, int multi,
+                                  bool eflag, double &e, T f1[3], T f2[3], T f3[3], T f4[3]) {
+  T vb2m[3] = {-vb2[0], -vb2[1], -vb2[2]};
+  T ax = vb1[1] * vb2m[2] - vb1[2] * vb2m[1];
+  T ay = vb1[2] * vb2m[0] - vb1[0] * vb2m[2];
+  T az = vb1[0] * vb2m[1] - vb1[1] * vb2m[0];
+  T bx = vb3[1] * vb2m[2] - vb3[2] * vb2m[1];
+  T by = vb3[2] * vb2m[0] - vb3[0] * vb2m[2];
+  T bz = vb3[0] * vb2m[1] - vb3[1] * vb2m[0];
+  T rasq = ax * ax + ay * ay + az * az;
+  T rbsq = bx * bx + by * by + bz * bz;
+  T rgsq = vb2m[0] * vb2m[0] + vb2m[1] * vb2m[1] + vb2m[2] * vb2m[2];
+  T rg = sqrt(rgsq);
+  T rginv = rg > 0 ? 1.0 / rg : T(0.0);
+  T ra2inv = rasq > 0 ? 1.0 / rasq : T(0.0);
+  T rb2inv = rbsq > 0 ? 1.0 / rbsq : T(0.0);
+  T rabinv = sqrt(ra2inv * rb2inv);
+  T c = (ax * bx + ay * by + az * bz) * rabinv;
+  T s = rg * rabinv * (ax * vb3[0] + ay * vb3[1] + az * vb3[2]);
+  T df = 0.0;
   for (int JJ = 0; JJ < multi; JJ++) {
     const double *t = co + 5 * JJ;
-    double p = 1.0, df1 = 0.0, ddf1 = 0.0;
+    T p = 1.0, df1 = 0.0, ddf1 = 0.0;
     for (int l = 0; l < t[1]; l++) {
       ddf1 = p * c - df1 * s;
       df1 = p * s + df1 * c;
@@ -310,17 +361,17 @@ __device__ inline void dihedral_f(const double vb1[3], const double vb2[3], cons
       df1 = 0.0;
     }
     df += -t[0] * df1;
-    if (eflag) e += t[0] * p;
+    if (eflag) e += val(t[0] * p);
   }
-  double fg = vb1[0] * vb2m[0] + vb1[1] * vb2m[1] + vb1[2] * vb2m[2];
-  double hg = vb3[0] * vb2m[0] + vb3[1] * vb2m[1] + vb3[2] * vb2m[2];
-  double fga = fg * ra2inv * rginv, hgb = hg * rb2inv * rginv;
-  double gaa = -ra2inv * rg, gbb = rb2inv * rg;
-  double dtf[3] = {gaa * ax, gaa * ay, gaa * az};
-  double dtg[3] = {fga * ax - hgb * bx, fga * ay - hgb * by, fga * az - hgb * bz};
-  double dth[3] = {gbb * bx, gbb * by, gbb * bz};
+  T fg = vb1[0] * vb2m[0] + vb1[1] * vb2m[1] + vb1[2] * vb2m[2];
+  T hg = vb3[0] * vb2m[0] + vb3[1] * vb2m[1] + vb3[2] * vb2m[2];
+  T fga = fg * ra2inv * rginv, hgb = hg * rb2inv * rginv;
+  T gaa = -ra2inv * rg, gbb = rb2inv * rg;
+  T dtf[3] = {gaa * ax, gaa * ay, gaa * az};
+  T dtg[3] = {fga * ax - hgb * bx, fga * ay - hgb * by, fga * az - hgb * bz};
+  T dth[3] = {gbb * bx, gbb * by, gbb * bz};
   for (int q = 0; q < 3; q++) {
-    double sq = df * dtg[q];
+    T sq = df * dtg[q];
     f1[q] = df * dtf[q];
     f2[q] = sq - f1[q];
     f4[q] = df * dth[q];

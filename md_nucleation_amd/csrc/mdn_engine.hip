@@ -619,6 +619,7 @@ struct SStoreT {
   const int16_t *kidx;
   const int *ptype;  // the ion's atom types (global memory; BMH is ion-only)
   typedef int H;
+  typedef double P;  // position and force type of the term routines
   __device__ H h(int a) const { return a; }
   // per-atom parameters: the ion + vapor range has its own, gas atoms read
   // their species atom's
@@ -1474,6 +1475,9 @@ struct mdn_ctx {
   // of the median) above which a replica may get a CU of its own
   int own_threads = 512, shared_threads = 128, dense_runl = 0;
   double heavy_factor = 2.0;
+  // direction and result buffers of mdn_force_jvp ([2][jvp_cap] doubles)
+  double *d_jvp = nullptr;
+  size_t jvp_cap = 0;
   hipStream_t stream2 = nullptr, stream3 = nullptr;
   hipEvent_t evj = nullptr, evk = nullptr, evl = nullptr;
   int *d_order = nullptr;
@@ -1852,6 +1856,30 @@ static int launch_terms(mdn_ctx *c, const HostReplica &h, const DevReplica *R, u
   return MDN_OK;
 }
 
+// the derivative along u of the forces of the selected pair and bonded terms
+// (gi, vi, gg, gv, vv, AMBER, vapor and gas intra) at the replica's
+// positions, in FP64, added to hu; the Efield force does not depend on the
+// positions
+__global__ void __launch_bounds__(kBlk) k_force_jvp(const DevReplica *__restrict__ Rp, uint32_t bits,
+                                                    const double *__restrict__ u, double *__restrict__ hu) {
+  const DevReplica &R = *Rp;
+  JStore s;
+  s.R = Rp;
+  s.L = view_from_dyn(R);
+  s.u = u;
+  s.hu = hu;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x, nt = gridDim.x * blockDim.x;
+  double e[U_N];
+  if (bits & B_GI) term_gi<Dual>(R, s, t, nt, false, e);
+  if (bits & B_VI) term_vi<Dual>(R, s, t, nt, false, e);
+  if (bits & B_GG) term_gg<Dual>(R, s, t, nt, false, e);
+  if (bits & B_GV) term_gv<Dual>(R, s, t, nt, false, e);
+  if (bits & B_VV) term_vv<Dual>(R, s, t, nt, false, e);
+  if (bits & B_AMBER) term_amber<Dual>(R, s, t, nt, false, e);
+  if (bits & B_VINTRA) term_vintra(R, s, t, nt, false, e);
+  if (bits & B_GINTRA) term_gintra(R, s, t, nt, false, e);
+}
+
 // start of the rest of a step the batched kernel left unfinished: clears the
 // flag and, on an energy step, the energy buckets
 __global__ void k_resume(const DevReplica *__restrict__ Rp, int zeroU) {
@@ -1976,6 +2004,7 @@ void mdn_destroy(mdn_ctx *c) {
   drop_graphs(c);
   if (c->stream) hipStreamSynchronize(c->stream);
   if (c->dbuf) hipFree(c->dbuf);
+  if (c->d_jvp) hipFree(c->d_jvp);
   if (c->h_dyn) hipHostFree(c->h_dyn);
   if (c->h_order) hipHostFree(c->h_order);
   if (c->ev0) hipEventDestroy(c->ev0);
@@ -2269,6 +2298,31 @@ int mdn_get_forces(mdn_ctx *c, int32_t r, double *atom_f) {
   HIPCHK(hipSetDevice(c->device));
   HIPCHK(hipMemcpyAsync(atom_f, c->reps[r].dev.atom_f, sizeof(double) * 3 * (size_t)c->reps[r].def.n_atoms,
                         hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return MDN_OK;
+}
+
+int mdn_force_jvp(mdn_ctx *c, int32_t r, uint32_t bits, const double *atom_u, double *atom_hu) {
+  int rc = check_replica(c, r);
+  if (rc) return rc;
+  if (!atom_u || !atom_hu) return set_error(MDN_E_ARG, "mdn_force_jvp: null array");
+  if (bits & (MDN_ION_BMH | MDN_ION_TERSOFF | MDN_ION_SW))
+    return set_error(MDN_E_ARG, "mdn_force_jvp: no derivative for the BMH, Tersoff or SW ion terms");
+  HIPCHK(hipSetDevice(c->device));
+  const size_t n3 = 3 * (size_t)c->reps[r].def.n_atoms;
+  if (c->jvp_cap < n3) {
+    if (c->d_jvp) HIPCHK(hipFree(c->d_jvp));
+    c->d_jvp = nullptr;
+    c->jvp_cap = 0;
+    HIPCHK(hipMalloc((void **)&c->d_jvp, sizeof(double) * 2 * n3));
+    c->jvp_cap = n3;
+  }
+  double *du = c->d_jvp, *dhu = c->d_jvp + c->jvp_cap;
+  HIPCHK(hipMemcpyAsync(du, atom_u, sizeof(double) * n3, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemsetAsync(dhu, 0, sizeof(double) * n3, c->stream));
+  hipLaunchKernelGGL(k_force_jvp, dim3(512), dim3(kBlk), 0, c->stream, c->d_reps + r, bits, du, dhu);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(atom_hu, dhu, sizeof(double) * n3, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   return MDN_OK;
 }

@@ -32,6 +32,7 @@ __device__ inline LView view_from_dyn(const DevReplica &R) {
 struct GStore {
   const DevReplica *R;
   LView L;
+  typedef double P;  // position and force type of the term routines
   struct H { int g, sp, idx; };
   __device__ H h(int a) const {
     H r;
@@ -72,6 +73,25 @@ struct GStore {
   }
 };
 
+// forward-mode store over global memory: the positions are x + eps u as
+// dual numbers, and each force's derivative along u (d F / d eps, the
+// Hessian of -U applied to u) is added to hu; the forces themselves are not
+// written.  Pair lists and blend weights are held as they stand, as in one
+// force evaluation.
+struct JStore : GStore {
+  typedef Dual P;
+  const double *u;  // [n_atoms][3] direction
+  double *hu;       // [n_atoms][3] derivative of the forces along u
+  __device__ void pos(const H &q, Dual p[3]) const {
+    const double *x = R->atom_x + 3 * (size_t)q.g, *d = u + 3 * (size_t)q.g;
+    p[0] = Dual(x[0], d[0]); p[1] = Dual(x[1], d[1]); p[2] = Dual(x[2], d[2]);
+  }
+  __device__ void fadd(const H &q, const Dual &fx, const Dual &fy, const Dual &fz) const {
+    double *f = hu + 3 * (size_t)q.g;
+    atomicAdd(f, fx.d); atomicAdd(f + 1, fy.d); atomicAdd(f + 2, fz.d);
+  }
+};
+
 __device__ inline void pc(const DevReplica &R, int t1, int t2, double &c0, double &c1) {
   const double *p = R.pair_coeff + 2 * ((size_t)t1 * R.n_types + t2);
   c0 = p[0]; c1 = p[1];
@@ -81,10 +101,11 @@ __device__ inline void pc(const DevReplica &R, int t1, int t2, double &c0, doubl
 template <typename Rl, bool COUL, class S>
 __device__ inline void pair_ab(const DevReplica &R, const S &s, const typename S::H &ha, const typename S::H &hb,
                                bool weighted, bool eflag, double &e) {
-  double xa[3], xb[3];
+  typedef typename S::P P;
+  P xa[3], xb[3];
   s.pos(ha, xa);
   s.pos(hb, xb);
-  double dx = xa[0] - xb[0], dy = xa[1] - xb[1], dz = xa[2] - xb[2];
+  P dx = xa[0] - xb[0], dy = xa[1] - xb[1], dz = xa[2] - xb[2];
   Rl c0, c1;
   s.template coeff<Rl>(R, ha, hb, c0, c1);
   Rl rx = (Rl)dx, ry = (Rl)dy, rz = (Rl)dz;
@@ -95,7 +116,7 @@ __device__ inline void pair_ab(const DevReplica &R, const S &s, const typename S
     fp = ljcoul_fpair<Rl>(rsq, c0, c1, (Rl)(R.qqrd2e * s.q(ha) * s.q(hb)), w, eflag, e);
   else
     fp = lj_fpair<Rl>(rsq, c0, c1, w, eflag, e);
-  double fx = (double)(fp * rx), fy = (double)(fp * ry), fz = (double)(fp * rz);
+  P fx = (P)(fp * rx), fy = (P)(fp * ry), fz = (P)(fp * rz);
   s.fadd(ha, fx, fy, fz);
   s.fadd(hb, -fx, -fy, -fz);
 }
@@ -183,22 +204,24 @@ __device__ void term_efield(const DevReplica &R, const S &s, int t, int nt) {
 // bonded terms of one species' molecule whose first local atom is `base`
 template <class S>
 __device__ inline void bond_term(const DevReplica &R, const S &s, int base, const int *b, bool eflag, double &e) {
+  typedef typename S::P P;
   typename S::H hi = s.h(base + b[0]), hj = s.h(base + b[1]);
-  double xi[3], xj[3];
+  P xi[3], xj[3];
   s.pos(hi, xi); s.pos(hj, xj);
-  double d[3] = {xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2]};
-  double fb = bond_fb(d[0] * d[0] + d[1] * d[1] + d[2] * d[2], R.btype[2 * b[2]], R.btype[2 * b[2] + 1], eflag, e);
+  P d[3] = {xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2]};
+  P fb = bond_fb<P>(d[0] * d[0] + d[1] * d[1] + d[2] * d[2], R.btype[2 * b[2]], R.btype[2 * b[2] + 1], eflag, e);
   s.fadd(hi, fb * d[0], fb * d[1], fb * d[2]);
   s.fadd(hj, -fb * d[0], -fb * d[1], -fb * d[2]);
 }
 template <class S>
 __device__ inline void angle_term(const DevReplica &R, const S &s, int base, const int *a, bool eflag, double &e) {
+  typedef typename S::P P;
   typename S::H hi = s.h(base + a[0]), hj = s.h(base + a[1]), hk = s.h(base + a[2]);
-  double xi[3], xj[3], xk[3], f1[3], f3[3];
+  P xi[3], xj[3], xk[3], f1[3], f3[3];
   s.pos(hi, xi); s.pos(hj, xj); s.pos(hk, xk);
-  double d1[3] = {xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2]};
-  double d2[3] = {xk[0] - xj[0], xk[1] - xj[1], xk[2] - xj[2]};
-  angle_f(d1, d2, R.ctype[2 * a[3]], R.ctype[2 * a[3] + 1], eflag, e, f1, f3);
+  P d1[3] = {xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2]};
+  P d2[3] = {xk[0] - xj[0], xk[1] - xj[1], xk[2] - xj[2]};
+  angle_f<P>(d1, d2, R.ctype[2 * a[3]], R.ctype[2 * a[3] + 1], eflag, e, f1, f3);
   s.fadd(hi, f1[0], f1[1], f1[2]);
   s.fadd(hj, -(f1[0] + f3[0]), -(f1[1] + f3[1]), -(f1[2] + f3[2]));
   s.fadd(hk, f3[0], f3[1], f3[2]);
@@ -206,13 +229,14 @@ __device__ inline void angle_term(const DevReplica &R, const S &s, int base, con
 template <class S>
 __device__ inline void dihedral_term(const DevReplica &R, const S &s, int base, const int *d, bool eflag,
                                      double &e) {
+  typedef typename S::P P;
   typename S::H h1 = s.h(base + d[0]), h2 = s.h(base + d[1]), h3 = s.h(base + d[2]), h4 = s.h(base + d[3]);
-  double x1[3], x2[3], x3[3], x4[3], f1[3], f2[3], f3[3], f4[3];
+  P x1[3], x2[3], x3[3], x4[3], f1[3], f2[3], f3[3], f4[3];
   s.pos(h1, x1); s.pos(h2, x2); s.pos(h3, x3); s.pos(h4, x4);
-  double vb1[3] = {x1[0] - x2[0], x1[1] - x2[1], x1[2] - x2[2]};
-  double vb2[3] = {x3[0] - x2[0], x3[1] - x2[1], x3[2] - x2[2]};
-  double vb3[3] = {x4[0] - x3[0], x4[1] - x3[1], x4[2] - x3[2]};
-  dihedral_f(vb1, vb2, vb3, R.dcoeff + 15 * d[4], R.dmulti[d[4]], eflag, e, f1, f2, f3, f4);
+  P vb1[3] = {x1[0] - x2[0], x1[1] - x2[1], x1[2] - x2[2]};
+  P vb2[3] = {x3[0] - x2[0], x3[1] - x2[1], x3[2] - x2[2]};
+  P vb3[3] = {x4[0] - x3[0], x4[1] - x3[1], x4[2] - x3[2]};
+  dihedral_f<P>(vb1, vb2, vb3, R.dcoeff + 15 * d[4], R.dmulti[d[4]], eflag, e, f1, f2, f3, f4);
   s.fadd(h1, f1[0], f1[1], f1[2]);
   s.fadd(h2, f2[0], f2[1], f2[2]);
   s.fadd(h3, f3[0], f3[1], f3[2]);

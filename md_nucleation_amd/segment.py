@@ -23,12 +23,15 @@ v_{n+1}, a step back is
     g   <- g + k c (dF(x_n)/dtheta)^T u
 where k = 2 for 0 < n (x_n enters the second half kick of step n - 1 and the
 first of step n with the same u) and k = 1 for n = 0.  H(x) u, the Hessian
-of -U applied to u, is the central difference
-(F(x + e u) - F(x - e u)) / (2 e) of the FP64 forces with e |u|_max = 1e-5 A;
-(dF/dtheta)^T u is ``mdn_vi_adjoint`` with gF = u (the hand-written adjoint
-kernel), one call for all steps of a checkpoint interval.  The forward pass
-keeps (x, v) every ``every`` steps; the reverse pass recomputes each
-interval's states from its checkpoint.
+of -U applied to u, is the derivative of the forces along u, computed
+exactly (to FP64 rounding) on the GPU: ``mdn_vi_jvp`` (closed form) for the
+vi term with the parameters theta, ``mdn_force_jvp`` (the force kernels
+instantiated with forward-mode dual numbers) for AMBER, vv and vapor intra;
+the Efield force does not depend on x.  (dF/dtheta)^T u is
+``mdn_vi_adjoint`` with gF = u (the hand-written adjoint kernel), one call
+for all steps of a checkpoint interval.  The forward pass keeps (x, v) every
+``every`` steps; the reverse pass recomputes each interval's states from its
+checkpoint.
 """
 from __future__ import annotations
 
@@ -39,22 +42,28 @@ from . import _lib as M
 from . import adjoint as A
 
 KICK_UNIT = 4.184e-4  # verlet.cpp:49
-HVP_STEP = 1e-5       # largest displacement of a Hessian-vector difference, A
+HVP_STEP = 1e-5       # largest displacement of a finite-difference Hessian-vector product, A
 
 
 class Integrator:
     """MD::verlet over ``mass`` [n] with time step ``dt``; a subclass gives
     ``forces(theta, X [S, n, 3])`` and ``theta_vjp(theta, frames, us)``
-    (the sum over frames of (dF(frame)/dtheta)^T u)."""
+    (the sum over frames of (dF(frame)/dtheta)^T u), and may give an exact
+    ``hvp``."""
 
     def setup(self, mass, dt):
         self.mass = np.asarray(mass, np.float64)
         self.dt = float(dt)
         self.c = 0.5 * self.dt * KICK_UNIT
         self.n_force_calls = 0
+        self.n_hvp_calls = 0
 
     def hvp(self, theta, x, u):
-        """H(x) u by central differences of the forces"""
+        """H(x) u; here by central differences of the forces"""
+        return self.hvp_fd(theta, x, u)
+
+    def hvp_fd(self, theta, x, u):
+        """H(x) u by central differences of the forces (e |u|_max = 1e-5 A)"""
         um = np.abs(u).max()
         if um == 0:
             return np.zeros_like(u)
@@ -129,6 +138,24 @@ class Segment(Integrator):
             F[s] += self.eng.get_forces(0)[self.idx]
         self.n_force_calls += X.shape[0]
         return F
+
+    def hvp(self, theta, x, u):
+        """H(x) u, exact to rounding: the derivative of the forces along u
+        (mdn_vi_jvp for vi with the parameters theta, mdn_force_jvp for the
+        other position-dependent terms)"""
+        self.n_hvp_calls += 1
+        x = np.asarray(x, np.float64)
+        u = np.asarray(u, np.float64)
+        out = A.vi_jvp(self._snap(x[None]), *theta, u[None])[0] if self.vi else np.zeros_like(x)
+        bits = self.rest_bits & ~M.MDN_EFIELD
+        if bits:
+            full = self.atom_x.copy()
+            full[self.idx] = x
+            self.eng.set_positions(0, full)
+            U = np.zeros_like(full)
+            U[self.idx] = u
+            out = out + self.eng.force_jvp(0, bits, U)[self.idx]
+        return out
 
     def theta_vjp(self, theta, frames, us):
         """sum over frames of (dF/dtheta)^T u: the hand-written vi adjoint"""

@@ -15,6 +15,10 @@ parameters perturbed by +-10%) at steps 10, 20, ..., 100.
              gradient)
   spacing    checkpoints every 10 and every 25 steps give the same gradient
   autograd   TrajectoryLoss.backward == the reverse pass
+  hvp        the exact Hessian-vector products (mdn_vi_jvp, mdn_force_jvp)
+             == central differences of the FP64 forces, term by term (vi,
+             AMBER, vv, vapor intra) and in total, and symmetric:
+             v . (H u) == u . (H v)
 """
 import numpy as np
 import pytest
@@ -112,7 +116,7 @@ def test_gradient_vs_gpu_finite_differences(seg):
         tm[p][i] -= H
         fd = (SG.segment_loss(s, tp, x0, v0, T, tgt, OBS) - SG.segment_loss(s, tm, x0, v0, T, tgt, OBS)) / (2 * H)
         print("param", p, "atom", i, "reverse", g[p][i], "fd", fd)
-        assert abs(g[p][i] - fd) <= 1e-5 * scale[p], (p, i, g[p][i], fd)
+        assert abs(g[p][i] - fd) <= 1e-7 * scale[p], (p, i, g[p][i], fd)
 
 
 def test_gradient_vs_oracle_trajectory(seg):
@@ -130,7 +134,7 @@ def test_gradient_vs_oracle_trajectory(seg):
             vals.append(sum(float(((xo[s.idx] - tgt[(k + 1) * EVERY]) ** 2).sum()) / N for k, xo in enumerate(ox)))
         fd = (vals[0] - vals[1]) / (2 * H)
         print("param", p, "atom", i, "reverse", g[p][i], "oracle fd", fd)
-        assert abs(g[p][i] - fd) <= 1e-5 * np.abs(g[p]).max()
+        assert abs(g[p][i] - fd) <= 1e-7 * np.abs(g[p]).max()
 
 
 def test_checkpoint_spacing(seg):
@@ -151,3 +155,61 @@ def test_autograd_wrapper(seg):
     assert abs(float(L.detach()) - L0) <= 1e-12 * L0
     for t, w in zip((sig, eps, q), g):
         np.testing.assert_allclose(t.grad.numpy(), 3.0 * w, rtol=1e-7, atol=1e-9 * np.abs(w).max())
+
+
+def _term_forces(s, theta, x, term):
+    """FP64 forces of one term at the segment positions x"""
+    if term == "vi":
+        return A.vi_forward(s._snap(x[None]), *theta)[1][0]
+    full = s.atom_x.copy()
+    full[s.idx] = x
+    s.eng.set_positions(0, full)
+    s.eng.zero_forces(0)
+    s.eng.compute(0, term, 0, M.FP64)
+    return s.eng.get_forces(0)[s.idx]
+
+
+def _term_hvp(s, theta, x, u, term):
+    if term == "vi":
+        return A.vi_jvp(s._snap(x[None]), *theta, u[None])[0]
+    full = s.atom_x.copy()
+    full[s.idx] = x
+    s.eng.set_positions(0, full)
+    U = np.zeros_like(full)
+    U[s.idx] = u
+    return s.eng.force_jvp(0, term, U)[s.idx]
+
+
+def test_exact_hvp_vs_finite_differences(seg):
+    sd, st1, s, x0, v0, theta, tgt = seg
+    rng = np.random.default_rng(11)
+    u = rng.normal(0, 1, x0.shape)
+    terms = [("vi", "vi"), ("AMBER", M.MDN_ION_AMBER), ("vv", M.MDN_VV), ("vapor intra", M.MDN_VAPOR_INTRA)]
+    for name, term in terms:
+        hv = _term_hvp(s, theta, x0, u, term)
+        e = 1e-5 / np.abs(u).max()
+        fd = (_term_forces(s, theta, x0 + e * u, term) - _term_forces(s, theta, x0 - e * u, term)) / (2 * e)
+        err = E.rel(hv, fd)
+        print("hvp %-12s |Hu| %.3e  rel diff to central differences %.2e" % (name, np.linalg.norm(hv), err))
+        assert np.linalg.norm(hv) > 0
+        assert err < 1e-6, (name, err)
+    # the whole of it, as the reverse pass uses it
+    hv = s.hvp(theta, x0, u)
+    fd = s.hvp_fd(theta, x0, u)
+    err = E.rel(hv, fd)
+    print("hvp total rel diff to central differences %.2e" % err)
+    assert err < 1e-6
+
+
+def test_exact_hvp_symmetric(seg):
+    sd, st1, s, x0, v0, theta, tgt = seg
+    rng = np.random.default_rng(12)
+    u, v = rng.normal(0, 1, x0.shape), rng.normal(0, 1, x0.shape)
+    hu, hv = s.hvp(theta, x0, u), s.hvp(theta, x0, v)
+    a, b = float((v * hu).sum()), float((u * hv).sum())
+    scale = np.linalg.norm(v) * np.linalg.norm(hu)
+    print("v.Hu %.12e u.Hv %.12e" % (a, b))
+    assert abs(a - b) <= 1e-10 * scale
+    # linear in u
+    hw = s.hvp(theta, x0, 2.0 * u - 3.0 * v)
+    np.testing.assert_allclose(hw, 2.0 * hu - 3.0 * hv, rtol=0, atol=1e-10 * np.abs(hw).max())
