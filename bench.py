@@ -257,7 +257,7 @@ def _class_shapes(cfg, plan):
     return {"own_cu": {"replicas": plan["own_cu"], "threads": cfg["own_threads"]},
             "three_per_cu": {"replicas": plan["three_per_cu"], "threads": cfg["shared_threads"]},
             "four_per_cu": {"replicas": plan["four_per_cu"], "threads": cfg["shared_threads"]},
-            "dense_run": cfg["dense_run"] or "per replica", "heavy_factor": cfg["heavy_factor"],
+            "dense_run": "per replica (in the kernel)", "heavy_factor": cfg["heavy_factor"],
             "chunk": cfg["chunk"]}
 
 

@@ -270,14 +270,11 @@ int mdn_run_batched(mdn_ctx *c, int32_t nsteps, int32_t eflag_every,
 int mdn_set_batch_shape(mdn_ctx *c, int32_t threads, int32_t lds_cap_atoms, int32_t grid, int32_t chunk);
 /* tuning of the split launch (threads = 0 above): workgroup size of the
  * replicas with a CU of their own (256 or 512, default 512) and of those
- * sharing CUs (64, 128 or 256, default 128); run length of the dense
- * pair-block work items (8, 16 or 32 pairs per lane; 0 = chosen per replica
- * from its block counts, the default); a replica may get a CU of its own
- * when its cost exceeds heavy_factor times the median (default 2).  Results
- * do not depend on these, only the accumulation order of forces does.  No
- * reference counterpart: the autotuner's knobs. */
-int mdn_set_batch_tuning(mdn_ctx *c, int32_t own_threads, int32_t shared_threads, int32_t dense_run,
-                         double heavy_factor);
+ * sharing CUs (64, 128 or 256, default 128); a replica may get a CU of its
+ * own when its cost exceeds heavy_factor times the median (default 2).
+ * Results do not depend on these, only the accumulation order of forces
+ * does.  No reference counterpart: the autotuner's knobs. */
+int mdn_set_batch_tuning(mdn_ctx *c, int32_t own_threads, int32_t shared_threads, double heavy_factor);
 
 /* LDS capacity (in-domain atoms per replica), LDS bytes per workgroup and the
  * number of replicas given a CU of their own, of the last mdn_run_batched

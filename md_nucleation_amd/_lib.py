@@ -93,7 +93,7 @@ _SIGS = {
     "mdn_verlet_graph": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int64, C.c_int32]),
     "mdn_run_batched": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int64, C.c_int32]),
     "mdn_set_batch_shape": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32]),
-    "mdn_set_batch_tuning": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_double]),
+    "mdn_set_batch_tuning": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_double]),
     "mdn_get_counters": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_int64), C.POINTER(C.c_int64), _ip,
                                    C.POINTER(C.c_int64)]),
     "mdn_get_batch_info": (C.c_int, [C.c_void_p, _ip, _ip, _ip]),
@@ -382,9 +382,9 @@ class Engine:
         check(lib().mdn_set_batch_shape(self.h, int(threads), int(lds_cap_atoms), int(grid), int(chunk)),
               "mdn_set_batch_shape")
 
-    def set_batch_tuning(self, own_threads=512, shared_threads=128, dense_run=0, heavy_factor=2.0):
-        check(lib().mdn_set_batch_tuning(self.h, int(own_threads), int(shared_threads), int(dense_run),
-                                         float(heavy_factor)), "mdn_set_batch_tuning")
+    def set_batch_tuning(self, own_threads=512, shared_threads=128, heavy_factor=2.0):
+        check(lib().mdn_set_batch_tuning(self.h, int(own_threads), int(shared_threads), float(heavy_factor)),
+              "mdn_set_batch_tuning")
 
     def counters(self, r):
         """(in-domain atom-steps, rebuilds, max SHAKE iterations, SHAKE failures)"""

@@ -13,13 +13,15 @@ values, the others held at the best so far; two passes):
     256;
   * shared_threads: workgroup size of the replicas that share CUs: 128, 64,
     256;
-  * dense_run: run length of the dense pair-block work items: chosen per
-    replica (0), 8, 16, 32;
   * heavy_factor: the cost, as a multiple of the median, above which a
     replica may get a CU of its own: 2, 1.5, 3.
 When there are no more replicas than CUs the split launch does not engage;
 the workgroup size of the one launch (256, 128, 512, 64) then takes the
-place of the three class knobs.  A candidate replaces the incumbent only
+place of the three class knobs.  The dense pass's run length is not a knob:
+it is chosen per replica from its block counts inside the kernel (a
+run-time override cost the kernel 20-270 spilled registers; forced run
+lengths of 8, 16 and 32 measured 3-16 % slower than the per-replica
+choice).  A candidate replaces the incumbent only
 when it is faster by more than `margin` (relative), so that run-to-run noise
 does not steer the search.
 """
@@ -31,22 +33,19 @@ SPLIT_KNOBS = (
     ("chunk", (1000, 500, 250)),
     ("own_threads", (512, 256)),
     ("shared_threads", (128, 64, 256)),
-    ("dense_run", (0, 8, 16, 32)),
     ("heavy_factor", (2.0, 1.5, 3.0)),
 )
 UNIFORM_KNOBS = (
     ("chunk", (1000, 500, 250)),
     ("threads", (256, 128, 512, 64)),
-    ("dense_run", (0, 8, 16, 32)),
 )
-DEFAULT = {"threads": 0, "chunk": 1000, "own_threads": 512, "shared_threads": 128, "dense_run": 0,
-           "heavy_factor": 2.0}
+DEFAULT = {"threads": 0, "chunk": 1000, "own_threads": 512, "shared_threads": 128, "heavy_factor": 2.0}
 
 
 def apply(eng, cfg):
     """sets a configuration (a dict with DEFAULT's keys) on the engine"""
     eng.set_batch_shape(cfg["threads"], 0, 0, cfg["chunk"])
-    eng.set_batch_tuning(cfg["own_threads"], cfg["shared_threads"], cfg["dense_run"], cfg["heavy_factor"])
+    eng.set_batch_tuning(cfg["own_threads"], cfg["shared_threads"], cfg["heavy_factor"])
 
 
 def _atom_steps(eng, n):

@@ -320,6 +320,116 @@ template <typename Rl> struct Pack4;
 template <> struct Pack4<float> { typedef float4 T; };
 template <> struct Pack4<double> { typedef double4 T; };
 
+// vv for in-domain vapor sets of up to a few thousand atoms: i-centric, j
+// tiles staged through LDS, no atomics on j; grid.y splits the j range so
+// that a launch fills the chip even for a few hundred atoms (the Newton-3
+// kernel below, one wave per 64 x 64 block pair, would leave such a set
+// with a handful of long serial chains).  Each unordered pair is evaluated
+// from both sides, so its energy is halved.  The partner loop has no
+// branches: pairs of one molecule (the atom itself included) take r^2 = 1
+// and weight 0, and a tile is padded to a multiple of four partners with
+// far-away, zero-weight, zero-charge entries, so four partners' LDS reads
+// issue together; a padding partner's energy is masked off.  Per type pair
+// the table holds c0, c1, c0 / 12 and c1 / 6 in pair precision (the same
+// values as dividing per pair).
+
+constexpr int kVVTile = 256;
+template <typename Rl, bool EF>
+__global__ void __launch_bounds__(kVVTile) k_vv_tiled(const DevReplica *__restrict__ Rp, int tv0, int ntv) {
+  const DevReplica &R = *Rp;
+  __shared__ double sx[kVVTile * 3];
+  __shared__ Rl sq[kVVTile], sw[kVVTile];
+  __shared__ int st[kVVTile], sm[kVVTile];
+  __shared__ typename Pack4<Rl>::T sc[64];
+  LView L = view_from_dyn(R);
+  const int nva = L.nv * L.na_v;
+  // vapor-vapor coefficient table (vapor types are contiguous)
+  for (int k = threadIdx.x; k < ntv * ntv; k += blockDim.x) {
+    int a = k / ntv, b = k - a * ntv;
+    const double *p = R.pair_coeff + 2 * ((size_t)(tv0 + a) * R.n_types + tv0 + b);
+    const Rl c0 = (Rl)p[0], c1 = (Rl)p[1];
+    sc[k] = {c0, c1, c0 / Rl(12), c1 / Rl(6)};
+  }
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool act = i < nva;
+  double xi[3] = {0, 0, 0}, qi = 0, wi = 0;
+  int ti = 0, mi = -1, gi = 0;
+  const DevSpecies &V = R.sp[2];
+  if (act) {
+    int k = i / L.na_v, ii = i - k * L.na_v;
+    gi = R.off_vap + (R.in_vap[k] - 1 - R.n_gas) * L.na_v + ii;
+    xi[0] = R.atom_x[3 * (size_t)gi]; xi[1] = R.atom_x[3 * (size_t)gi + 1]; xi[2] = R.atom_x[3 * (size_t)gi + 2];
+    qi = R.qqrd2e * V.charge[ii];
+    wi = R.atom_w[gi];
+    ti = V.type[ii] - tv0;
+    mi = k;
+  }
+  const int trow = ti * ntv;
+  // charge (times qqrd2e) and weight products in pair precision
+  const Rl qr = (Rl)qi, wr = (Rl)wi;
+  int per = (nva + gridDim.y - 1) / gridDim.y;
+  int j0 = blockIdx.y * per, j1 = min(nva, j0 + per);
+  double fx = 0, fy = 0, fz = 0, e = 0;
+  // one partner b; valid: b is a real entry of the tile (uniform)
+  auto pair = [&](int b, bool valid) {
+    const double dx = xi[0] - sx[3 * b], dy = xi[1] - sx[3 * b + 1], dz = xi[2] - sx[3 * b + 2];
+    const bool on = sm[b] != mi;
+    const Rl r2 = (Rl)(dx * dx + dy * dy + dz * dz);
+    const typename Pack4<Rl>::T cc = sc[trow + st[b]];
+    const Rl qq = qr * sq[b], w = wr * sw[b];
+    Rl r2inv, rinv;
+    inv_r<Rl>(on ? r2 : (Rl)1, r2inv, rinv);
+    const Rl r6inv = r2inv * r2inv * r2inv;
+    const Rl flj = r6inv * (cc.x * r6inv - cc.y);
+    const Rl fc = qq * rinv;
+    if (EF) {
+      const double ep = (double)(r6inv * (cc.z * r6inv - cc.w)) + (double)fc;
+      e += on && valid ? ep : 0.0;
+    }
+    // the pair-precision scalar times the FP64 displacement
+    const double fpd = (double)((flj + fc) * r2inv * (on ? w : (Rl)0));
+    fx = fma(fpd, dx, fx); fy = fma(fpd, dy, fy); fz = fma(fpd, dz, fz);
+  };
+  for (int jt = j0; jt < j1; jt += kVVTile) {
+    __syncthreads();
+    const int nj = min(kVVTile, j1 - jt), nj4 = (nj + 3) & ~3;
+    const int b = threadIdx.x;
+    if (b < nj) {
+      const int j = jt + b;
+      int k = j / L.na_v, jj = j - k * L.na_v;
+      int g = R.off_vap + (R.in_vap[k] - 1 - R.n_gas) * L.na_v + jj;
+      sx[3 * b] = R.atom_x[3 * (size_t)g];
+      sx[3 * b + 1] = R.atom_x[3 * (size_t)g + 1];
+      sx[3 * b + 2] = R.atom_x[3 * (size_t)g + 2];
+      sq[b] = (Rl)V.charge[jj];
+      sw[b] = (Rl)R.atom_w[g];
+      st[b] = V.type[jj] - tv0;
+      sm[b] = k;
+    } else if (b < nj4) {
+      sx[3 * b] = 1e6; sx[3 * b + 1] = 1e6; sx[3 * b + 2] = 1e6;
+      sq[b] = 0; sw[b] = 0; st[b] = 0; sm[b] = -2;
+    }
+    __syncthreads();
+    if (act) {
+      for (int b4 = 0; b4 < nj4; b4 += 4) {
+        pair(b4, true);
+        pair(b4 + 1, b4 + 1 < nj);
+        pair(b4 + 2, b4 + 2 < nj);
+        pair(b4 + 3, b4 + 3 < nj);
+      }
+    }
+  }
+  if (act) {
+    atomicAdd(&R.atom_f[3 * (size_t)gi], fx);
+    atomicAdd(&R.atom_f[3 * (size_t)gi + 1], fy);
+    atomicAdd(&R.atom_f[3 * (size_t)gi + 2], fz);
+  }
+  if (EF) {
+    double s = wave_sum(0.5 * e);
+    if ((threadIdx.x & 63) == 0 && s != 0.0) atomicAdd(&R.dyn->U[U_VV], s);
+  }
+}
+
 // vv (VaporVapor.cpp:5-45) for large in-domain vapor sets (e.g. a
 // 2,000-methanol cluster), each unordered pair evaluated once (Newton's
 // third law).  The in-domain vapor atoms are cut into 64-atom blocks and
@@ -563,7 +673,6 @@ struct BatchLds {
   double *kms;      // [kcap] species-atom masses
   ShakeLds shk;     // constraint tables of the ion and the vapor [2][scap]
   int *sint;        // [64]
-  int runl;         // dense work-item run length (8, 16, 32; 0: chosen per replica)
 };
 
 template <typename Rl>
@@ -852,7 +961,6 @@ __device__ inline int dense_n3(const DevReplica &R, const BatchLds<Rl> &B, int n
     const int items = n_diag * (32 / l) + n_off * (64 / l) + n_bc, c = (items + nw - 1) / nw * (l + 6);
     if (c < best) { best = c; L = l; }
   }
-  if (B.runl) L = B.runl;
   const int W = __builtin_amdgcn_readfirstlane(R.amber_words);
   double *__restrict__ pf = B.f;
   double e_ion = 0, e_vi = 0, e_vv = 0;
@@ -1132,10 +1240,9 @@ template <typename Rl>
 __global__ void __launch_bounds__(512) k_batched(const DevReplica *__restrict__ reps,
                                                  const int *__restrict__ order, int n_rep, int nsteps,
                                                  int eflag_every, long long step0, int cap, int cap_h, int kcap,
-                                                 int acap, int scap, int runl) {
+                                                 int acap, int scap) {
   extern __shared__ __align__(16) char smem[];
   BatchLds<Rl> B = carve<Rl>(smem, cap, cap_h, kcap, acap, scap);
-  B.runl = runl;
   const int tid = threadIdx.x, nt = blockDim.x;
   for (int rr = blockIdx.x; rr < n_rep; rr += gridDim.x) {
     const DevReplica &R = reps[order[rr]];
@@ -1471,9 +1578,9 @@ struct mdn_ctx {
   mdn::Dyn *d_dyn = nullptr;
   int last_cap = 0, last_lds = 0, batch_chunk = 1000, batch_split = 1, n_cu = 256;
   // split-launch tuning: workgroup sizes of the own-CU and shared-CU
-  // classes, the dense run length (0: per replica), and the cost (a multiple
-  // of the median) above which a replica may get a CU of its own
-  int own_threads = 512, shared_threads = 128, dense_runl = 0;
+  // classes, and the cost (a multiple of the median) above which a replica
+  // may get a CU of its own
+  int own_threads = 512, shared_threads = 128;
   double heavy_factor = 2.0;
   // direction and result buffers of mdn_force_jvp ([2][jvp_cap] doubles)
   double *d_jvp = nullptr;
@@ -1835,7 +1942,10 @@ static int grid_for(int n, int blk = kBlk, int cap = 2048) {
 static int com_blocks(int natoms) { return std::max(1, std::min(32, (natoms + 1023) / 1024)); }
 
 // every selected potential of one replica through the per-call kernels; vv
-// goes to the Newton-3 all-pairs kernel when the vapor's types fit its table
+// goes to an all-pairs kernel when the vapor's types fit its table: the
+// Newton-3 one when the vapor (in-domain or not) holds at least
+// kVVNewtonMin atoms, the two-sided tiled one below that
+constexpr int kVVNewtonMin = 4096;
 template <typename Rl>
 static int launch_terms(mdn_ctx *c, const HostReplica &h, const DevReplica *R, uint32_t bits, bool eflag) {
   const SysDef &d = h.def;
@@ -1845,12 +1955,20 @@ static int launch_terms(mdn_ctx *c, const HostReplica &h, const DevReplica *R, u
   uint32_t rest = tiled ? bits & ~(uint32_t)MDN_VV : bits;
   if (rest) hipLaunchKernelGGL(k_terms<Rl>, dim3(512), dim3(kBlk), 0, c->stream, R, rest, eflag ? 1 : 0);
   if ((bits & MDN_VV) && tiled) {
-    // one workgroup per super-block pair, for the largest possible
-    // in-domain vapor set
-    const int nsb = std::max(1, (d.n_vap * d.sp[2].natoms + kVVSuper - 1) / kVVSuper);
-    const int gx = nsb * (nsb + 1) / 2;
-    if (eflag) hipLaunchKernelGGL((k_vv_n3<Rl, true>), dim3(gx), dim3(kVVSuper), 0, c->stream, R, tv0, ntv);
-    else hipLaunchKernelGGL((k_vv_n3<Rl, false>), dim3(gx), dim3(kVVSuper), 0, c->stream, R, tv0, ntv);
+    const int nva = d.n_vap * d.sp[2].natoms;  // bound on the in-domain vapor atoms
+    if (nva >= kVVNewtonMin) {
+      // one workgroup per super-block pair, for the largest possible
+      // in-domain vapor set
+      const int nsb = std::max(1, (nva + kVVSuper - 1) / kVVSuper);
+      const int gx = nsb * (nsb + 1) / 2;
+      if (eflag) hipLaunchKernelGGL((k_vv_n3<Rl, true>), dim3(gx), dim3(kVVSuper), 0, c->stream, R, tv0, ntv);
+      else hipLaunchKernelGGL((k_vv_n3<Rl, false>), dim3(gx), dim3(kVVSuper), 0, c->stream, R, tv0, ntv);
+    } else {
+      const int gx = std::max(1, (nva + kVVTile - 1) / kVVTile);
+      const int gy = std::max(1, std::min(64, (4 * c->n_cu + gx - 1) / gx));
+      if (eflag) hipLaunchKernelGGL((k_vv_tiled<Rl, true>), dim3(gx, gy), dim3(kVVTile), 0, c->stream, R, tv0, ntv);
+      else hipLaunchKernelGGL((k_vv_tiled<Rl, false>), dim3(gx, gy), dim3(kVVTile), 0, c->stream, R, tv0, ntv);
+    }
   }
   HIPCHK(hipGetLastError());
   return MDN_OK;
@@ -2523,7 +2641,7 @@ static int launch_batched(mdn_ctx *c, hipStream_t st, const int *order, int n, i
   HIPCHK(hipFuncSetAttribute((const void *)k_batched<Rl>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
   int grid = c->batch_grid > 0 ? std::min(c->batch_grid, n) : n;
   hipLaunchKernelGGL(k_batched<Rl>, dim3(grid), dim3(threads), b.lds, st, c->d_reps, order, n, ns, eflag_every,
-                     step0, b.cap, b.cap_h, b.kcap, b.acap, b.scap, c->dense_runl);
+                     step0, b.cap, b.cap_h, b.kcap, b.acap, b.scap);
   HIPCHK(hipGetLastError());
   return MDN_OK;
 }
@@ -2741,20 +2859,16 @@ int mdn_set_batch_shape(mdn_ctx *c, int32_t threads, int32_t lds_cap_atoms, int3
   return MDN_OK;
 }
 
-int mdn_set_batch_tuning(mdn_ctx *c, int32_t own_threads, int32_t shared_threads, int32_t dense_run,
-                         double heavy_factor) {
+int mdn_set_batch_tuning(mdn_ctx *c, int32_t own_threads, int32_t shared_threads, double heavy_factor) {
   if (!c) return set_error(MDN_E_ARG, "null context");
   if (own_threads != 256 && own_threads != 512)
     return set_error(MDN_E_ARG, "mdn_set_batch_tuning: own_threads must be 256 or 512");
   if (shared_threads != 64 && shared_threads != 128 && shared_threads != 256)
     return set_error(MDN_E_ARG, "mdn_set_batch_tuning: shared_threads must be 64, 128 or 256");
-  if (dense_run != 0 && dense_run != 8 && dense_run != 16 && dense_run != 32)
-    return set_error(MDN_E_ARG, "mdn_set_batch_tuning: dense_run must be 0, 8, 16 or 32");
   if (!(heavy_factor >= 1.0 && heavy_factor <= 100.0))
     return set_error(MDN_E_ARG, "mdn_set_batch_tuning: heavy_factor must lie in [1, 100]");
   c->own_threads = own_threads;
   c->shared_threads = shared_threads;
-  c->dense_runl = dense_run;
   c->heavy_factor = heavy_factor;
   return MDN_OK;
 }

@@ -17,8 +17,8 @@ chunks).
     for identical x1 (x1 depends only on the starting state), relative RMS
     force error <= 1e-5 per replica and over the whole workload, and the
     per-label energy error;
-  * every value of each autotuner knob (workgroup sizes of the classes, dense
-    run length, own-CU threshold, chunk): 300 FP64 steps that agree with the
+  * every value of each autotuner knob (workgroup sizes of the classes,
+    own-CU threshold, chunk): 300 FP64 steps that agree with the
     default launch's to rounding (positions within 1e-9 A).
 """
 import os
@@ -175,8 +175,8 @@ def test_split_launch_mixed_force_error_vs_oracle(workload):
 
 
 KNOB_VALUES = [
-    {"own_threads": 256}, {"shared_threads": 64}, {"shared_threads": 256}, {"dense_run": 8},
-    {"dense_run": 16}, {"dense_run": 32}, {"heavy_factor": 1.5}, {"heavy_factor": 3.0}, {"chunk": 100},
+    {"own_threads": 256}, {"shared_threads": 64}, {"shared_threads": 256}, {"heavy_factor": 1.5},
+    {"heavy_factor": 3.0}, {"chunk": 100},
 ]
 
 
