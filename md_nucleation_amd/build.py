@@ -4,7 +4,9 @@
 """
 from __future__ import annotations
 
+import json
 import os
+import re
 import subprocess
 
 PKG = os.path.dirname(os.path.abspath(__file__))
@@ -19,16 +21,40 @@ FLAGS = ["-O3", "-std=c++17", "-fPIC", "-shared", "--offload-arch=gfx950",
          "-Wno-unused-result", "-I" + os.path.join(ROOT, "include"), "-I" + CSRC]
 
 
+RESOURCES = os.path.join(PKG, "libmdn.resources.json")
+
+
+def kernel_resources(remarks: str) -> dict:
+    """{kernel: {"vgprs", "vgpr_spill", "sgpr_spill", "scratch"}} from the
+    compiler's kernel-resource-usage remarks"""
+    out, name = {}, None
+    for line in remarks.splitlines():
+        m = re.search(r"Function Name: (\S+)", line)
+        if m:
+            name = m.group(1)
+            out[name] = {}
+            continue
+        m = re.search(r"remark: +(VGPRs|VGPRs Spill|SGPRs Spill|ScratchSize \[bytes/lane\]): (\d+)", line)
+        if m and name:
+            key = {"VGPRs": "vgprs", "VGPRs Spill": "vgpr_spill", "SGPRs Spill": "sgpr_spill",
+                   "ScratchSize [bytes/lane]": "scratch"}[m.group(1)]
+            out[name][key] = int(m.group(2))
+    return out
+
+
 def build(verbose: bool = False) -> str:
     srcs = [os.path.join(CSRC, s) for s in SOURCES]
     tmp = LIB + ".tmp"
-    cmd = [HIPCC] + FLAGS + srcs + ["-o", tmp]
+    cmd = [HIPCC] + FLAGS + ["-Rpass-analysis=kernel-resource-usage"] + srcs + ["-o", tmp]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("hipcc failed:\n" + r.stderr[-6000:])
     if verbose and r.stderr:
         print(r.stderr[-3000:])
     os.replace(tmp, LIB)
+    # the register budget of every kernel, for the build checks in tests/
+    with open(RESOURCES, "w") as f:
+        json.dump(kernel_resources(r.stderr), f, indent=1, sort_keys=True)
     return LIB
 
 
