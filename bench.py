@@ -42,10 +42,13 @@ sys.path.insert(0, ROOT)
 METRIC = "atom-steps/s: batched mixed-prec vs plain FP64 GPU at ≤1e-5 rel force err"
 PEAK_FP32_TFLOPS = 157.3
 PEAK_FP64_TFLOPS = 78.6
-# HBM bytes per step of the heavy launch (156 replicas with a CU each), from
-# the FETCH_SIZE / WRITE_SIZE passes in profiles/k_batched_mixed_512replicas.md
-# (copy-calibrated: FETCH_SIZE x 2.0, WRITE_SIZE x 1.0; 31.5 MB per 200 steps)
-HEAVY_HBM_BYTES_PER_STEP = 0.158e6
+# HBM bytes per launch of the own-CU class (156 replicas with a CU each),
+# from the FETCH_SIZE / WRITE_SIZE passes in
+# profiles/k_batched_mixed_512replicas.md (copy-calibrated: FETCH_SIZE x 2.0,
+# WRITE_SIZE x 1.0): 31.5 MB for a 200-step launch and the same for a 1-step
+# one, i.e. the replicas' state read at the launch's start and written at its
+# end; the steps themselves run out of LDS
+HEAVY_HBM_BYTES_PER_LAUNCH = 31.5e6
 # algorithmic FLOPs per unique pair / bonded term, as written in the reference
 FLOP = {"ljcoul_w": 31, "ljcoul": 30, "lj": 24, "bond": 22, "angle": 70, "dihedral": 150}
 
@@ -620,9 +623,10 @@ def main():
         "force_err": force_err,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS,
-                     "traffic": HEAVY_HBM_BYTES_PER_STEP * steps_per_launch,
-                     "traffic_note": "HBM bytes per launch of the own-CU class (%d steps), counter passes in "
-                                     "profiles/k_batched_mixed_512replicas.md" % steps_per_launch,
+                     "traffic": HEAVY_HBM_BYTES_PER_LAUNCH,
+                     "traffic_note": "HBM bytes per launch of the own-CU class (here %d steps; the same for 1 "
+                                     "or 200 steps: the state in and out at the launch edges), counter passes "
+                                     "in profiles/k_batched_mixed_512replicas.md" % steps_per_launch,
                      "kernel": "k_batched<float>", "kernel_ms": kern_ms, "lds_bytes_per_wg": lds_bytes,
                      "replicas_with_own_cu": n_heavy, "lds_cap_atoms": lds_cap, "launch_plan": plan,
                      "flops_per_step": flops},
