@@ -627,7 +627,7 @@ def main():
                      "traffic_note": "HBM bytes per launch of the own-CU class (here %d steps; the same for 1 "
                                      "or 200 steps: the state in and out at the launch edges), counter passes "
                                      "in profiles/k_batched_mixed_512replicas.md" % steps_per_launch,
-                     "kernel": "k_batched<float>", "kernel_ms": kern_ms, "lds_bytes_per_wg": lds_bytes,
+                     "kernel": "k_batched<float, false>", "kernel_ms": kern_ms, "lds_bytes_per_wg": lds_bytes,
                      "replicas_with_own_cu": n_heavy, "lds_cap_atoms": lds_cap, "launch_plan": plan,
                      "flops_per_step": flops},
         "cpu_baseline": cpu,

@@ -1124,7 +1124,11 @@ __device__ int dense_heavy(const DevReplica &R, const BatchLds<Rl> &B, const LVi
 
 // the batched path's force phase: registration order of setPotential.cpp
 // with vi, vv and the AMBER non-bonded pairs in the dense pass
-template <typename Rl>
+// MB: the kernel carries the many-body and BMH ion terms (Tersoff, SW,
+// Born-Mayer-Huggins).  Without them the compiler keeps every value in
+// registers (no spills); a launch takes the MB variant only when one of its
+// replicas selects such a term.
+template <typename Rl, bool MB>
 __device__ void batch_terms(const DevReplica &R, const SStoreT<Rl> &s, const BatchLds<Rl> &B, int t, int nt,
                             bool eflag, double *e) {
   const uint32_t bits = R.flags;
@@ -1133,10 +1137,10 @@ __device__ void batch_terms(const DevReplica &R, const SStoreT<Rl> &s, const Bat
   if (bits & B_GG) term_gg<Rl>(R, s, t, nt, eflag, e);
   if (bits & B_GV) term_gv<Rl>(R, s, t, nt, eflag, e);
   if (bits & B_EFIELD) term_efield(R, s, t, nt);
-  if (bits & B_TERSOFF) term_tersoff(R, s, t, nt, eflag, e);
-  if (bits & B_SW) term_sw(R, s, t, nt, eflag, e);
+  if (MB && (bits & B_TERSOFF)) term_tersoff(R, s, t, nt, eflag, e);
+  if (MB && (bits & B_SW)) term_sw(R, s, t, nt, eflag, e);
   if (bits & B_AMBER) term_amber_bonded(R, s, t, nt, eflag, e);
-  if (bits & B_BMH) term_born(R, s, t, nt, eflag, e);
+  if (MB && (bits & B_BMH)) term_born(R, s, t, nt, eflag, e);
   if (bits & B_VINTRA) term_vintra(R, s, t, nt, eflag, e);
   if (bits & B_GINTRA) term_gintra(R, s, t, nt, eflag, e);
 }
@@ -1144,7 +1148,7 @@ __device__ void batch_terms(const DevReplica &R, const SStoreT<Rl> &s, const Bat
 // The forces of a step whose vapor SHAKE runs beside them: phase A uses
 // only ion and gas positions (threads t of [t0, t0 + nt), whole waves);
 // phase B (all threads, after SHAKE and the vapor weights) the rest.
-template <typename Rl>
+template <typename Rl, bool MB>
 __device__ void batch_terms_a(const DevReplica &R, const SStoreT<Rl> &s, const BatchLds<Rl> &B, int t, int t0, int nt,
                               bool eflag, double *e) {
   const uint32_t bits = R.flags;
@@ -1152,10 +1156,10 @@ __device__ void batch_terms_a(const DevReplica &R, const SStoreT<Rl> &s, const B
   if (bits & B_GI) term_gi<Rl>(R, s, tl, nt, eflag, e);
   dense_heavy<Rl>(R, B, s.L, t, t0, nt, kDenseIonOnly, eflag, e);
   if (bits & B_EFIELD) term_efield(R, s, tl, nt);
-  if (bits & B_TERSOFF) term_tersoff(R, s, tl, nt, eflag, e);
-  if (bits & B_SW) term_sw(R, s, tl, nt, eflag, e);
+  if (MB && (bits & B_TERSOFF)) term_tersoff(R, s, tl, nt, eflag, e);
+  if (MB && (bits & B_SW)) term_sw(R, s, tl, nt, eflag, e);
   if (bits & B_AMBER) term_amber_bonded(R, s, tl, nt, eflag, e);
-  if (bits & B_BMH) term_born(R, s, tl, nt, eflag, e);
+  if (MB && (bits & B_BMH)) term_born(R, s, tl, nt, eflag, e);
   if (bits & B_GINTRA) term_gintra(R, s, tl, nt, eflag, e);
 }
 template <typename Rl>
@@ -1236,7 +1240,7 @@ __device__ inline void batch_store(const DevReplica &R, const LView &L, const Ba
   }
 }
 
-template <typename Rl>
+template <typename Rl, bool MB>
 __global__ void __launch_bounds__(512) k_batched(const DevReplica *__restrict__ reps,
                                                  const int *__restrict__ order, int n_rep, int nsteps,
                                                  int eflag_every, long long step0, int cap, int cap_h, int kcap,
@@ -1410,7 +1414,7 @@ __global__ void __launch_bounds__(512) k_batched(const DevReplica *__restrict__ 
             double dx = B.xr[a].x[0] - c[0], dy = B.xr[a].x[1] - c[1], dz = B.xr[a].x[2] - c[2];
             B.xr[a].w = blend_weight_r<Rl>(dx * dx + dy * dy + dz * dz);
           }
-          batch_terms_a<Rl>(R, ss, B, tid, 0, 64 * nA, eflag, e);
+          batch_terms_a<Rl, MB>(R, ss, B, tid, 0, 64 * nA, eflag, e);
         }
         __syncthreads();
         {
@@ -1486,7 +1490,7 @@ __global__ void __launch_bounds__(512) k_batched(const DevReplica *__restrict__ 
         }
         __syncthreads();
         // forces of every registered potential (verlet.cpp:31-32)
-        batch_terms<Rl>(R, ss, B, tid, nt, eflag, e);
+        batch_terms<Rl, MB>(R, ss, B, tid, nt, eflag, e);
       }
       __syncthreads();
       // second half-kick (verlet.cpp:34), then MD::nosehoover_ion on the ion
@@ -2632,18 +2636,26 @@ static BatchCaps batch_caps(const mdn_ctx *c, const std::vector<Dyn> &dyn, const
 
 template <typename Rl>
 static int launch_batched(mdn_ctx *c, hipStream_t st, const int *order, int n, int threads, int ns, int eflag_every,
-                          long long step0, const BatchCaps &b) {
+                          long long step0, const BatchCaps &b, bool mb) {
   if (n <= 0) return MDN_OK;
   if (b.lds > 160 * 1024)
     return set_error(MDN_E_CAPACITY, "mdn_run_batched: a replica's in-domain atoms exceed one workgroup's LDS; "
                                      "use mdn_verlet for it");
   // launches of different sizes may run concurrently: allow the full LDS
-  HIPCHK(hipFuncSetAttribute((const void *)k_batched<Rl>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+  auto kern = mb ? k_batched<Rl, true> : k_batched<Rl, false>;
+  HIPCHK(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
   int grid = c->batch_grid > 0 ? std::min(c->batch_grid, n) : n;
-  hipLaunchKernelGGL(k_batched<Rl>, dim3(grid), dim3(threads), b.lds, st, c->d_reps, order, n, ns, eflag_every,
-                     step0, b.cap, b.cap_h, b.kcap, b.acap, b.scap);
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(threads), b.lds, st, c->d_reps, order, n, ns, eflag_every, step0,
+                     b.cap, b.cap_h, b.kcap, b.acap, b.scap);
   HIPCHK(hipGetLastError());
   return MDN_OK;
+}
+
+// a launch over these replicas needs the many-body / BMH variant
+static bool needs_manybody(const mdn_ctx *c, const std::vector<int> &set) {
+  for (int r : set)
+    if (c->reps[r].def.flags & (MDN_ION_BMH | MDN_ION_TERSOFF | MDN_ION_SW)) return true;
+  return false;
 }
 }  // namespace mdn
 
@@ -2808,8 +2820,10 @@ int mdn_run_batched(mdn_ctx *c, int32_t nsteps, int32_t eflag_every, int64_t ste
         if (pt.cnt == 0) continue;
         BatchCaps bc = batch_caps(c, dyn, *pt.set, rl, pt.budget);
         rc = precision == MDN_MIXED
-                 ? launch_batched<float>(c, pt.st, c->d_order + pt.off, pt.cnt, pt.threads, ns, eflag_every, s0, bc)
-                 : launch_batched<double>(c, pt.st, c->d_order + pt.off, pt.cnt, pt.threads, ns, eflag_every, s0, bc);
+                 ? launch_batched<float>(c, pt.st, c->d_order + pt.off, pt.cnt, pt.threads, ns, eflag_every, s0, bc,
+                                         needs_manybody(c, *pt.set))
+                 : launch_batched<double>(c, pt.st, c->d_order + pt.off, pt.cnt, pt.threads, ns, eflag_every, s0, bc,
+                                          needs_manybody(c, *pt.set));
         if (rc) return rc;
         lds = std::max(lds, bc.lds);
         cap = std::max(cap, bc.cap);
@@ -2823,8 +2837,10 @@ int mdn_run_batched(mdn_ctx *c, int32_t nsteps, int32_t eflag_every, int64_t ste
     } else {
       BatchCaps b = batch_caps(c, dyn, order, rl, 80 * 1024);
       rc = precision == MDN_MIXED
-               ? launch_batched<float>(c, c->stream, c->d_order, n, c->batch_threads, ns, eflag_every, s0, b)
-               : launch_batched<double>(c, c->stream, c->d_order, n, c->batch_threads, ns, eflag_every, s0, b);
+               ? launch_batched<float>(c, c->stream, c->d_order, n, c->batch_threads, ns, eflag_every, s0, b,
+                                       needs_manybody(c, order))
+               : launch_batched<double>(c, c->stream, c->d_order, n, c->batch_threads, ns, eflag_every, s0, b,
+                                        needs_manybody(c, order));
       if (rc) return rc;
       c->last_cap = b.cap;
       c->last_lds = (int)b.lds;
