@@ -781,15 +781,24 @@ __device__ int shake_group_star(const DevReplica &R, const ShakeLds &T, int s, i
   const int i0 = ij[2 * k0], j0 = ij[2 * k0 + 1];
   const int c = n >= 2 && !(i0 == ij[2 * k0 + 2] || i0 == ij[2 * k0 + 3]) ? j0 : i0;
   const double tolerance = 1e-20, rho_zero = 0.01;
-  // only the positions, which carry the chain from one update to the next,
-  // stay in registers; bond data and old positions are re-read from LDS
-  double xc[3], xs[kStarMax][3];
+  // the positions, which carry the chain from one update to the next, and
+  // every constraint's fixed data (old bond vector, squared length, masses)
+  // sit in registers for the whole iteration: a sweep reads no LDS
+  double xc[3], xs[kStarMax][3], rk[kStarMax][3], dk2[kStarMax], hm[kStarMax], mc[kStarMax], ms[kStarMax];
   for (int q = 0; q < 3; q++) xc[q] = x(c, q);
 #pragma unroll
   for (int k = 0; k < kStarMax; k++) {
     const int kk = k0 + (k < n ? k : 0);
-    const int sk = ij[2 * kk] == c ? ij[2 * kk + 1] : ij[2 * kk];
-    for (int q = 0; q < 3; q++) xs[k][q] = x(sk, q);
+    const bool ci = ij[2 * kk] == c;
+    const int sk = ci ? ij[2 * kk + 1] : ij[2 * kk];
+    for (int q = 0; q < 3; q++) {
+      xs[k][q] = x(sk, q);
+      rk[k][q] = r(c, q) - r(sk, q);
+    }
+    dk2[k] = cc[4 * kk];
+    hm[k] = cc[4 * kk + 3];
+    mc[k] = ci ? cc[4 * kk + 1] : cc[4 * kk + 2];
+    ms[k] = ci ? cc[4 * kk + 2] : cc[4 * kk + 1];
   }
   int it = 0;
   bool converge = false;
@@ -800,34 +809,29 @@ __device__ int shake_group_star(const DevReplica &R, const ShakeLds &T, int s, i
 #pragma unroll
     for (int k = 0; k < kStarMax; k++) {
       if (k >= n) break;
-      const int kk = k0 + k;
-      const bool ci = ij[2 * kk] == c;
-      const int sk = ci ? ij[2 * kk + 1] : ij[2 * kk];
-      const double dk2 = cc[4 * kk], hm = cc[4 * kk + 3];
-      const double mc = ci ? cc[4 * kk + 1] : cc[4 * kk + 2], ms = ci ? cc[4 * kk + 2] : cc[4 * kk + 1];
-      double rk[3], rn[3];
-      for (int q = 0; q < 3; q++) { rk[q] = r(c, q) - r(sk, q); rn[q] = xc[q] - xs[k][q]; }
+      double rn[3];
+      for (int q = 0; q < 3; q++) rn[q] = xc[q] - xs[k][q];
       double rn2 = rn[0] * rn[0] + rn[1] * rn[1] + rn[2] * rn[2];
-      double dot = rk[0] * rn[0] + rk[1] * rn[1] + rk[2] * rn[2];
+      double dot = rk[k][0] * rn[0] + rk[k][1] * rn[1] + rk[k][2] * rn[2];
       if (R.shake != 2) {
-        double d0 = rn2 - dk2;
+        double d0 = rn2 - dk2[k];
         if (d0 * d0 > tolerance) conv = 0;
       }
-      double ramk = (dk2 - rn2) * hm;
+      double ramk = (dk2[k] - rn2) * hm[k];
       if (dot > 1e-10) ramk *= rcp_nr(dot);
       else {
-        double rk2 = rk[0] * rk[0] + rk[1] * rk[1] + rk[2] * rk[2];
-        ramk = 0.5 * rho_zero + ramk / (rk2 * rho_zero) * (mc + ms);
+        double rk2 = rk[k][0] * rk[k][0] + rk[k][1] * rk[k][1] + rk[k][2] * rk[k][2];
+        ramk = 0.5 * rho_zero + ramk / (rk2 * rho_zero) * (mc[k] + ms[k]);
       }
-      double cfc = ramk * mc, cfs = ramk * ms;
+      double cfc = ramk * mc[k], cfs = ramk * ms[k];
       for (int q = 0; q < 3; q++) {
-        double nc = xc[q] + rk[q] * cfc, ns = xs[k][q] - rk[q] * cfs;
+        double nc = xc[q] + rk[k][q] * cfc, ns = xs[k][q] - rk[k][q] * cfs;
         xc[q] = nc;
         xs[k][q] = ns;
         rn[q] = nc - ns;
       }
       rn2 = rn[0] * rn[0] + rn[1] * rn[1] + rn[2] * rn[2];
-      double dr = rn2 - dk2;
+      double dr = rn2 - dk2[k];
       if (!(dr * dr <= 1e30)) { ok = 0; break; }
       if (R.shake == 2 && dr * dr > tolerance) conv = 0;
     }
