@@ -323,6 +323,28 @@ int mdn_vi_jvp(const double *x, const double *w, int32_t S, int32_t n_ion, int32
                const double *ion_sigma, const double *ion_eps, const double *ion_q,
                const double *vap_sigma, const double *vap_eps, const double *vap_q,
                const uint8_t *vap_in, const double *u, double *hu);
+/* Trajectory-segment gradient, every state array in device memory: nsteps
+ * steps of MD::verlet as shipped (verlet.cpp:20-39: half kick, drift,
+ * forces, half kick; no SHAKE, no rebuild, every vapor molecule in the
+ * domain with blend weight 1) of the segment atoms (the ion's atoms, then
+ * every vapor atom in molecule order; N of them) of replica r, from x0, v0
+ * ([N][3] host arrays; NULL = the replica's current ones), with the vi term
+ * (VaporIon.cpp:5-43) taking the ion's per-atom sigma, eps, q as inputs and
+ * the other selected ion and vapor terms as mdn_compute runs them.  Loss:
+ * sum over the observed steps obs[k] (strictly ascending in [0, nsteps]) of
+ * mean over atoms of |x_obs[k] - targets[k]|^2 (targets [n_obs][N][3]).
+ * Writes *loss and, when g_sigma, g_eps, g_q are given (all three or none),
+ * dloss/d(sigma, eps, q) of the ion atoms by a checkpointed reverse pass
+ * (checkpoints every `every` steps; exact Hessian-vector products); x_end,
+ * v_end (may be NULL) receive the last state.  The replica's own state is
+ * left as it was.  No reference counterpart (the reference has no
+ * gradients): the segment gradients of the fitting workload. */
+int mdn_segment_grad(mdn_ctx *c, int32_t replica, int32_t nsteps, int32_t every,
+                     const double *x0, const double *v0,
+                     const double *ion_sigma, const double *ion_eps,
+                     const double *ion_q, const int32_t *obs, int32_t n_obs,
+                     const double *targets, double *loss, double *g_sigma,
+                     double *g_eps, double *g_q, double *x_end, double *v_end);
 /* mdn_vi_adjoint for callers that cannot allocate (code under HIP graph
  * capture, tight fitting loops, harnesses that forbid hipMalloc).  No
  * counterpart in the reference.  Every pointer is a DEVICE pointer on the

@@ -84,6 +84,8 @@ _SIGS = {
     "mdn_set_positions": (C.c_int, [C.c_void_p, C.c_int32, _dp]),
     "mdn_get_forces": (C.c_int, [C.c_void_p, C.c_int32, _dp]),
     "mdn_force_jvp": (C.c_int, [C.c_void_p, C.c_int32, C.c_uint32, _dp, _dp]),
+    "mdn_segment_grad": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, _dp, _dp, _dp, _dp, _dp,
+                                   C.POINTER(C.c_int32), C.c_int32, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
     "mdn_make_pair": (C.c_int, [C.c_void_p, C.c_int32]),
     "mdn_zero_forces": (C.c_int, [C.c_void_p, C.c_int32]),
     "mdn_compute": (C.c_int, [C.c_void_p, C.c_int32, C.c_uint32, C.c_int32, C.c_int32]),
@@ -350,6 +352,43 @@ class Engine:
         check(lib().mdn_force_jvp(self.h, int(r), int(bits), uu.ctypes.data_as(_dp), hu.ctypes.data_as(_dp)),
               "mdn_force_jvp")
         return hu.reshape(-1, 3)
+
+    def segment_grad(self, r, nsteps, every, theta, obs, targets, x0=None, v0=None, grad=True, end=False):
+        """mdn_segment_grad: (loss, (dL/dsigma, dL/deps, dL/dq) or None[, x_end, v_end])
+        of the trajectory loss over ``nsteps`` MD::verlet steps of the
+        replica's ion + vapor atoms, computed on the device; ``obs`` the
+        observed steps (ascending), ``targets`` [len(obs), N, 3]"""
+        sig, eps, q = (np.ascontiguousarray(np.asarray(a, np.float64)) for a in theta)
+        n_ion = sig.size
+        obs = np.ascontiguousarray(np.asarray(obs, np.int32))
+        tg = np.ascontiguousarray(np.asarray(targets, np.float64))
+        N = tg.shape[1] if tg.ndim == 3 and tg.shape[0] else None
+
+        def opt(a):
+            if a is None:
+                return None, None
+            a = np.ascontiguousarray(np.asarray(a, np.float64))
+            return a, a.ctypes.data_as(_dp)
+        x0a, px0 = opt(x0)
+        v0a, pv0 = opt(v0)
+        if N is None:
+            N = (x0a.size // 3) if x0a is not None else None
+        loss = C.c_double(0)
+        gs = [np.zeros(n_ion) for _ in range(3)] if grad else None
+        pg = [g.ctypes.data_as(_dp) for g in gs] if grad else [None] * 3
+        xe = ve = None
+        if end:
+            if N is None:
+                raise ValueError("segment_grad: give x0 or targets to size the end state")
+            xe, ve = np.zeros((N, 3)), np.zeros((N, 3))
+        check(lib().mdn_segment_grad(self.h, int(r), int(nsteps), int(every), px0, pv0,
+                                     sig.ctypes.data_as(_dp), eps.ctypes.data_as(_dp), q.ctypes.data_as(_dp),
+                                     obs.ctypes.data_as(C.POINTER(C.c_int32)), int(obs.size),
+                                     tg.ctypes.data_as(_dp) if obs.size else None, C.byref(loss), *pg,
+                                     xe.ctypes.data_as(_dp) if end else None,
+                                     ve.ctypes.data_as(_dp) if end else None), "mdn_segment_grad")
+        out = (loss.value, tuple(gs) if grad else None)
+        return out + (xe, ve) if end else out
 
     def make_pair(self, r):
         check(lib().mdn_make_pair(self.h, int(r)), "mdn_make_pair")

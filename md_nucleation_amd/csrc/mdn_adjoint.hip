@@ -28,15 +28,6 @@ namespace mdn {
 
 extern "C" double mdn_qqrd2e(void);
 
-struct ViBatch {
-  const double *x, *w;  // [S][n][3], [S][n]; atoms: n_ion ion atoms, then n_vap vapor atoms
-  int S, n_ion, n_vap;
-  const double *sig_i, *eps_i, *q_i;  // [n_ion]
-  const double *sig_v, *eps_v, *q_v;  // [n_vap]
-  const uint8_t *vin;                 // [S][n_vap] vapor atom in the domain, or null (all)
-  double K;                           // qqrd2e
-};
-
 __device__ inline bool vap_in(const ViBatch &b, int s, int j) {
   return !b.vin || b.vin[(size_t)s * b.n_vap + j];
 }
@@ -213,6 +204,38 @@ __global__ void k_vi_sum_g(const double *part, int S, int n_ion, double *g_sig, 
     a += p[0]; b += p[1]; c += p[2];
   }
   g_sig[i] = a; g_eps[i] = b; g_q[i] = c;
+}
+
+// the kernels of the three entry points on device buffers, queued on a
+// stream (the trajectory-segment gradient drives them from device memory)
+int vi_forward_dev(const ViBatch &b, double *F, double *Epart, double *E, void *stream) {
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const int ni = b.S * b.n_ion, nv = b.S * b.n_vap;
+  hipLaunchKernelGGL(k_vi_fwd_ion, dim3((ni + 255) / 256), dim3(256), 0, st, b, F, Epart);
+  if (nv) hipLaunchKernelGGL(k_vi_fwd_vap, dim3((nv + 255) / 256), dim3(256), 0, st, b, F);
+  if (E) hipLaunchKernelGGL(k_vi_sum_e, dim3((b.S + 255) / 256), dim3(256), 0, st, Epart, b.S, b.n_ion, E);
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? MDN_OK : set_error(MDN_E_HIP, std::string("vi forward: ") + hipGetErrorString(e));
+}
+
+int vi_jvp_dev(const ViBatch &b, const double *U, double *HU, void *stream) {
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const int ni = b.S * b.n_ion, nv = b.S * b.n_vap;
+  hipLaunchKernelGGL(k_vi_jvp_ion, dim3((ni + 255) / 256), dim3(256), 0, st, b, U, HU);
+  if (nv) hipLaunchKernelGGL(k_vi_jvp_vap, dim3((nv + 255) / 256), dim3(256), 0, st, b, U, HU);
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? MDN_OK : set_error(MDN_E_HIP, std::string("vi jvp: ") + hipGetErrorString(e));
+}
+
+int vi_adjoint_dev(const ViBatch &b, const double *gE, const double *gF, double *part, double *g_sigma,
+                   double *g_eps, double *g_q, void *stream) {
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const int ni = b.S * b.n_ion;
+  hipLaunchKernelGGL(k_vi_adj, dim3((ni + 255) / 256), dim3(256), 0, st, b, gE, gF, part);
+  hipLaunchKernelGGL(k_vi_sum_g, dim3((b.n_ion + 255) / 256), dim3(256), 0, st, part, b.S, b.n_ion, g_sigma, g_eps,
+                     g_q);
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? MDN_OK : set_error(MDN_E_HIP, std::string("vi adjoint: ") + hipGetErrorString(e));
 }
 
 static int check_vi(const double *x, const double *w, int S, int n_ion, int n_vap, const double *si, const double *ei,

@@ -2943,3 +2943,396 @@ int mdn_last_kernel_ms(mdn_ctx *c, double *ms) {
 }
 
 }  // extern "C"
+
+// ======================= trajectory-segment gradient =====================
+// Reverse-mode gradient of a trajectory loss over one segment of MD::verlet
+// (verlet.cpp:20-39 as shipped: half kick, drift, forces, half kick; no
+// SHAKE, no rebuild within the segment, blend weights 1) with respect to the
+// ion's per-atom sigma, eps, q of the vi term (VaporIon.cpp:5-43), every
+// array in device memory.  The segment's atoms are the ion atoms and every
+// vapor atom (molecule order); their positions reach the replica's atom
+// arrays for the per-call force kernels of the other terms, and the vi term
+// runs over the segment's own arrays with the parameters as inputs.
+//
+// One step n -> n + 1:
+//   vh = v_n + (c F(x_n)) / m,  x_{n+1} = x_n + dt vh,  v_{n+1} = vh + (c F(x_{n+1})) / m
+// with c = 0.5 dt 4.184e-4 (verlet.cpp:49).  With a_x, a_v the adjoints of
+// x_{n+1}, v_{n+1}, one step back is
+//   a_v += dt a_x;  u = a_v / m;  a_x += k c H(x_n) u (+ dL/dx_n);
+//   g += k c (dF(x_n)/dtheta)^T u
+// (k = 2 for n > 0: x_n enters two half kicks with the same u; k = 1 for
+// n = 0).  H(x) u is the derivative of the forces along u: the vi term's
+// closed form and the dual-number instantiation of the per-call force
+// kernels (k_force_jvp).  The forward pass keeps (x, v) at the start of each
+// checkpoint interval; the reverse pass recomputes an interval's positions
+// from its checkpoint, runs its steps back, and takes the interval's
+// parameter gradient in one adjoint call over its frames.
+namespace mdn {
+
+constexpr int kSegBlk = 256;
+
+static int seg_grid(size_t n) { return (int)std::max<size_t>(1, (n + kSegBlk - 1) / kSegBlk); }
+
+// dst[3 idx[a] + q] = src[3 a + q]
+__global__ void k_seg_scatter(const int *__restrict__ idx, int n, const double *__restrict__ src,
+                              double *__restrict__ dst) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 3 * n) return;
+  const int a = t / 3, q = t - 3 * a;
+  dst[3 * (size_t)idx[a] + q] = src[t];
+}
+
+// dst[3 a + q] = src[3 idx[a] + q], or += with add
+__global__ void k_seg_gather(const int *__restrict__ idx, int n, const double *__restrict__ src,
+                             double *__restrict__ dst, int add) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 3 * n) return;
+  const int a = t / 3, q = t - 3 * a;
+  const double s = src[3 * (size_t)idx[a] + q];
+  dst[t] = add ? dst[t] + s : s;
+}
+
+// half kick: out = v + (c F) / m
+__global__ void k_seg_kick(int n, const double *__restrict__ v, const double *__restrict__ F,
+                           const double *__restrict__ m, double c, double *__restrict__ out) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 3 * n) return;
+  out[t] = v[t] + c * F[t] / m[t / 3];
+}
+
+// drift: out = x + dt vh
+__global__ void k_seg_drift(int n3, const double *__restrict__ x, const double *__restrict__ vh, double dt,
+                            double *__restrict__ out) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n3) out[t] = x[t] + dt * vh[t];
+}
+
+// out[0] = sum of (x - y)^2 over n3 values, in a fixed order (one workgroup)
+__global__ void __launch_bounds__(kSegBlk) k_seg_sqdist(int n3, const double *__restrict__ x,
+                                                        const double *__restrict__ y, double *__restrict__ out) {
+  __shared__ double s[kSegBlk];
+  double acc = 0;
+  for (int t = threadIdx.x; t < n3; t += kSegBlk) {
+    const double d = x[t] - y[t];
+    acc += d * d;
+  }
+  s[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = kSegBlk / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = s[0];
+}
+
+// a step back, first half: a_v += dt a_x; u = a_v / m; us = kc u
+__global__ void k_seg_back_u(int n, double *__restrict__ a_v, const double *__restrict__ a_x,
+                             const double *__restrict__ m, double dt, double kc, double *__restrict__ u,
+                             double *__restrict__ us) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 3 * n) return;
+  const double av = a_v[t] + dt * a_x[t];
+  a_v[t] = av;
+  const double uu = av / m[t / 3];
+  u[t] = uu;
+  us[t] = kc * uu;
+}
+
+// a step back, second half: a_x += kc H u, then + 2 (x - target) / N when
+// the step is observed (target != null)
+__global__ void k_seg_back_x(int n3, double *__restrict__ a_x, const double *__restrict__ hu, double kc,
+                             const double *__restrict__ x, const double *__restrict__ tgt, double N) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n3) return;
+  double a = a_x[t] + kc * hu[t];
+  if (tgt) a = a + 2.0 * (x[t] - tgt[t]) / N;
+  a_x[t] = a;
+}
+
+// a_x = 2 (x - target) / N
+__global__ void k_seg_dloss(int n3, const double *__restrict__ x, const double *__restrict__ tgt, double N,
+                            double *__restrict__ a_x) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n3) a_x[t] = 2.0 * (x[t] - tgt[t]) / N;
+}
+
+__global__ void k_seg_add(int n, double *__restrict__ g, const double *__restrict__ d) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) g[t] += d[t];
+}
+
+__global__ void k_seg_fill(size_t n, double *__restrict__ p, double v) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) p[t] = v;
+}
+
+// device buffers and launch helpers of one mdn_segment_grad call
+struct Segment {
+  mdn_ctx *c = nullptr;
+  const HostReplica *h = nullptr;
+  const DevReplica *Rd = nullptr;  // device copy of the replica's descriptor
+  DevReplica R;                    // host copy (its device pointers)
+  int n_ion = 0, n_vap = 0, N = 0, n_atoms = 0;
+  uint32_t rest = 0;  // terms other than vi (mdn_compute bits)
+  bool vi = false;
+  char *buf = nullptr;
+  int *idx = nullptr;
+  double *mass = nullptr, *ones = nullptr, *theta = nullptr, *vap = nullptr, *epart = nullptr;
+  double *gU = nullptr, *gHU = nullptr;  // direction and result in the replica's global atom order
+  ~Segment() {
+    if (buf) (void)hipFree(buf);
+  }
+  ViBatch batch(const double *x, int S) const {
+    ViBatch b;
+    b.x = x; b.w = ones; b.S = S; b.n_ion = n_ion; b.n_vap = n_vap;
+    b.sig_i = theta; b.eps_i = theta + n_ion; b.q_i = theta + 2 * n_ion;
+    b.sig_v = vap; b.eps_v = vap + n_vap; b.q_v = vap + 2 * n_vap;
+    b.vin = nullptr;
+    b.K = mdn_qqrd2e();
+    return b;
+  }
+  // F = forces of the selected terms at the segment positions x
+  int forces(const double *x, double *F) {
+    const int g = seg_grid(3 * (size_t)N);
+    hipLaunchKernelGGL(k_seg_scatter, dim3(g), dim3(kSegBlk), 0, c->stream, idx, N, x, R.atom_x);
+    if (rest) {
+      HIPCHK(hipMemsetAsync(R.atom_f, 0, sizeof(double) * 3 * (size_t)n_atoms, c->stream));
+      int rc = launch_terms<double>(c, *h, Rd, rest, false);
+      if (rc) return rc;
+    }
+    if (vi) {
+      int rc = vi_forward_dev(batch(x, 1), F, epart, nullptr, c->stream);
+      if (rc) return rc;
+    } else {
+      HIPCHK(hipMemsetAsync(F, 0, sizeof(double) * 3 * (size_t)N, c->stream));
+    }
+    if (rest) hipLaunchKernelGGL(k_seg_gather, dim3(g), dim3(kSegBlk), 0, c->stream, idx, N, R.atom_f, F, 1);
+    HIPCHK(hipGetLastError());
+    return MDN_OK;
+  }
+  // hu = H(x) u, the derivative of the forces along u
+  int hvp(const double *x, const double *u, double *hu) {
+    const int g = seg_grid(3 * (size_t)N);
+    if (vi) {
+      int rc = vi_jvp_dev(batch(x, 1), u, hu, c->stream);
+      if (rc) return rc;
+    } else {
+      HIPCHK(hipMemsetAsync(hu, 0, sizeof(double) * 3 * (size_t)N, c->stream));
+    }
+    const uint32_t bits = rest & ~(uint32_t)MDN_EFIELD;  // the Efield force does not depend on x
+    if (bits) {
+      hipLaunchKernelGGL(k_seg_scatter, dim3(g), dim3(kSegBlk), 0, c->stream, idx, N, x, R.atom_x);
+      hipLaunchKernelGGL(k_seg_scatter, dim3(g), dim3(kSegBlk), 0, c->stream, idx, N, u, gU);
+      HIPCHK(hipMemsetAsync(gHU, 0, sizeof(double) * 3 * (size_t)n_atoms, c->stream));
+      hipLaunchKernelGGL(k_force_jvp, dim3(512), dim3(kBlk), 0, c->stream, Rd, bits, gU, gHU);
+      hipLaunchKernelGGL(k_seg_gather, dim3(g), dim3(kSegBlk), 0, c->stream, idx, N, gHU, hu, 1);
+    }
+    HIPCHK(hipGetLastError());
+    return MDN_OK;
+  }
+  // one MD::verlet step: (x, v, F) -> (xn, vn, F); vh is scratch
+  int step(const double *x, const double *v, double *F, double *vh, double *xn, double *vn, double cc, double dt) {
+    const int g = seg_grid(3 * (size_t)N);
+    hipLaunchKernelGGL(k_seg_kick, dim3(g), dim3(kSegBlk), 0, c->stream, N, v, F, mass, cc, vh);
+    hipLaunchKernelGGL(k_seg_drift, dim3(g), dim3(kSegBlk), 0, c->stream, 3 * N, x, vh, dt, xn);
+    int rc = forces(xn, F);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_seg_kick, dim3(g), dim3(kSegBlk), 0, c->stream, N, vh, F, mass, cc, vn);
+    HIPCHK(hipGetLastError());
+    return MDN_OK;
+  }
+};
+
+}  // namespace mdn
+
+extern "C" {
+
+int mdn_segment_grad(mdn_ctx *c, int32_t r, int32_t nsteps, int32_t every, const double *x0, const double *v0,
+                     const double *ion_sigma, const double *ion_eps, const double *ion_q, const int32_t *obs,
+                     int32_t n_obs, const double *targets, double *loss, double *g_sigma, double *g_eps,
+                     double *g_q, double *x_end, double *v_end) {
+  int rc = check_replica(c, r);
+  if (rc) return rc;
+  if ((rc = refuse_pending(c, r, "mdn_segment_grad"))) return rc;
+  if (nsteps < 0 || every < 1) return set_error(MDN_E_ARG, "mdn_segment_grad: nsteps >= 0 and every >= 1 required");
+  if (!ion_sigma || !ion_eps || !ion_q || !loss) return set_error(MDN_E_ARG, "mdn_segment_grad: null argument");
+  if (n_obs < 0 || (n_obs && (!obs || !targets)))
+    return set_error(MDN_E_ARG, "mdn_segment_grad: observed steps need obs and targets");
+  const bool grad = g_sigma || g_eps || g_q;
+  if (grad && !(g_sigma && g_eps && g_q))
+    return set_error(MDN_E_ARG, "mdn_segment_grad: give all three gradient outputs or none");
+  std::vector<int> tgt_of((size_t)nsteps + 1, -1);
+  for (int k = 0; k < n_obs; k++) {
+    if (obs[k] < 0 || obs[k] > nsteps || (k && obs[k] <= obs[k - 1]))
+      return set_error(MDN_E_ARG, "mdn_segment_grad: obs must ascend strictly within [0, nsteps]");
+    tgt_of[obs[k]] = k;
+  }
+  HIPCHK(hipSetDevice(c->device));
+  const HostReplica &h = c->reps[r];
+  const SysDef &d = h.def;
+  if (d.shake) return set_error(MDN_E_ARG, "mdn_segment_grad: the segment follows MD::verlet without SHAKE");
+  const uint32_t nodiff = MDN_ION_BMH | MDN_ION_TERSOFF | MDN_ION_SW;
+  if (grad && (d.flags & nodiff))
+    return set_error(MDN_E_ARG, "mdn_segment_grad: no derivative for the BMH, Tersoff or SW ion terms");
+  Segment sg;
+  sg.c = c; sg.h = &h; sg.Rd = c->d_reps + r; sg.R = h.dev;
+  sg.n_ion = d.sp[0].natoms;
+  sg.n_vap = d.n_vap * d.sp[2].natoms;
+  sg.N = sg.n_ion + sg.n_vap;
+  sg.n_atoms = d.n_atoms;
+  sg.vi = (d.flags & MDN_VI) != 0;
+  sg.rest = d.flags & (MDN_ION_AMBER | MDN_ION_BMH | MDN_ION_TERSOFF | MDN_ION_SW | MDN_VV | MDN_VAPOR_INTRA |
+                       MDN_EFIELD);
+  const int N = sg.N, N3 = 3 * N, ni = sg.n_ion, nv = sg.n_vap;
+  const int g0 = d.sp[0].natoms + d.n_gas * d.sp[1].natoms;  // first vapor atom
+  std::vector<int> idx(N);
+  for (int a = 0; a < ni; a++) idx[a] = a;
+  for (int a = 0; a < nv; a++) idx[ni + a] = g0 + a;
+  // the segment's conditions on the replica's current state: every vapor
+  // molecule in the domain, blend weight 1 on every segment atom
+  {
+    std::vector<uint8_t> reg(d.n_mol);
+    std::vector<double> w(d.n_atoms);
+    HIPCHK(hipMemcpyAsync(reg.data(), h.dev.region, d.n_mol, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(w.data(), h.dev.atom_w, sizeof(double) * d.n_atoms, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int m = 1 + d.n_gas; m < d.n_mol; m++)
+      if (!reg[m]) return set_error(MDN_E_ARG, "mdn_segment_grad: every vapor molecule must be in the domain");
+    for (int a = 0; a < N; a++)
+      if (w[idx[a]] != 1.0)
+        return set_error(MDN_E_ARG, "mdn_segment_grad: every segment atom must have blend weight 1");
+  }
+  const int nint = (nsteps + every - 1) / every;
+  const int S = std::max(1, std::min(every, std::max(nsteps, 1)));  // frames per interval
+  // one allocation, carved twice: first to size it, then for real
+  double *xa, *xb, *vv, *vh, *F, *u, *hu, *ax, *av, *ck, *frames, *us, *tg, *gsum, *gint, *lp, *save_x, *save_f;
+  size_t used = 0;
+  auto carve = [&](double *base) {
+    used = 0;
+    auto take = [&](size_t n) { double *q = base ? base + used : nullptr; used += n; return q; };
+    xa = take(N3); xb = take(N3); vv = take(N3); vh = take(N3); F = take(N3);
+    u = take(N3); hu = take(N3); ax = take(N3); av = take(N3);
+    ck = take((size_t)2 * nint * N3);
+    frames = take((size_t)S * N3); us = take((size_t)S * N3);
+    tg = take((size_t)n_obs * N3);
+    sg.mass = take(N);
+    sg.ones = take((size_t)S * N);
+    sg.theta = take(3 * (size_t)ni);
+    sg.vap = take(3 * (size_t)nv);
+    sg.epart = take(3 * (size_t)S * ni);
+    gsum = take(3 * (size_t)ni); gint = take(3 * (size_t)ni);
+    lp = take((size_t)nsteps + 1);
+    sg.gU = take(3 * (size_t)d.n_atoms); sg.gHU = take(3 * (size_t)d.n_atoms);
+    save_x = take(3 * (size_t)d.n_atoms); save_f = take(3 * (size_t)d.n_atoms);
+  };
+  carve(nullptr);
+  const size_t nd = used;
+  HIPCHK(hipMalloc((void **)&sg.buf, sizeof(double) * nd + sizeof(int) * (size_t)N));
+  carve(reinterpret_cast<double *>(sg.buf));
+  sg.idx = reinterpret_cast<int *>(sg.buf + sizeof(double) * nd);
+  // host tables: masses, the vapor's per-atom parameters (type sigma, eps;
+  // charge), the ion parameters
+  std::vector<double> hm(N), hv(3 * (size_t)nv), ht(3 * (size_t)ni);
+  for (int a = 0; a < ni; a++) hm[a] = d.sp[0].mass[a];
+  const Species &V = d.sp[2];
+  for (int a = 0; a < nv; a++) {
+    const int k = a % V.natoms;
+    hm[ni + a] = V.mass[k];
+    hv[a] = d.t_sig[V.type[k]];
+    hv[nv + a] = d.t_eps[V.type[k]];
+    hv[2 * nv + a] = V.charge[k];
+  }
+  for (int a = 0; a < ni; a++) { ht[a] = ion_sigma[a]; ht[ni + a] = ion_eps[a]; ht[2 * ni + a] = ion_q[a]; }
+  hipStream_t st = c->stream;
+  HIPCHK(hipMemcpyAsync(sg.idx, idx.data(), sizeof(int) * N, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(sg.mass, hm.data(), sizeof(double) * N, hipMemcpyHostToDevice, st));
+  if (nv) HIPCHK(hipMemcpyAsync(sg.vap, hv.data(), sizeof(double) * 3 * nv, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(sg.theta, ht.data(), sizeof(double) * 3 * ni, hipMemcpyHostToDevice, st));
+  if (n_obs) HIPCHK(hipMemcpyAsync(tg, targets, sizeof(double) * N3 * (size_t)n_obs, hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(k_seg_fill, dim3(seg_grid((size_t)S * N)), dim3(kSegBlk), 0, st, (size_t)S * N, sg.ones, 1.0);
+  hipLaunchKernelGGL(k_seg_fill, dim3(seg_grid(3 * (size_t)d.n_atoms)), dim3(kSegBlk), 0, st,
+                     3 * (size_t)d.n_atoms, sg.gU, 0.0);
+  // the replica's positions and forces are restored at the end
+  HIPCHK(hipMemcpyAsync(save_x, h.dev.atom_x, sizeof(double) * 3 * d.n_atoms, hipMemcpyDeviceToDevice, st));
+  HIPCHK(hipMemcpyAsync(save_f, h.dev.atom_f, sizeof(double) * 3 * d.n_atoms, hipMemcpyDeviceToDevice, st));
+  const int gN = seg_grid(3 * (size_t)N);
+  if (x0) HIPCHK(hipMemcpyAsync(xa, x0, sizeof(double) * N3, hipMemcpyHostToDevice, st));
+  else hipLaunchKernelGGL(k_seg_gather, dim3(gN), dim3(kSegBlk), 0, st, sg.idx, N, h.dev.atom_x, xa, 0);
+  if (v0) HIPCHK(hipMemcpyAsync(vv, v0, sizeof(double) * N3, hipMemcpyHostToDevice, st));
+  else hipLaunchKernelGGL(k_seg_gather, dim3(gN), dim3(kSegBlk), 0, st, sg.idx, N, h.dev.atom_v, vv, 0);
+  const double dt = d.dt, cc = 0.5 * d.dt * kKickUnit;
+  auto tgt = [&](int step) -> const double * { return tgt_of[step] >= 0 ? tg + (size_t)tgt_of[step] * N3 : nullptr; };
+  auto sqd = [&](const double *x, int step) {
+    if (tgt(step)) hipLaunchKernelGGL(k_seg_sqdist, dim3(1), dim3(kSegBlk), 0, st, N3, x, tgt(step), lp + step);
+  };
+  // forward: checkpoints at the start of every interval
+  double *x = xa, *xn = xb;
+  if ((rc = sg.forces(x, F))) return rc;
+  sqd(x, 0);
+  for (int i = 0; i < nint; i++) {
+    const int k0 = i * every, n = std::min(every, nsteps - k0);
+    HIPCHK(hipMemcpyAsync(ck + (size_t)2 * i * N3, x, sizeof(double) * N3, hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemcpyAsync(ck + (size_t)(2 * i + 1) * N3, vv, sizeof(double) * N3, hipMemcpyDeviceToDevice, st));
+    for (int j = 1; j <= n; j++) {
+      if ((rc = sg.step(x, vv, F, vh, xn, vv, cc, dt))) return rc;
+      std::swap(x, xn);
+      sqd(x, k0 + j);
+    }
+  }
+  if (x_end) HIPCHK(hipMemcpyAsync(x_end, x, sizeof(double) * N3, hipMemcpyDeviceToHost, st));
+  if (v_end) HIPCHK(hipMemcpyAsync(v_end, vv, sizeof(double) * N3, hipMemcpyDeviceToHost, st));
+  // reverse, one checkpoint interval at a time
+  if (grad) {
+    const size_t n3 = N3;
+    hipLaunchKernelGGL(k_seg_fill, dim3(seg_grid(n3)), dim3(kSegBlk), 0, st, n3, av, 0.0);
+    hipLaunchKernelGGL(k_seg_fill, dim3(seg_grid(3 * (size_t)ni)), dim3(kSegBlk), 0, st, 3 * (size_t)ni, gsum, 0.0);
+    if (tgt(nsteps)) hipLaunchKernelGGL(k_seg_dloss, dim3(gN), dim3(kSegBlk), 0, st, N3, x, tgt(nsteps), (double)N, ax);
+    else hipLaunchKernelGGL(k_seg_fill, dim3(seg_grid(n3)), dim3(kSegBlk), 0, st, n3, ax, 0.0);
+    for (int i = nint - 1; i >= 0; i--) {
+      const int k0 = i * every, n = std::min(every, nsteps - k0);
+      // positions x_{k0} .. x_{k0+n-1} into frame slots n-1 .. 0 (the
+      // reverse order in which the steps are taken back)
+      auto slot = [&](int j) { return frames + (size_t)(n - 1 - j) * N3; };
+      HIPCHK(hipMemcpyAsync(slot(0), ck + (size_t)2 * i * N3, sizeof(double) * N3, hipMemcpyDeviceToDevice, st));
+      if (n > 1) {
+        HIPCHK(hipMemcpyAsync(vv, ck + (size_t)(2 * i + 1) * N3, sizeof(double) * N3, hipMemcpyDeviceToDevice, st));
+        if ((rc = sg.forces(slot(0), F))) return rc;
+        for (int j = 1; j < n; j++)
+          if ((rc = sg.step(slot(j - 1), vv, F, vh, slot(j), vv, cc, dt))) return rc;
+      }
+      for (int j = n - 1; j >= 0; j--) {
+        const int stp = k0 + j;
+        const double kc = (stp > 0 ? 2.0 : 1.0) * cc;
+        double *usl = us + (size_t)(n - 1 - j) * N3;
+        hipLaunchKernelGGL(k_seg_back_u, dim3(gN), dim3(kSegBlk), 0, st, N, av, ax, sg.mass, dt, kc, u, usl);
+        if ((rc = sg.hvp(slot(j), u, hu))) return rc;
+        hipLaunchKernelGGL(k_seg_back_x, dim3(gN), dim3(kSegBlk), 0, st, N3, ax, hu, kc, slot(j), tgt(stp),
+                           (double)N);
+      }
+      if (sg.vi) {
+        if ((rc = vi_adjoint_dev(sg.batch(frames, n), nullptr, us, sg.epart, gint, gint + ni, gint + 2 * ni, st)))
+          return rc;
+        hipLaunchKernelGGL(k_seg_add, dim3(seg_grid(3 * (size_t)ni)), dim3(kSegBlk), 0, st, 3 * ni, gsum, gint);
+      }
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(g_sigma, gsum, sizeof(double) * ni, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(g_eps, gsum + ni, sizeof(double) * ni, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(g_q, gsum + 2 * ni, sizeof(double) * ni, hipMemcpyDeviceToHost, st));
+  }
+  std::vector<double> parts((size_t)nsteps + 1, 0.0);
+  HIPCHK(hipMemcpyAsync(parts.data(), lp, sizeof(double) * (nsteps + 1), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(h.dev.atom_x, save_x, sizeof(double) * 3 * d.n_atoms, hipMemcpyDeviceToDevice, st));
+  HIPCHK(hipMemcpyAsync(h.dev.atom_f, save_f, sizeof(double) * 3 * d.n_atoms, hipMemcpyDeviceToDevice, st));
+  HIPCHK(hipStreamSynchronize(st));
+  // the loss in step order (step 0 last, as its term was added after the
+  // segment's)
+  double L = 0;
+  for (int s = 1; s <= nsteps; s++)
+    if (tgt_of[s] >= 0) L += parts[s] / N;
+  if (tgt_of[0] >= 0) L += parts[0] / N;
+  *loss = L;
+  return MDN_OK;
+}
+
+}  // extern "C"

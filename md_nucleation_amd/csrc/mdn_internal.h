@@ -58,6 +58,26 @@ void init_state(const SysDef &d, uint64_t seed, mdn_state *st);
 
 int set_error(int code, const std::string &msg);
 
+// the vi term over S snapshots with per-atom ion parameters (device
+// pointers; mdn_adjoint.hip)
+struct ViBatch {
+  const double *x, *w;  // [S][n][3], [S][n]; atoms: n_ion ion atoms, then n_vap vapor atoms
+  int S, n_ion, n_vap;
+  const double *sig_i, *eps_i, *q_i;  // [n_ion]
+  const double *sig_v, *eps_v, *q_v;  // [n_vap]
+  const uint8_t *vin;                 // [S][n_vap] vapor atom in the domain, or null (all)
+  double K;                           // qqrd2e
+};
+// kernels queued on `stream` (a hipStream_t): F [S][n][3] written (not
+// accumulated), Epart [S][n_ion] scratch, E [S] (may be null)
+int vi_forward_dev(const ViBatch &b, double *F, double *Epart, double *E, void *stream);
+// HU [S][n][3] written: the derivative of the forces along U
+int vi_jvp_dev(const ViBatch &b, const double *U, double *HU, void *stream);
+// g_sigma, g_eps, g_q [n_ion] written from dL/dE (gE, may be null) and dL/dF
+// (gF [S][n][3], may be null); part [S][n_ion][3] scratch
+int vi_adjoint_dev(const ViBatch &b, const double *gE, const double *gF, double *part, double *g_sigma,
+                   double *g_eps, double *g_q, void *stream);
+
 // engine facts the host run driver reads (committed context)
 int engine_replicas(const mdn_ctx *c);  // -1 when not committed
 const SysDef &engine_sysdef(const mdn_ctx *c, int r);

@@ -6,12 +6,17 @@ differentiation through the integrator with checkpoints.
 Dynamics: MD::verlet as the reference ships it (verlet.cpp:20-39): half
 kick, drift, forces at the new positions, half kick; no SHAKE (the call is
 commented out, verlet.cpp:24), no rebuild within a segment, blend weights 1
-(every vapor molecule within 100 A of the ion).  Positions and velocities of
-the ion and vapor atoms are integrated in FP64 on the host; forces come from
-libmdn on the GPU: the vi term through ``mdn_vi_forward`` with the ion
-parameters as inputs, every other selected term (AMBER, vv, vapor intra,
-Efield) through ``mdn_compute`` on the replica's device state
-(``mdn_set_positions`` / ``mdn_get_forces``).
+(every vapor molecule within 100 A of the ion).  The vi term takes the ion
+parameters as inputs; every other selected term (AMBER, vv, vapor intra,
+Efield) runs as ``mdn_compute`` runs it.
+
+``loss_and_grad`` (and ``TrajectoryLoss``, its ``torch.autograd.Function``)
+run the whole segment on the device through one C-ABI call,
+``mdn_segment_grad``: state, checkpoints, recomputation, Hessian-vector
+products and the parameter adjoint all in device memory.  ``Integrator`` /
+``loss_and_grad_host`` are the same reverse pass written over host numpy
+arrays, calling the device per force evaluation (``Segment``) or a numpy
+pair model (the CPU tests); they check the device path.
 
 Reverse pass.  One step n -> n + 1 is
     vh = v_n + c F(x_n) / m,  x_{n+1} = x_n + dt vh,  v_{n+1} = vh + c F(x_{n+1}) / m
@@ -24,14 +29,13 @@ v_{n+1}, a step back is
 where k = 2 for 0 < n (x_n enters the second half kick of step n - 1 and the
 first of step n with the same u) and k = 1 for n = 0.  H(x) u, the Hessian
 of -U applied to u, is the derivative of the forces along u, computed
-exactly (to FP64 rounding) on the GPU: ``mdn_vi_jvp`` (closed form) for the
-vi term with the parameters theta, ``mdn_force_jvp`` (the force kernels
-instantiated with forward-mode dual numbers) for AMBER, vv and vapor intra;
-the Efield force does not depend on x.  (dF/dtheta)^T u is
-``mdn_vi_adjoint`` with gF = u (the hand-written adjoint kernel), one call
-for all steps of a checkpoint interval.  The forward pass keeps (x, v) every
-``every`` steps; the reverse pass recomputes each interval's states from its
-checkpoint.
+exactly (to FP64 rounding): the vi term's closed form (``mdn_vi_jvp``) with
+the parameters theta, and the force kernels instantiated with forward-mode
+dual numbers (``mdn_force_jvp``) for AMBER, vv and vapor intra; the Efield
+force does not depend on x.  (dF/dtheta)^T u is the hand-written vi adjoint
+with gF = u, one call for all steps of a checkpoint interval.  The forward
+pass keeps (x, v) every ``every`` steps; the reverse pass recomputes each
+interval's states from its checkpoint.
 """
 from __future__ import annotations
 
@@ -178,7 +182,35 @@ def trajectory_loss(xs, targets, obs):
 
 def loss_and_grad(seg, theta, x0, v0, nsteps, targets, obs, every=10):
     """(L, (dL/dsigma, dL/deps, dL/dq)) of the trajectory loss over one
-    segment, reverse mode with checkpoints every ``every`` steps"""
+    segment of a ``Segment``, on the device (``mdn_segment_grad``): reverse
+    mode with checkpoints every ``every`` steps"""
+    obs, tg = _targets(seg, x0, v0, targets, obs)
+    return seg.eng.segment_grad(0, nsteps, every, theta, obs, tg, x0=x0, v0=v0)
+
+
+def _targets(seg, x0, v0, targets, obs):
+    """observed steps in ascending order and their targets [n_obs, N, 3],
+    shapes checked against the segment (the C ABI trusts them)"""
+    N = seg.idx.size
+    for a in (x0, v0):
+        if np.shape(a) != (N, 3):
+            raise ValueError("segment state must be [%d, 3]" % N)
+    obs = sorted(obs)
+    tg = np.stack([np.asarray(targets[n], np.float64) for n in obs]) if obs else np.zeros((0, N, 3))
+    if tg.shape != (len(obs), N, 3):
+        raise ValueError("targets must be [%d, 3] per observed step" % N)
+    return obs, tg
+
+
+def segment_loss(seg, theta, x0, v0, nsteps, targets, obs):
+    """the trajectory loss alone, on the device"""
+    obs, tg = _targets(seg, x0, v0, targets, obs)
+    return seg.eng.segment_grad(0, nsteps, max(nsteps, 1), theta, obs, tg, x0=x0, v0=v0, grad=False)[0]
+
+
+def loss_and_grad_host(seg, theta, x0, v0, nsteps, targets, obs, every=10):
+    """``loss_and_grad`` over host arrays: the same reverse pass with an
+    ``Integrator``'s forces, Hessian-vector products and parameter adjoint"""
     theta = tuple(np.asarray(a, np.float64) for a in theta)
     # forward: checkpoints only
     ck = {0: (np.array(x0, np.float64), np.array(v0, np.float64))}
@@ -226,7 +258,7 @@ def loss_and_grad(seg, theta, x0, v0, nsteps, targets, obs, every=10):
     return L, tuple(g)
 
 
-def segment_loss(seg, theta, x0, v0, nsteps, targets, obs):
+def segment_loss_host(seg, theta, x0, v0, nsteps, targets, obs):
     xs, _, _ = seg.run(theta, x0, v0, nsteps)
     return trajectory_loss(xs, targets, obs)[0]
 

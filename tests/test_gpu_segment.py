@@ -6,8 +6,11 @@ MD::verlet ships it (no SHAKE), after 100 warm-up steps.  Loss: mean squared
 distance of the ion and water atoms to a target trajectory (same start, ion
 parameters perturbed by +-10%) at steps 10, 20, ..., 100.
 
-  dynamics   the segment's host-integrated steps (FP64, libmdn forces) ==
-             the oracle's MD::verlet from the same state (1e-10)
+  dynamics   the segment's steps on the device (mdn_segment_grad) and the
+             host integrator's (FP64, libmdn forces) == the oracle's
+             MD::verlet from the same state (1e-10)
+  device     the device reverse pass == the same reverse pass over host
+             arrays (loss_and_grad_host); the replica's state is untouched
   gradient   the checkpointed reverse pass == central differences (h = 1e-6)
              of the GPU segment loss, and == central differences of the
              oracle's own 100-step trajectory loss with the perturbed sigma
@@ -101,6 +104,37 @@ def test_segment_dynamics_is_md_verlet(seg):
     moved = np.abs(xs[-1] - x0).max()
     print("segment: max displacement %.3f A over %d steps" % (moved, T))
     assert moved > 0.1
+
+
+def test_device_segment_dynamics_is_md_verlet(seg):
+    sd, st1, s, x0, v0, theta, tgt = seg
+    ox = _oracle_run(sd, st1, T)
+    L, g, xe, ve = s.eng.segment_grad(0, T, EVERY, theta, [], np.zeros((0, x0.shape[0], 3)), x0=x0, v0=v0,
+                                      grad=False, end=True)
+    assert L == 0 and g is None
+    err = E.rel(xe, ox[-1][s.idx])
+    print("device segment: rel |x_T - x_T(oracle)| = %.2e" % err)
+    assert err < 1e-10
+    xs, vs, _ = s.run(theta, x0, v0, T)
+    assert E.rel(xe, xs[-1]) < 1e-12 and E.rel(ve, vs[-1]) < 1e-12
+
+
+def test_device_matches_host_reverse_pass(seg):
+    sd, st1, s, x0, v0, theta, tgt = seg
+    before = s.eng.get_state(0)
+    Ld, gd = SG.loss_and_grad(s, theta, x0, v0, T, tgt, OBS, every=EVERY)
+    Lh, gh = SG.loss_and_grad_host(s, theta, x0, v0, T, tgt, OBS, every=EVERY)
+    print("loss device %.12e host %.12e" % (Ld, Lh))
+    assert abs(Ld - Lh) <= 1e-10 * Lh
+    for a, b in zip(gd, gh):
+        np.testing.assert_allclose(a, b, rtol=1e-8, atol=1e-10 * np.abs(b).max())
+    # mdn_segment_grad leaves the replica's own state as it found it (the
+    # host path moved its positions, so compare against a fresh call's)
+    s.eng.set_state(0, before)
+    SG.loss_and_grad(s, theta, x0, v0, T, tgt, OBS, every=EVERY)
+    after = s.eng.get_state(0)
+    np.testing.assert_array_equal(after.atom_x, before.atom_x)
+    np.testing.assert_array_equal(after.atom_v, before.atom_v)
 
 
 def test_gradient_vs_gpu_finite_differences(seg):

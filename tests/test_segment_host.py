@@ -67,7 +67,7 @@ def _setup():
 
 def test_reverse_pass_matches_central_differences():
     t, x0, v0, theta, tgt, obs = _setup()
-    L, g = SG.loss_and_grad(t, theta, x0, v0, 30, tgt, obs, every=7)
+    L, g = SG.loss_and_grad_host(t, theta, x0, v0, 30, tgt, obs, every=7)
     assert L > 0
     h = 1e-6
     worst = 0.0
@@ -77,7 +77,7 @@ def test_reverse_pass_matches_central_differences():
             tm = [a.copy() for a in theta]
             tp[p][k] += h
             tm[p][k] -= h
-            fd = (SG.segment_loss(t, tp, x0, v0, 30, tgt, obs) - SG.segment_loss(t, tm, x0, v0, 30, tgt, obs)) / (2 * h)
+            fd = (SG.segment_loss_host(t, tp, x0, v0, 30, tgt, obs) - SG.segment_loss_host(t, tm, x0, v0, 30, tgt, obs)) / (2 * h)
             scale = max(abs(fd), 1e-3 * max(abs(a).max() for a in g))
             worst = max(worst, abs(g[p][k] - fd) / scale)
     assert worst < 1e-5, worst
@@ -85,9 +85,9 @@ def test_reverse_pass_matches_central_differences():
 
 def test_checkpoint_spacing_does_not_change_the_gradient():
     t, x0, v0, theta, tgt, obs = _setup()
-    L1, g1 = SG.loss_and_grad(t, theta, x0, v0, 30, tgt, obs, every=1)
-    L2, g2 = SG.loss_and_grad(t, theta, x0, v0, 30, tgt, obs, every=10)
-    L3, g3 = SG.loss_and_grad(t, theta, x0, v0, 30, tgt, obs, every=30)
+    L1, g1 = SG.loss_and_grad_host(t, theta, x0, v0, 30, tgt, obs, every=1)
+    L2, g2 = SG.loss_and_grad_host(t, theta, x0, v0, 30, tgt, obs, every=10)
+    L3, g3 = SG.loss_and_grad_host(t, theta, x0, v0, 30, tgt, obs, every=30)
     assert L1 == L2 == L3
     for a, b, c in zip(g1, g2, g3):
         np.testing.assert_allclose(a, b, rtol=1e-12, atol=1e-15)
