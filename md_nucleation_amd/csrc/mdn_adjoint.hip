@@ -13,8 +13,8 @@
 //     dL/dtheta_i = sum_s sum_j [ (gF_j - gF_i) . d  df/dtheta_i
 //                                 + gE[s] dE_pair/dtheta_i ]
 // for theta in {sigma, eps, q} of ion atom i.  Every output element is owned
-// by one thread (i-centric over partners, then a fixed-order sum over
-// snapshots), so results do not depend on scheduling.
+// by one team of lanes (i-centric over partners, a fixed reduction tree, then
+// a fixed-order sum over snapshots), so results do not depend on scheduling.
 #include <hip/hip_runtime.h>
 
 #include <mutex>
@@ -66,19 +66,39 @@ __device__ inline double vi_df_dr2(double r2, double s, double e, double qi, dou
   return (-7.0 * c0 * r6inv * r6inv * r4inv + 4.0 * c1 * r6inv * r4inv - 1.5 * K * qi * qj * rinv * r4inv) * w;
 }
 
+// Work layout of the kernels below: a team of T lanes (a power of two up to
+// the wave's 64, aligned within the wave) owns one (snapshot, atom) and
+// strides over that atom's partners; the team's partial sums are reduced by
+// lane exchanges in a fixed tree, so results do not depend on scheduling.
+// The host picks T from the partner count (about eight partners per lane),
+// so a one-atom ion's vapor side runs one lane per atom and a 150-atom
+// chain's ion side a whole wave per atom.
+__device__ inline double team_sum(double v, int T) {
+  for (int o = T >> 1; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
+// the team's (snapshot, atom) index and lane; false for a team past n
+__device__ inline bool team_of(int n, int T, int &k, int &lane) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  k = t / T;
+  lane = t - k * T;
+  return k < n;
+}
+
 // derivative of the pair forces along the displacement u of the atoms (the
 // Hessian of -E applied to u): with d = x_j - x_i and du = u_j - u_i,
 //   dF_j = f du + 2 (d . du) (df/dr^2) d,   dF_i = -dF_j
-// ion side: one thread per (snapshot, ion atom)
-__global__ void k_vi_jvp_ion(ViBatch b, const double *U, double *HU) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= b.S * b.n_ion) return;
-  int s = t / b.n_ion, i = t - s * b.n_ion, n = b.n_ion + b.n_vap;
+// ion side: one team per (snapshot, ion atom)
+__global__ void k_vi_jvp_ion(ViBatch b, const double *U, double *HU, int T) {
+  int k, lane;
+  const bool on = team_of(b.S * b.n_ion, T, k, lane);
+  const int s = on ? k / b.n_ion : 0, i = on ? k - s * b.n_ion : 0, n = b.n_ion + b.n_vap;
   const double *X = b.x + (size_t)s * n * 3, *W = b.w + (size_t)s * n, *Us = U + (size_t)s * n * 3;
-  double xi = X[3 * i], yi = X[3 * i + 1], zi = X[3 * i + 2];
-  double ux = Us[3 * i], uy = Us[3 * i + 1], uz = Us[3 * i + 2];
+  const double xi = X[3 * i], yi = X[3 * i + 1], zi = X[3 * i + 2];
+  const double ux = Us[3 * i], uy = Us[3 * i + 1], uz = Us[3 * i + 2];
   double hx = 0, hy = 0, hz = 0;
-  for (int j = 0; j < b.n_vap; j++) {
+  for (int j = lane; on && j < b.n_vap; j += T) {
     if (!vap_in(b, s, j)) continue;
     int a = b.n_ion + j;
     double dx = X[3 * a] - xi, dy = X[3 * a + 1] - yi, dz = X[3 * a + 2] - zi;
@@ -89,20 +109,24 @@ __global__ void k_vi_jvp_ion(ViBatch b, const double *U, double *HU) {
     double g = 2.0 * (dx * ex + dy * ey + dz * ez) * vi_df_dr2(r2, sg, ep, b.q_i[i], b.q_v[j], b.K, w);
     hx -= f * ex + g * dx; hy -= f * ey + g * dy; hz -= f * ez + g * dz;
   }
-  double *H = HU + ((size_t)s * n + i) * 3;
-  H[0] = hx; H[1] = hy; H[2] = hz;
+  hx = team_sum(hx, T); hy = team_sum(hy, T); hz = team_sum(hz, T);
+  if (on && lane == 0) {
+    double *H = HU + ((size_t)s * n + i) * 3;
+    H[0] = hx; H[1] = hy; H[2] = hz;
+  }
 }
 
-// vapor side: one thread per (snapshot, vapor atom)
-__global__ void k_vi_jvp_vap(ViBatch b, const double *U, double *HU) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= b.S * b.n_vap) return;
-  int s = t / b.n_vap, j = t - s * b.n_vap, n = b.n_ion + b.n_vap, a = b.n_ion + j;
+// vapor side: one team per (snapshot, vapor atom)
+__global__ void k_vi_jvp_vap(ViBatch b, const double *U, double *HU, int T) {
+  int k, lane;
+  const bool on = team_of(b.S * b.n_vap, T, k, lane);
+  const int s = on ? k / b.n_vap : 0, j = on ? k - s * b.n_vap : 0, n = b.n_ion + b.n_vap, a = b.n_ion + j;
   const double *X = b.x + (size_t)s * n * 3, *W = b.w + (size_t)s * n, *Us = U + (size_t)s * n * 3;
-  double xj = X[3 * a], yj = X[3 * a + 1], zj = X[3 * a + 2];
-  double ux = Us[3 * a], uy = Us[3 * a + 1], uz = Us[3 * a + 2];
+  const double xj = X[3 * a], yj = X[3 * a + 1], zj = X[3 * a + 2];
+  const double ux = Us[3 * a], uy = Us[3 * a + 1], uz = Us[3 * a + 2];
+  const bool in = on && vap_in(b, s, j);
   double hx = 0, hy = 0, hz = 0;
-  for (int i = 0; i < b.n_ion && vap_in(b, s, j); i++) {
+  for (int i = lane; in && i < b.n_ion; i += T) {
     double dx = xj - X[3 * i], dy = yj - X[3 * i + 1], dz = zj - X[3 * i + 2];
     double ex = ux - Us[3 * i], ey = uy - Us[3 * i + 1], ez = uz - Us[3 * i + 2];
     double sg = (b.sig_i[i] + b.sig_v[j]) * 0.5, ep = sqrt(b.eps_i[i] * b.eps_v[j]);
@@ -111,20 +135,23 @@ __global__ void k_vi_jvp_vap(ViBatch b, const double *U, double *HU) {
     double g = 2.0 * (dx * ex + dy * ey + dz * ez) * vi_df_dr2(r2, sg, ep, b.q_i[i], b.q_v[j], b.K, w);
     hx += f * ex + g * dx; hy += f * ey + g * dy; hz += f * ez + g * dz;
   }
-  double *H = HU + ((size_t)s * n + a) * 3;
-  H[0] = hx; H[1] = hy; H[2] = hz;
+  hx = team_sum(hx, T); hy = team_sum(hy, T); hz = team_sum(hz, T);
+  if (on && lane == 0) {
+    double *H = HU + ((size_t)s * n + a) * 3;
+    H[0] = hx; H[1] = hy; H[2] = hz;
+  }
 }
 
-// forward, ion side: one thread per (snapshot, ion atom): F_i and the
+// forward, ion side: one team per (snapshot, ion atom): F_i and the
 // snapshot's energy partial of atom i
-__global__ void k_vi_fwd_ion(ViBatch b, double *F, double *Epart) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= b.S * b.n_ion) return;
-  int s = t / b.n_ion, i = t - s * b.n_ion, n = b.n_ion + b.n_vap;
+__global__ void k_vi_fwd_ion(ViBatch b, double *F, double *Epart, int T) {
+  int k, lane;
+  const bool on = team_of(b.S * b.n_ion, T, k, lane);
+  const int s = on ? k / b.n_ion : 0, i = on ? k - s * b.n_ion : 0, n = b.n_ion + b.n_vap;
   const double *X = b.x + (size_t)s * n * 3, *W = b.w + (size_t)s * n;
-  double xi = X[3 * i], yi = X[3 * i + 1], zi = X[3 * i + 2];
+  const double xi = X[3 * i], yi = X[3 * i + 1], zi = X[3 * i + 2];
   double fx = 0, fy = 0, fz = 0, E = 0;
-  for (int j = 0; j < b.n_vap; j++) {
+  for (int j = lane; on && j < b.n_vap; j += T) {
     if (!vap_in(b, s, j)) continue;
     int a = b.n_ion + j;
     double dx = X[3 * a] - xi, dy = X[3 * a + 1] - yi, dz = X[3 * a + 2] - zi;
@@ -133,27 +160,34 @@ __global__ void k_vi_fwd_ion(ViBatch b, double *F, double *Epart) {
     fx -= p.f * dx; fy -= p.f * dy; fz -= p.f * dz;
     E += p.e;
   }
-  double *Fi = F + ((size_t)s * n + i) * 3;
-  Fi[0] = fx; Fi[1] = fy; Fi[2] = fz;
-  Epart[t] = E;
+  fx = team_sum(fx, T); fy = team_sum(fy, T); fz = team_sum(fz, T); E = team_sum(E, T);
+  if (on && lane == 0) {
+    double *Fi = F + ((size_t)s * n + i) * 3;
+    Fi[0] = fx; Fi[1] = fy; Fi[2] = fz;
+    Epart[k] = E;
+  }
 }
 
-// forward, vapor side: one thread per (snapshot, vapor atom): F_j
-__global__ void k_vi_fwd_vap(ViBatch b, double *F) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= b.S * b.n_vap) return;
-  int s = t / b.n_vap, j = t - s * b.n_vap, n = b.n_ion + b.n_vap, a = b.n_ion + j;
+// forward, vapor side: one team per (snapshot, vapor atom): F_j
+__global__ void k_vi_fwd_vap(ViBatch b, double *F, int T) {
+  int k, lane;
+  const bool on = team_of(b.S * b.n_vap, T, k, lane);
+  const int s = on ? k / b.n_vap : 0, j = on ? k - s * b.n_vap : 0, n = b.n_ion + b.n_vap, a = b.n_ion + j;
   const double *X = b.x + (size_t)s * n * 3, *W = b.w + (size_t)s * n;
-  double xj = X[3 * a], yj = X[3 * a + 1], zj = X[3 * a + 2];
+  const double xj = X[3 * a], yj = X[3 * a + 1], zj = X[3 * a + 2];
+  const bool in = on && vap_in(b, s, j);
   double fx = 0, fy = 0, fz = 0;
-  for (int i = 0; i < b.n_ion && vap_in(b, s, j); i++) {
+  for (int i = lane; in && i < b.n_ion; i += T) {
     double dx = xj - X[3 * i], dy = yj - X[3 * i + 1], dz = zj - X[3 * i + 2];
     double sg = (b.sig_i[i] + b.sig_v[j]) * 0.5, ep = sqrt(b.eps_i[i] * b.eps_v[j]);
     PairD p = vi_pair(dx * dx + dy * dy + dz * dz, sg, ep, b.q_i[i], b.q_v[j], b.K, W[i] * W[a]);
     fx += p.f * dx; fy += p.f * dy; fz += p.f * dz;
   }
-  double *Fj = F + ((size_t)s * n + a) * 3;
-  Fj[0] = fx; Fj[1] = fy; Fj[2] = fz;
+  fx = team_sum(fx, T); fy = team_sum(fy, T); fz = team_sum(fz, T);
+  if (on && lane == 0) {
+    double *Fj = F + ((size_t)s * n + a) * 3;
+    Fj[0] = fx; Fj[1] = fy; Fj[2] = fz;
+  }
 }
 
 // E[s] = sum over ion atoms of the partials (fixed order)
@@ -165,19 +199,19 @@ __global__ void k_vi_sum_e(const double *Epart, int S, int n_ion, double *E) {
   E[s] = e;
 }
 
-// adjoint: one thread per (snapshot, ion atom): the snapshot's contribution
+// adjoint: one team per (snapshot, ion atom): the snapshot's contribution
 // to dL/d(sigma_i, eps_i, q_i)
-__global__ void k_vi_adj(ViBatch b, const double *gE, const double *gF, double *part) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= b.S * b.n_ion) return;
-  int s = t / b.n_ion, i = t - s * b.n_ion, n = b.n_ion + b.n_vap;
+__global__ void k_vi_adj(ViBatch b, const double *gE, const double *gF, double *part, int T) {
+  int k, lane;
+  const bool on = team_of(b.S * b.n_ion, T, k, lane);
+  const int s = on ? k / b.n_ion : 0, i = on ? k - s * b.n_ion : 0, n = b.n_ion + b.n_vap;
   const double *X = b.x + (size_t)s * n * 3, *W = b.w + (size_t)s * n;
   const double *G = gF ? gF + (size_t)s * n * 3 : nullptr;
   const double ge = gE ? gE[s] : 0.0;
-  double xi = X[3 * i], yi = X[3 * i + 1], zi = X[3 * i + 2];
-  double gix = G ? G[3 * i] : 0, giy = G ? G[3 * i + 1] : 0, giz = G ? G[3 * i + 2] : 0;
+  const double xi = X[3 * i], yi = X[3 * i + 1], zi = X[3 * i + 2];
+  const double gix = G ? G[3 * i] : 0, giy = G ? G[3 * i + 1] : 0, giz = G ? G[3 * i + 2] : 0;
   double gs = 0, ge_ = 0, gq = 0;
-  for (int j = 0; j < b.n_vap; j++) {
+  for (int j = lane; on && j < b.n_vap; j += T) {
     if (!vap_in(b, s, j)) continue;
     int a = b.n_ion + j;
     double dx = X[3 * a] - xi, dy = X[3 * a + 1] - yi, dz = X[3 * a + 2] - zi;
@@ -191,8 +225,11 @@ __global__ void k_vi_adj(ViBatch b, const double *gE, const double *gF, double *
     ge_ += (gf * p.df_de + ge * p.de_de) * de;
     gq += gf * p.df_dq + ge * p.de_dq;
   }
-  double *o = part + (size_t)t * 3;
-  o[0] = gs; o[1] = ge_; o[2] = gq;
+  gs = team_sum(gs, T); ge_ = team_sum(ge_, T); gq = team_sum(gq, T);
+  if (on && lane == 0) {
+    double *o = part + (size_t)k * 3;
+    o[0] = gs; o[1] = ge_; o[2] = gq;
+  }
 }
 
 __global__ void k_vi_sum_g(const double *part, int S, int n_ion, double *g_sig, double *g_eps, double *g_q) {
@@ -206,13 +243,23 @@ __global__ void k_vi_sum_g(const double *part, int S, int n_ion, double *g_sig, 
   g_sig[i] = a; g_eps[i] = b; g_q[i] = c;
 }
 
+// team size for atoms with `partners` partners each: about eight partners
+// per lane, a power of two from 1 to 64
+static int team_size(int partners) {
+  int T = 1;
+  while (T < 64 && 8 * T < partners) T <<= 1;
+  return T;
+}
+// grid of 256-thread workgroups for `teams` teams of T lanes
+static dim3 team_grid(int teams, int T) { return dim3((unsigned)(((size_t)teams * T + 255) / 256)); }
+
 // the kernels of the three entry points on device buffers, queued on a
 // stream (the trajectory-segment gradient drives them from device memory)
 int vi_forward_dev(const ViBatch &b, double *F, double *Epart, double *E, void *stream) {
   hipStream_t st = static_cast<hipStream_t>(stream);
-  const int ni = b.S * b.n_ion, nv = b.S * b.n_vap;
-  hipLaunchKernelGGL(k_vi_fwd_ion, dim3((ni + 255) / 256), dim3(256), 0, st, b, F, Epart);
-  if (nv) hipLaunchKernelGGL(k_vi_fwd_vap, dim3((nv + 255) / 256), dim3(256), 0, st, b, F);
+  const int ni = b.S * b.n_ion, nv = b.S * b.n_vap, Ti = team_size(b.n_vap), Tv = team_size(b.n_ion);
+  hipLaunchKernelGGL(k_vi_fwd_ion, team_grid(ni, Ti), dim3(256), 0, st, b, F, Epart, Ti);
+  if (nv) hipLaunchKernelGGL(k_vi_fwd_vap, team_grid(nv, Tv), dim3(256), 0, st, b, F, Tv);
   if (E) hipLaunchKernelGGL(k_vi_sum_e, dim3((b.S + 255) / 256), dim3(256), 0, st, Epart, b.S, b.n_ion, E);
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? MDN_OK : set_error(MDN_E_HIP, std::string("vi forward: ") + hipGetErrorString(e));
@@ -220,9 +267,9 @@ int vi_forward_dev(const ViBatch &b, double *F, double *Epart, double *E, void *
 
 int vi_jvp_dev(const ViBatch &b, const double *U, double *HU, void *stream) {
   hipStream_t st = static_cast<hipStream_t>(stream);
-  const int ni = b.S * b.n_ion, nv = b.S * b.n_vap;
-  hipLaunchKernelGGL(k_vi_jvp_ion, dim3((ni + 255) / 256), dim3(256), 0, st, b, U, HU);
-  if (nv) hipLaunchKernelGGL(k_vi_jvp_vap, dim3((nv + 255) / 256), dim3(256), 0, st, b, U, HU);
+  const int ni = b.S * b.n_ion, nv = b.S * b.n_vap, Ti = team_size(b.n_vap), Tv = team_size(b.n_ion);
+  hipLaunchKernelGGL(k_vi_jvp_ion, team_grid(ni, Ti), dim3(256), 0, st, b, U, HU, Ti);
+  if (nv) hipLaunchKernelGGL(k_vi_jvp_vap, team_grid(nv, Tv), dim3(256), 0, st, b, U, HU, Tv);
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? MDN_OK : set_error(MDN_E_HIP, std::string("vi jvp: ") + hipGetErrorString(e));
 }
@@ -230,8 +277,8 @@ int vi_jvp_dev(const ViBatch &b, const double *U, double *HU, void *stream) {
 int vi_adjoint_dev(const ViBatch &b, const double *gE, const double *gF, double *part, double *g_sigma,
                    double *g_eps, double *g_q, void *stream) {
   hipStream_t st = static_cast<hipStream_t>(stream);
-  const int ni = b.S * b.n_ion;
-  hipLaunchKernelGGL(k_vi_adj, dim3((ni + 255) / 256), dim3(256), 0, st, b, gE, gF, part);
+  const int ni = b.S * b.n_ion, Ti = team_size(b.n_vap);
+  hipLaunchKernelGGL(k_vi_adj, team_grid(ni, Ti), dim3(256), 0, st, b, gE, gF, part, Ti);
   hipLaunchKernelGGL(k_vi_sum_g, dim3((b.n_ion + 255) / 256), dim3(256), 0, st, part, b.S, b.n_ion, g_sigma, g_eps,
                      g_q);
   hipError_t e = hipGetLastError();
@@ -312,9 +359,9 @@ int mdn_vi_forward(const double *x, const double *w, int32_t S, int32_t n_ion, i
   double *dE, *dF, *work;
   if ((rc = db.alloc(S, &dE)) || (rc = db.alloc((size_t)S * n * 3, &dF)) || (rc = db.alloc((size_t)S * n_ion, &work)))
     return rc;
-  int ni = S * n_ion, nv = S * n_vap;
-  hipLaunchKernelGGL(k_vi_fwd_ion, dim3((ni + 255) / 256), dim3(256), 0, 0, b, dF, work);
-  if (nv) hipLaunchKernelGGL(k_vi_fwd_vap, dim3((nv + 255) / 256), dim3(256), 0, 0, b, dF);
+  int ni = S * n_ion, nv = S * n_vap, Ti = team_size(n_vap), Tv = team_size(n_ion);
+  hipLaunchKernelGGL(k_vi_fwd_ion, team_grid(ni, Ti), dim3(256), 0, 0, b, dF, work, Ti);
+  if (nv) hipLaunchKernelGGL(k_vi_fwd_vap, team_grid(nv, Tv), dim3(256), 0, 0, b, dF, Tv);
   hipLaunchKernelGGL(k_vi_sum_e, dim3((S + 255) / 256), dim3(256), 0, 0, work, S, n_ion, dE);
   if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess)
     return set_error(MDN_E_HIP, "mdn_vi_forward: kernel failed");
@@ -339,9 +386,9 @@ int mdn_vi_jvp(const double *x, const double *w, int32_t S, int32_t n_ion, int32
   const double *dU = nullptr;
   double *dH = nullptr;
   if ((rc = db.up(u, (size_t)S * n * 3, &dU)) || (rc = db.alloc((size_t)S * n * 3, &dH))) return rc;
-  int ni = S * n_ion, nv = S * n_vap;
-  hipLaunchKernelGGL(k_vi_jvp_ion, dim3((ni + 255) / 256), dim3(256), 0, 0, b, dU, dH);
-  if (nv) hipLaunchKernelGGL(k_vi_jvp_vap, dim3((nv + 255) / 256), dim3(256), 0, 0, b, dU, dH);
+  int ni = S * n_ion, nv = S * n_vap, Ti = team_size(n_vap), Tv = team_size(n_ion);
+  hipLaunchKernelGGL(k_vi_jvp_ion, team_grid(ni, Ti), dim3(256), 0, 0, b, dU, dH, Ti);
+  if (nv) hipLaunchKernelGGL(k_vi_jvp_vap, team_grid(nv, Tv), dim3(256), 0, 0, b, dU, dH, Tv);
   if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess)
     return set_error(MDN_E_HIP, "mdn_vi_jvp: kernel failed");
   if (hipMemcpy(hu, dH, sizeof(double) * S * n * 3, hipMemcpyDeviceToHost) != hipSuccess)
@@ -369,8 +416,8 @@ int mdn_vi_adjoint(const double *x, const double *w, int32_t S, int32_t n_ion, i
   if ((rc = db.alloc((size_t)S * n_ion * 3, &work)) || (rc = db.alloc(n_ion, &gs)) || (rc = db.alloc(n_ion, &ge)) ||
       (rc = db.alloc(n_ion, &gq)))
     return rc;
-  int ni = S * n_ion;
-  hipLaunchKernelGGL(k_vi_adj, dim3((ni + 255) / 256), dim3(256), 0, 0, b, dgE, dgF, work);
+  int ni = S * n_ion, Ti = team_size(n_vap);
+  hipLaunchKernelGGL(k_vi_adj, team_grid(ni, Ti), dim3(256), 0, 0, b, dgE, dgF, work, Ti);
   hipLaunchKernelGGL(k_vi_sum_g, dim3((n_ion + 255) / 256), dim3(256), 0, 0, work, S, n_ion, gs, ge, gq);
   if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess)
     return set_error(MDN_E_HIP, "mdn_vi_adjoint: kernel failed");
@@ -443,8 +490,8 @@ int mdn_vi_adjoint_noalloc(const double *x, const double *w, int32_t S, int32_t 
   hipStream_t st = static_cast<hipStream_t>(stream);
   // every slot of the partials that k_vi_sum_g reads is written by k_vi_adj
   // first, so the scratch needs no clearing between calls
-  const int ni = S * n_ion;
-  hipLaunchKernelGGL(k_vi_adj, dim3((ni + 255) / 256), dim3(256), 0, st, b, gE, gF, part);
+  const int ni = S * n_ion, Ti = team_size(n_vap);
+  hipLaunchKernelGGL(k_vi_adj, team_grid(ni, Ti), dim3(256), 0, st, b, gE, gF, part, Ti);
   hipLaunchKernelGGL(k_vi_sum_g, dim3((n_ion + 255) / 256), dim3(256), 0, st, part, S, n_ion, g_sigma, g_eps, g_q);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return set_error(MDN_E_HIP, std::string("mdn_vi_adjoint_noalloc: launch: ") + hipGetErrorString(e));

@@ -1996,7 +1996,7 @@ __global__ void __launch_bounds__(kBlk) k_force_jvp(const DevReplica *__restrict
   if (bits & B_VI) term_vi<Dual>(R, s, t, nt, false, e);
   if (bits & B_GG) term_gg<Dual>(R, s, t, nt, false, e);
   if (bits & B_GV) term_gv<Dual>(R, s, t, nt, false, e);
-  if (bits & B_VV) term_vv<Dual>(R, s, t, nt, false, e);
+  if (bits & B_VV) term_vv_jvp_rows(R, s, t >> 6, nt >> 6, t & 63);
   if (bits & B_AMBER) term_amber<Dual>(R, s, t, nt, false, e);
   if (bits & B_VINTRA) term_vintra(R, s, t, nt, false, e);
   if (bits & B_GINTRA) term_gintra(R, s, t, nt, false, e);

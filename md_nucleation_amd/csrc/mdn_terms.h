@@ -92,6 +92,18 @@ struct JStore : GStore {
   }
 };
 
+// JStore that keeps the derivative of the force on one atom (global index
+// `row`) in registers and drops the rest: the i-centric form of a pair
+// term's derivative, each pair evaluated once from each side
+struct JRowStore : JStore {
+  int row;
+  double *acc;  // [3]
+  __device__ JRowStore(const JStore &b, int r, double *a) : JStore(b), row(r), acc(a) {}
+  __device__ void fadd(const H &q, const Dual &fx, const Dual &fy, const Dual &fz) const {
+    if (q.g == row) { acc[0] += fx.d; acc[1] += fy.d; acc[2] += fz.d; }
+  }
+};
+
 __device__ inline void pc(const DevReplica &R, int t1, int t2, double &c0, double &c1) {
   const double *p = R.pair_coeff + 2 * ((size_t)t1 * R.n_types + t2);
   c0 = p[0]; c1 = p[1];
@@ -188,6 +200,29 @@ __device__ void term_vv(const DevReplica &R, const S &s, int t, int nt, bool efl
     int a = (int)(p / nva), b = (int)(p - (long long)a * nva);
     if (a / L.na_v >= b / L.na_v) continue;
     pair_ab<Rl, true>(R, s, s.h(L.vbase + a), s.h(L.vbase + b), true, eflag, e[U_VV]);
+  }
+}
+
+// the derivative of vv along u, i-centric: one wave per in-domain vapor atom
+// a (waves w of nw), its lanes over every vapor atom of another molecule;
+// the wave's sum reaches hu with one atomic per component (no atomic per
+// pair)
+__device__ inline void term_vv_jvp_rows(const DevReplica &R, const JStore &s, int w, int nw, int lane) {
+  const LView &L = s.L;
+  const int nva = L.nv * L.na_v;
+  for (int a = w; a < nva; a += nw) {
+    double acc[3] = {0, 0, 0}, e = 0;
+    const JStore::H ha = s.h(L.vbase + a);
+    const JRowStore rs(s, ha.g, acc);
+    for (int b = lane; b < nva; b += 64) {
+      if (a / L.na_v == b / L.na_v) continue;
+      pair_ab<Dual, true>(R, rs, ha, rs.h(L.vbase + b), true, false, e);
+    }
+    for (int q = 0; q < 3; q++) acc[q] = wave_sum_full(acc[q]);
+    if (lane == 0) {
+      double *h = s.hu + 3 * (size_t)ha.g;
+      atomicAdd(h, acc[0]); atomicAdd(h + 1, acc[1]); atomicAdd(h + 2, acc[2]);
+    }
   }
 }
 
