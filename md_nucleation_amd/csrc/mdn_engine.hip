@@ -133,7 +133,7 @@ __global__ void k_drift(const DevReplica *__restrict__ Rp, int save_old) {
 // larger molecules iterate in place in global memory.  Either way the
 // arithmetic is shake_molecule's, so the positions are the same.
 constexpr int kShakeBlk = 64, kShakeLdsAtoms = 12;
-__global__ void __launch_bounds__(kShakeBlk) k_shake(const DevReplica *__restrict__ Rp) {
+__global__ void __launch_bounds__(kShakeBlk) k_shake(const DevReplica *__restrict__ Rp, int skip_vapor) {
   const DevReplica &R = *Rp;
   __shared__ double s_x[kShakeLdsAtoms * 3 * kShakeBlk], s_xo[kShakeLdsAtoms * 3 * kShakeBlk];
   LView L = view_from_dyn(R);
@@ -150,7 +150,7 @@ __global__ void __launch_bounds__(kShakeBlk) k_shake(const DevReplica *__restric
     else if (m <= L.nv) { sp = 2; o = R.off_vap + (R.in_vap[m - 1] - 1 - R.n_gas) * L.na_v; }
     else { sp = 1; o = R.off_gas + (R.in_gas[m - 1 - L.nv] - 1) * L.na_g; }
     const DevSpecies &S = R.sp[sp];
-    if (!S.nrigid) continue;
+    if (!S.nrigid || (sp == 2 && skip_vapor)) continue;
     double *x = R.atom_x + 3 * (size_t)o, *v = R.atom_v + 3 * (size_t)o;
     const double *xo = R.atom_xo + 3 * (size_t)o;
     auto V = [&](int i, int q) -> double & { return v[3 * i + q]; };
@@ -182,6 +182,65 @@ __global__ void __launch_bounds__(kShakeBlk) k_shake(const DevReplica *__restric
   // every lane of the block reaches this point (no early return above)
   for (int k = 32; k > 0; k >>= 1) itmax = max(itmax, __shfl_xor(itmax, k));
   if ((tid & 63) == 0 && itmax > 0) atomicMax(&R.dyn->shake_iters_max, itmax);
+}
+
+// SHAKE of the in-domain vapor molecules of a species whose constraints fall
+// into star groups (DevSpecies::shake_star: each group joins one centre atom
+// to at most three satellites; methanol under SHAKE H: C-H3 and O-H): one
+// lane per (molecule, group), a molecule's G lanes adjacent within a quad,
+// each group iterating with its atoms in registers (shake_group_star) and the
+// lanes agreeing on convergence after every sweep.  The groups move disjoint
+// atoms, so the positions are those of k_shake's sweeps over the molecule,
+// to the bit; the longest chain of dependent updates per sweep is the
+// largest group's instead of the molecule's.  The ion and gas molecules stay
+// with k_shake.
+constexpr int kShakeGrpBlk = 256, kShakeGrpCap = 16;
+__global__ void __launch_bounds__(kShakeGrpBlk) k_shake_groups(const DevReplica *__restrict__ Rp) {
+  const DevReplica &R = *Rp;
+  __shared__ int s_n[2], s_ij[2 * kShakeGrpCap * 2], s_ng[2], s_gs[10], s_star[2];
+  __shared__ double s_c[2 * kShakeGrpCap * 4];
+  __shared__ unsigned long long s_cons[2], s_gm[8];
+  ShakeLds T;
+  T.n = s_n; T.ij = s_ij; T.c = s_c; T.cons = s_cons; T.ng = s_ng; T.gs = s_gs; T.gm = s_gm; T.star = s_star;
+  T.cap = kShakeGrpCap;
+  const DevSpecies &S = R.sp[2];
+  if (threadIdx.x == 0) shake_lds_fill(R, S, T, 1);
+  __syncthreads();
+  const LView L = view_from_dyn(R);
+  const int G = T.ng[1], gsh = G >= 4 ? 2 : G >= 2 ? 1 : 0, W = 1 << gsh;
+  const int nslot = L.nv << gsh;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x, nt = gridDim.x * blockDim.x;
+  int itmax = 0;
+  // slots advance by nt (a multiple of 4): a molecule's lanes stay in one
+  // quad and are all active or all idle
+  for (int slot = t; slot < nslot; slot += nt) {
+    const int m = slot >> gsh, g = slot & (W - 1);
+    const size_t o = (size_t)R.off_vap + (size_t)(R.in_vap[m] - 1 - R.n_gas) * L.na_v;
+    double *x = R.atom_x + 3 * o, *v = R.atom_v + 3 * o;
+    const double *xo = R.atom_xo + 3 * o;
+    auto X = [&](int i, int q) -> double & { return x[3 * i + q]; };
+    auto XO = [&](int i, int q) -> double { return xo[3 * i + q]; };
+    auto V = [&](int i, int q) -> double & { return v[3 * i + q]; };
+    const int it = shake_group_lds(R, T, 1, g, W, X, XO, R.shake_max_iter);
+    if (it < 0) {
+      // every lane of the molecule sees the failure; the first undoes it
+      itmax = max(itmax, -it);
+      if (g == 0) {
+        atomicAdd(&R.dyn->shake_fail, 1ull);
+        shake_undo(S, X, XO, V, R.dt);
+      }
+      continue;
+    }
+    itmax = max(itmax, it);
+    if (R.shake == 1) {
+      const unsigned long long gm = T.gm[4 + g];
+      for (int a = 0; a < S.natoms; a++)
+        if ((gm >> a) & 1ull)
+          for (int q = 0; q < 3; q++) V(a, q) = (X(a, q) - XO(a, q)) / R.dt;
+    }
+  }
+  for (int k = 32; k > 0; k >>= 1) itmax = max(itmax, __shfl_xor(itmax, k));
+  if ((threadIdx.x & 63) == 0 && itmax > 0) atomicMax(&R.dyn->shake_iters_max, itmax);
 }
 
 // Observer::computeProps (observer.cpp:5-36), one workgroup per replica:
@@ -296,7 +355,7 @@ __device__ inline void flush_energy(const DevReplica &R, const double *e) {
 }
 
 template <typename Rl>
-__global__ void k_terms(const DevReplica *__restrict__ Rp, uint32_t bits, int eflag) {
+__global__ void __launch_bounds__(kBlk) k_terms(const DevReplica *__restrict__ Rp, uint32_t bits, int eflag) {
   const DevReplica &R = *Rp;
   GStore s;
   s.R = Rp;
@@ -2028,6 +2087,21 @@ static int launch_resume(mdn_ctx *c, HostReplica &h, const DevReplica *R, bool e
   return MDN_OK;
 }
 
+// MD::shake (shake.cpp:44-92) of every constrained molecule: the vapor a
+// lane per constraint group when its groups are stars, the rest (and any
+// other vapor) one thread per molecule
+static void launch_shake(mdn_ctx *c, const HostReplica &h, const DevReplica *R) {
+  const SysDef &d = h.def;
+  const DevSpecies &V = h.dev.sp[2];
+  const bool grp = V.nrigid > 0 && V.nrigid <= kShakeGrpCap && V.shake_star;
+  if (grp) {
+    const int W = V.shake_ng >= 4 ? 4 : V.shake_ng >= 2 ? 2 : 1;
+    hipLaunchKernelGGL(k_shake_groups, dim3(grid_for(d.n_vap * W, kShakeGrpBlk)), dim3(kShakeGrpBlk), 0, c->stream, R);
+  }
+  if (!grp || h.dev.sp[0].nrigid || h.dev.sp[1].nrigid)
+    hipLaunchKernelGGL(k_shake, dim3(grid_for(d.n_mol, kShakeBlk)), dim3(kShakeBlk), 0, c->stream, R, grp ? 1 : 0);
+}
+
 // one MD::verlet step of one replica through the per-call kernels
 template <typename Rl>
 static int launch_step(mdn_ctx *c, const HostReplica &h, const DevReplica *R, bool eflag) {
@@ -2039,7 +2113,7 @@ static int launch_step(mdn_ctx *c, const HostReplica &h, const DevReplica *R, bo
   if (h.thermo == THERMO_SCALE) hipLaunchKernelGGL(k_ion_scale, dim3(1), dim3(1024), 0, c->stream, R);
   bool sh = d.shake && (d.sp[0].rigid.size() || d.sp[1].rigid.size() || d.sp[2].rigid.size());
   hipLaunchKernelGGL(k_drift, dim3(g), dim3(kBlk), 0, c->stream, R, sh ? 1 : 0);
-  if (sh) hipLaunchKernelGGL(k_shake, dim3(grid_for(d.n_mol, kShakeBlk)), dim3(kShakeBlk), 0, c->stream, R);
+  if (sh) launch_shake(c, h, R);
   hipLaunchKernelGGL(k_com_weights, dim3(com_blocks(d.n_atoms)), dim3(1024), 0, c->stream, R, nh);
   hipLaunchKernelGGL(k_rebuild, dim3(1), dim3(1024), 0, c->stream, R, 0);
   int rc = launch_terms<Rl>(c, h, R, d.flags, eflag);
@@ -2509,7 +2583,7 @@ int mdn_update_position(mdn_ctx *c, int32_t r) {
   bool sh = h.def.shake && (h.def.sp[0].rigid.size() || h.def.sp[1].rigid.size() || h.def.sp[2].rigid.size());
   const DevReplica *R = c->d_reps + r;
   hipLaunchKernelGGL(k_drift, dim3(grid_for(h.def.n_atoms)), dim3(kBlk), 0, c->stream, R, sh ? 1 : 0);
-  if (sh) hipLaunchKernelGGL(k_shake, dim3(grid_for(h.def.n_mol, kShakeBlk)), dim3(kShakeBlk), 0, c->stream, R);
+  if (sh) launch_shake(c, h, R);
   hipLaunchKernelGGL(k_com_weights, dim3(com_blocks(h.def.n_atoms)), dim3(1024), 0, c->stream, R, 0);
   HIPCHK(hipGetLastError());
   return MDN_OK;

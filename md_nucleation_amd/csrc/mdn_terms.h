@@ -568,18 +568,34 @@ __device__ void term_sw(const DevReplica &R, const S &s, int t, int nt, bool efl
 // IntraInter Tersoff, SW, AMBER, BMH, vapor intra, gas intra.
 template <typename Rl, class S>
 __device__ void all_terms(const DevReplica &R, const S &s, uint32_t bits, int t, int nt, bool eflag, double *e) {
-  if (bits & B_GI) term_gi<Rl>(R, s, t, nt, eflag, e);
-  if (bits & B_VI) term_vi<Rl>(R, s, t, nt, eflag, e);
-  if (bits & B_GG) term_gg<Rl>(R, s, t, nt, eflag, e);
-  if (bits & B_GV) term_gv<Rl>(R, s, t, nt, eflag, e);
-  if (bits & B_VV) term_vv<Rl>(R, s, t, nt, eflag, e);
-  if (bits & B_EFIELD) term_efield(R, s, t, nt);
-  if (bits & B_TERSOFF) term_tersoff(R, s, t, nt, eflag, e);
-  if (bits & B_SW) term_sw(R, s, t, nt, eflag, e);
-  if (bits & B_AMBER) term_amber<Rl>(R, s, t, nt, eflag, e);
-  if (bits & B_BMH) term_born(R, s, t, nt, eflag, e);
-  if (bits & B_VINTRA) term_vintra(R, s, t, nt, eflag, e);
-  if (bits & B_GINTRA) term_gintra(R, s, t, nt, eflag, e);
+  // Each term's work items start where the previous term's ended (modulo
+  // the grid, in whole waves), so the terms spread over the grid side by
+  // side instead of all starting on thread 0: a thread then takes about
+  // (total items / threads) items rather than one of every term, and the
+  // kernel's latency is that of its slowest items, not of their sum.  A
+  // wave still holds items of one term only (GStore::fadd's wave sums rely
+  // on it).
+  const LView &L = s.L;
+  int off = 0;
+  auto at = [&](int items) {
+    const int tt = t - off;
+    off = (int)((off + ((long long)items + 63) / 64 * 64) % nt);
+    return tt < 0 ? tt + nt : tt;
+  };
+  const DevSpecies &I = R.sp[0], &G = R.sp[1], &V = R.sp[2];
+  const int nj = L.na_g < L.na_v ? L.na_g : L.na_v;
+  if (bits & B_GI) term_gi<Rl>(R, s, at(L.n_gi * L.na_g), nt, eflag, e);
+  if (bits & B_VI) term_vi<Rl>(R, s, at(L.nv * L.na_v * L.n_ion), nt, eflag, e);
+  if (bits & B_GG) term_gg<Rl>(R, s, at(L.n_gg * L.na_g * L.na_g), nt, eflag, e);
+  if (bits & B_GV) term_gv<Rl>(R, s, at(L.n_gv * L.na_g * nj), nt, eflag, e);
+  if (bits & B_VV) term_vv<Rl>(R, s, at(0), nt, eflag, e);
+  if (bits & B_EFIELD) term_efield(R, s, at(L.n_ion), nt);
+  if (bits & B_TERSOFF) term_tersoff(R, s, at(0), nt, eflag, e);
+  if (bits & B_SW) term_sw(R, s, at(0), nt, eflag, e);
+  if (bits & B_AMBER) term_amber<Rl>(R, s, at(R.n_long + I.nbond + I.nangle + I.ndih), nt, eflag, e);
+  if (bits & B_BMH) term_born(R, s, at(0), nt, eflag, e);
+  if (bits & B_VINTRA) term_vintra(R, s, at(L.nv * (V.nbond + V.nangle + V.ndih)), nt, eflag, e);
+  if (bits & B_GINTRA) term_gintra(R, s, at(L.ng * G.nbond), nt, eflag, e);
 }
 
 // ------------------------------------------------------------------------
