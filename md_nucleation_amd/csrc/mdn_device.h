@@ -150,9 +150,10 @@ __host__ __device__ inline void rng_uniform2(uint32_t k0, uint32_t k1, uint32_t 
   u[0] = u01_53(w[0], w[1]);
   u[1] = u01_53(w[2], w[3]);
 }
-// Box-Muller on (0,1] x [0,1)
-__host__ __device__ inline void rng_normal2(uint32_t k0, uint32_t k1, uint32_t mol, long long rb, uint32_t purpose,
-                                            double z[2]) {
+// Box-Muller on (0,1] x [0,1): z = r (cos th, sin th), the sine and cosine
+// from one argument reduction
+__device__ inline void rng_normal2(uint32_t k0, uint32_t k1, uint32_t mol, long long rb, uint32_t purpose,
+                                   double z[2]) {
   uint32_t w[4];
   rng_words(k0, k1, mol, rb, purpose, w);
   uint64_t v = ((uint64_t)w[0] << 32) | w[1];
@@ -160,8 +161,19 @@ __host__ __device__ inline void rng_normal2(uint32_t k0, uint32_t k1, uint32_t m
   double u2 = u01_53(w[2], w[3]);
   double r = sqrt(-2.0 * log(u1));
   double th = 2.0 * kPI * u2;
-  z[0] = r * cos(th);
-  z[1] = r * sin(th);
+  double sn, cs;
+  sincos(th, &sn, &cs);
+  z[0] = r * cs;
+  z[1] = r * sn;
+}
+// the first of rng_normal2's pair alone (r cos th)
+__device__ inline double rng_normal1(uint32_t k0, uint32_t k1, uint32_t mol, long long rb, uint32_t purpose) {
+  uint32_t w[4];
+  rng_words(k0, k1, mol, rb, purpose, w);
+  uint64_t v = ((uint64_t)w[0] << 32) | w[1];
+  double u1 = (double)((v >> 11) + 1) * (1.0 / 9007199254740992.0);
+  double u2 = u01_53(w[2], w[3]);
+  return sqrt(-2.0 * log(u1)) * cos(2.0 * kPI * u2);
 }
 
 // Variables::ROTATION, src/variables.hpp:94-98

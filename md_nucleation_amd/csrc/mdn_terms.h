@@ -1104,12 +1104,13 @@ __device__ inline void rebuild_wg(const DevReplica &R, int *s_int, double *s_dbl
       if (dy > HL) m[1] -= R.d_size;
       if (dz > HL) m[2] -= R.d_size;
       double sg = sp == 1 ? R.sig_gas : R.sig_vap;
-      double z[2], z2[2];
+      // three normals: a pair, then the first of another pair
+      double z[2];
       rng_normal2(R.key0, R.key1, (uint32_t)i, rb, 0, z);
-      rng_normal2(R.key0, R.key1, (uint32_t)i, rb, 1, z2);
+      const double z3 = rng_normal1(R.key0, R.key1, (uint32_t)i, rb, 1);
       mv[0] = sg * z[0] * 1e-5;
       mv[1] = sg * z[1] * 1e-5;
-      mv[2] = sg * z2[0] * 1e-5;
+      mv[2] = sg * z3 * 1e-5;
     }
   }
   __syncthreads();
