@@ -1661,6 +1661,28 @@ struct mdn_ctx {
   int any_shake = 0;
   int log_cap = 0;          // vapor events per replica between drains (0: off)
   double *d_obs = nullptr;  // [8 * n_rep] observables
+  // calls that may change the replicas' device state count up `mut`; the
+  // pinned counts h_dyn are those of the device while dyn_mut == mut (set
+  // when mdn_run_batched leaves them current), so the next batched call
+  // needs no read-back
+  unsigned long long mut = 0, dyn_mut = ~0ull;
+  int lds_attr = 0;  // k_batched variants whose LDS limit is raised
+  bool plan_on_device = false;  // d_order holds plan.order
+  // the split launch's classes, reused by the next chunk or call while they
+  // are younger than a chunk, nothing but mdn_run_batched has touched the
+  // replicas, the launch settings are the same and every replica's ion +
+  // vapor and in-domain atom counts stay within kPlanSlack of those the
+  // classes were made for (the capacities, sized per launch, keep their
+  // headroom)
+  struct SplitPlan {
+    bool valid = false;
+    long long age = 0;
+    unsigned long long mut = 0;
+    size_t rl = 0;
+    int own_threads = 0, shared_threads = 0;
+    double heavy_factor = 0;
+    std::vector<int> heavy, clsB, clsA, order, need, need_h;
+  } plan;
   // pinned host copies for the batched chunk loop: the replicas' counts
   // (read back behind each chunk's launches) and the launch order
   mdn::Dyn *h_dyn = nullptr;
@@ -1979,6 +2001,7 @@ static int upload_all(mdn_ctx *c) {
   c->d_reps = at<DevReplica>(B, reps_off);
   c->d_dyn = at<Dyn>(B, dyn_block);
   c->d_order = at<int>(B, order_off);
+  c->plan_on_device = false;
   c->d_obs = at<double>(B, obs_off);
   HIPCHK(hipMemcpy(c->dbuf, lay.host.data(), lay.host.size(), hipMemcpyHostToDevice));
   return MDN_OK;
@@ -2252,6 +2275,7 @@ static int copy_state_in(HostReplica &h, const mdn_state *st) {
 
 int mdn_add_replica(mdn_ctx *c, const mdn_sysdef *sd, const mdn_state *st, uint64_t seed, uint32_t replica_id,
                     int32_t *handle) {
+  if (c) c->mut++;
   if (!c || !sd || !st) return set_error(MDN_E_ARG, "mdn_add_replica: null argument");
   c->reps.emplace_back();
   HostReplica &h = c->reps.back();
@@ -2269,6 +2293,7 @@ int mdn_add_replica(mdn_ctx *c, const mdn_sysdef *sd, const mdn_state *st, uint6
 }
 
 int mdn_commit(mdn_ctx *c) {
+  if (c) c->mut++;
   if (!c) return set_error(MDN_E_ARG, "mdn_commit: null context");
   if (c->reps.empty()) return set_error(MDN_E_STATE, "mdn_commit: no replicas");
   int rc = upload_all(c);
@@ -2278,6 +2303,7 @@ int mdn_commit(mdn_ctx *c) {
 }
 
 int mdn_set_state(mdn_ctx *c, int32_t r, const mdn_state *st) {
+  if (c) c->mut++;
   int rc = check_replica(c, r);
   if (rc) return rc;
   HIPCHK(hipSetDevice(c->device));
@@ -2298,6 +2324,7 @@ int mdn_set_state(mdn_ctx *c, int32_t r, const mdn_state *st) {
 // MD::setNVE (nvt.cpp:38-43) and the NVTion modes (setCondition.cpp:158-172)
 // for one replica (r >= 0) or all (r = -1); Nose-Hoover also resets zeta
 int mdn_set_thermostat(mdn_ctx *c, int32_t r, int32_t mode, double Tnh) {
+  if (c) c->mut++;
   if (!c) return set_error(MDN_E_ARG, "mdn_set_thermostat: null context");
   if (mode != MDN_NVE && mode != MDN_VSCALE && mode != MDN_NOSE_HOOVER)
     return set_error(MDN_E_ARG, "mdn_set_thermostat: mode must be MDN_NVE, MDN_VSCALE or MDN_NOSE_HOOVER");
@@ -2349,6 +2376,7 @@ static int replica_range(mdn_ctx *c, int32_t r, int *r0, int *r1) {
 }
 
 int mdn_set_vapor_log(mdn_ctx *c, int32_t cap) {
+  if (c) c->mut++;
   if (!c) return set_error(MDN_E_ARG, "mdn_set_vapor_log: null context");
   if (cap < 0) return set_error(MDN_E_ARG, "mdn_set_vapor_log: negative capacity");
   if (cap == c->log_cap) return MDN_OK;
@@ -2364,6 +2392,7 @@ int mdn_set_vapor_log(mdn_ctx *c, int32_t cap) {
 }
 
 int mdn_drain_vapor_log(mdn_ctx *c, int32_t r, mdn_vapor_event *out, int32_t cap, int32_t *n) {
+  if (c) c->mut++;
   int rc = check_replica(c, r);
   if (rc) return rc;
   if (!n || (cap > 0 && !out)) return set_error(MDN_E_ARG, "mdn_drain_vapor_log: null argument");
@@ -2404,6 +2433,7 @@ int mdn_observe(mdn_ctx *c, int32_t r, double *out) {
 }
 
 int mdn_set_step(mdn_ctx *c, int32_t r, int64_t step) {
+  if (c) c->mut++;
   int r0, r1;
   int rc = replica_range(c, r, &r0, &r1);
   if (rc) return rc;
@@ -2419,6 +2449,7 @@ int mdn_set_step(mdn_ctx *c, int32_t r, int64_t step) {
 // run.cpp:42-53 as written: the loop zeroes position[0] on its first pass and
 // then subtracts zero from every other position
 int mdn_recentre(mdn_ctx *c, int32_t r) {
+  if (c) c->mut++;
   int r0, r1;
   int rc = replica_range(c, r, &r0, &r1);
   if (rc) return rc;
@@ -2477,6 +2508,7 @@ int mdn_get_state(mdn_ctx *c, int32_t r, mdn_state *st) {
 }
 
 int mdn_set_positions(mdn_ctx *c, int32_t r, const double *atom_x) {
+  if (c) c->mut++;
   int rc = check_replica(c, r);
   if (rc) return rc;
   if (!atom_x) return set_error(MDN_E_ARG, "mdn_set_positions: null array");
@@ -2531,6 +2563,7 @@ static int refuse_pending(mdn_ctx *c, int32_t r, const char *fn) {
 }
 
 int mdn_make_pair(mdn_ctx *c, int32_t r) {
+  if (c) c->mut++;
   int rc = check_replica(c, r);
   if (rc) return rc;
   if ((rc = refuse_pending(c, r, "mdn_make_pair"))) return rc;
@@ -2541,6 +2574,7 @@ int mdn_make_pair(mdn_ctx *c, int32_t r) {
 }
 
 int mdn_zero_forces(mdn_ctx *c, int32_t r) {
+  if (c) c->mut++;
   int rc = check_replica(c, r);
   if (rc) return rc;
   const DevReplica &R = c->reps[r].dev;
@@ -2550,6 +2584,7 @@ int mdn_zero_forces(mdn_ctx *c, int32_t r) {
 }
 
 int mdn_compute(mdn_ctx *c, int32_t r, uint32_t bits, int32_t eflag, int32_t precision) {
+  if (c) c->mut++;
   int rc = check_replica(c, r);
   if (rc) return rc;
   HIPCHK(hipSetDevice(c->device));
@@ -2563,6 +2598,7 @@ int mdn_compute(mdn_ctx *c, int32_t r, uint32_t bits, int32_t eflag, int32_t pre
 }
 
 int mdn_velocity_calculation(mdn_ctx *c, int32_t r) {
+  if (c) c->mut++;
   int rc = check_replica(c, r);
   if (rc) return rc;
   if (c->reps[r].pending) {  // this kick ends the step left unfinished
@@ -2576,6 +2612,7 @@ int mdn_velocity_calculation(mdn_ctx *c, int32_t r) {
 }
 
 int mdn_update_position(mdn_ctx *c, int32_t r) {
+  if (c) c->mut++;
   int rc = check_replica(c, r);
   if (rc) return rc;
   if ((rc = refuse_pending(c, r, "mdn_update_position"))) return rc;
@@ -2590,6 +2627,7 @@ int mdn_update_position(mdn_ctx *c, int32_t r) {
 }
 
 int mdn_verlet(mdn_ctx *c, int32_t r, int32_t nsteps, int32_t eflag_every, int64_t step0, int32_t precision) {
+  if (c) c->mut++;
   int rc = check_replica(c, r);
   if (rc) return rc;
   HIPCHK(hipSetDevice(c->device));
@@ -2613,6 +2651,7 @@ int mdn_verlet(mdn_ctx *c, int32_t r, int32_t nsteps, int32_t eflag_every, int64
 // mdn_verlet with each step's launch sequence captured once into a hipGraph
 // (per replica, precision and energy flag) and replayed
 int mdn_verlet_graph(mdn_ctx *c, int32_t r, int32_t nsteps, int32_t eflag_every, int64_t step0, int32_t precision) {
+  if (c) c->mut++;
   int rc = check_replica(c, r);
   if (rc) return rc;
   HIPCHK(hipSetDevice(c->device));
@@ -2716,8 +2755,13 @@ static int launch_batched(mdn_ctx *c, hipStream_t st, const int *order, int n, i
     return set_error(MDN_E_CAPACITY, "mdn_run_batched: a replica's in-domain atoms exceed one workgroup's LDS; "
                                      "use mdn_verlet for it");
   // launches of different sizes may run concurrently: allow the full LDS
+  // (once per kernel variant and context)
   auto kern = mb ? k_batched<Rl, true> : k_batched<Rl, false>;
-  HIPCHK(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+  const int vbit = 1 << ((sizeof(Rl) == 4 ? 2 : 0) + (mb ? 1 : 0));
+  if (!(c->lds_attr & vbit)) {
+    HIPCHK(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+    c->lds_attr |= vbit;
+  }
   int grid = c->batch_grid > 0 ? std::min(c->batch_grid, n) : n;
   hipLaunchKernelGGL(kern, dim3(grid), dim3(threads), b.lds, st, c->d_reps, order, n, ns, eflag_every, step0,
                      b.cap, b.cap_h, b.kcap, b.acap, b.scap);
@@ -2736,6 +2780,25 @@ static bool needs_manybody(const mdn_ctx *c, const std::vector<int> &set) {
 // XCDs of the device: CDNA3/4 parts are built from 32-CU XCDs (MI355X: 8 x
 // 32); other CU counts are taken as one
 static int xcds(int n_cu) { return n_cu >= 64 && n_cu % 32 == 0 ? n_cu / 32 : 1; }
+
+// atoms by which a replica's counts may grow before a reused split plan is
+// made anew (a rebuild moves a few gas and at most a vapor molecule or two)
+constexpr int kPlanSlack = 16;
+
+static void SplitPlan_store(mdn_ctx *c, const std::vector<int> &heavy, const std::vector<int> &clsB,
+                            const std::vector<int> &clsA, const std::vector<int> &order, size_t rl) {
+  auto &P = c->plan;
+  P.valid = true;
+  P.age = 0;
+  P.mut = c->mut;
+  P.rl = rl;
+  P.own_threads = c->own_threads;
+  P.shared_threads = c->shared_threads;
+  P.heavy_factor = c->heavy_factor;
+  P.heavy = heavy; P.clsB = clsB; P.clsA = clsA; P.order = order;
+  P.need.assign(order.size(), 0);
+  P.need_h.assign(order.size(), 0);
+}
 
 extern "C" {
 
@@ -2762,8 +2825,11 @@ int mdn_run_batched(mdn_ctx *c, int32_t nsteps, int32_t eflag_every, int64_t ste
   std::vector<Dyn> dyn(c->reps.size());
   std::vector<int> order(n);
   c->last_ms = 0;
-  HIPCHK(hipMemcpyAsync(c->h_dyn, c->d_dyn, sizeof(Dyn) * n, hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(hipStreamSynchronize(c->stream));
+  if (c->dyn_mut != c->mut) {
+    HIPCHK(hipMemcpyAsync(c->h_dyn, c->d_dyn, sizeof(Dyn) * n, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  }
+  c->dyn_mut = ~0ull;
   for (int done = 0; done < nsteps; done += chunk) {
     const int ns = std::min(chunk, nsteps - done);
     memcpy(dyn.data(), c->h_dyn, sizeof(Dyn) * n);
@@ -2779,18 +2845,39 @@ int mdn_run_batched(mdn_ctx *c, int32_t nsteps, int32_t eflag_every, int64_t ste
     // fit is that of the fullest XCD.  Within a launch the replicas go in
     // descending cost: a workgroup that finds no room starts when one ends,
     // and the cheapest can best afford that.
-    std::vector<double> cost(n);
-    for (int r = 0; r < n; r++) {
-      const SysDef &d = c->reps[r].def;
-      double nh = d.sp[0].natoms + dyn[r].n_in_vap * d.sp[2].natoms;
-      double nin = nh + dyn[r].n_in_gas * d.sp[1].natoms;
-      cost[r] = nh * nh + 40.0 * nin;
-      order[r] = r;
-    }
-    std::stable_sort(order.begin(), order.end(), [&](int a, int b2) { return cost[a] > cost[b2]; });
     const size_t kA = 40 * 1024, kB = 53 * 1024;
     std::vector<int> heavy, clsA, clsB;
-    if (c->batch_split && c->batch_threads == 256 && n > c->n_cu) {
+    const bool split = c->batch_split && c->batch_threads == 256 && n > c->n_cu;
+    auto needs = [&](int r, int &need, int &need_h) {
+      const SysDef &d = c->reps[r].def;
+      need_h = d.sp[0].natoms + dyn[r].n_in_vap * d.sp[2].natoms;
+      need = need_h + dyn[r].n_in_gas * d.sp[1].natoms;
+    };
+    bool reuse = split && c->plan.valid && c->plan.age + ns <= chunk && c->plan.mut == c->mut && c->plan.rl == rl &&
+                 c->plan.own_threads == c->own_threads && c->plan.shared_threads == c->shared_threads &&
+                 c->plan.heavy_factor == c->heavy_factor && (int)c->plan.order.size() == n;
+    for (int r = 0; r < n && reuse; r++) {
+      int need, need_h;
+      needs(r, need, need_h);
+      reuse = need <= c->plan.need[r] + kPlanSlack && need_h <= c->plan.need_h[r] + kPlanSlack;
+    }
+    std::vector<double> cost(n);
+    if (!reuse) {
+      for (int r = 0; r < n; r++) {
+        const SysDef &d = c->reps[r].def;
+        double nh = d.sp[0].natoms + dyn[r].n_in_vap * d.sp[2].natoms;
+        double nin = nh + dyn[r].n_in_gas * d.sp[1].natoms;
+        cost[r] = nh * nh + 40.0 * nin;
+        order[r] = r;
+      }
+      std::stable_sort(order.begin(), order.end(), [&](int a, int b2) { return cost[a] > cost[b2]; });
+    }
+    if (reuse) {
+      heavy = c->plan.heavy;
+      clsB = c->plan.clsB;
+      clsA = c->plan.clsA;
+      order = c->plan.order;
+    } else if (split) {
       // each replica's own LDS need at the smallest headroom
       std::vector<CapNeed> rq(n);
       std::vector<size_t> own(n);
@@ -2871,10 +2958,17 @@ int mdn_run_batched(mdn_ctx *c, int32_t nsteps, int32_t eflag_every, int64_t ste
       order.insert(order.end(), heavy.begin(), heavy.end());
       order.insert(order.end(), clsB.begin(), clsB.end());
       order.insert(order.end(), clsA.begin(), clsA.end());
+      SplitPlan_store(c, heavy, clsB, clsA, order, rl);
+      for (int r = 0; r < n; r++) needs(r, c->plan.need[r], c->plan.need_h[r]);
     }
+    if (split) c->plan.age += ns;
     const int H = (int)heavy.size(), NB_ = (int)clsB.size(), NA = (int)clsA.size();
-    memcpy(c->h_order, order.data(), sizeof(int) * n);
-    HIPCHK(hipMemcpyAsync(c->d_order, c->h_order, sizeof(int) * n, hipMemcpyHostToDevice, c->stream));
+    // the launch order on the device (still there when the plan is reused)
+    if (!(reuse && c->plan_on_device)) {
+      memcpy(c->h_order, order.data(), sizeof(int) * n);
+      HIPCHK(hipMemcpyAsync(c->d_order, c->h_order, sizeof(int) * n, hipMemcpyHostToDevice, c->stream));
+    }
+    c->plan_on_device = split;
     const long long s0 = (long long)(step0 + done);
     HIPCHK(hipEventRecord(c->ev0, c->stream));
     int rc;
@@ -2935,6 +3029,7 @@ int mdn_run_batched(mdn_ctx *c, int32_t nsteps, int32_t eflag_every, int64_t ste
   // replicas stopped inside a step (LDS capacity outgrown at a rebuild): the
   // per-call path finishes that step first
   for (int r = 0; r < n; r++) c->reps[r].pending = c->h_dyn[r].pending;
+  c->dyn_mut = c->mut;
   return MDN_OK;
 }
 
@@ -3225,6 +3320,7 @@ int mdn_segment_grad(mdn_ctx *c, int32_t r, int32_t nsteps, int32_t every, const
                      const double *ion_sigma, const double *ion_eps, const double *ion_q, const int32_t *obs,
                      int32_t n_obs, const double *targets, double *loss, double *g_sigma, double *g_eps,
                      double *g_q, double *x_end, double *v_end) {
+  if (c) c->mut++;
   int rc = check_replica(c, r);
   if (rc) return rc;
   if ((rc = refuse_pending(c, r, "mdn_segment_grad"))) return rc;

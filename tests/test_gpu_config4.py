@@ -19,7 +19,9 @@ chunks).
     per-label energy error;
   * every value of each autotuner knob (workgroup sizes of the classes,
     own-CU threshold, chunk): 300 FP64 steps that agree with the
-    default launch's to rounding (positions within 1e-9 A).
+    default launch's to rounding (positions within 1e-9 A);
+  * the same 300 steps as fifteen 20-step calls (launch classes reused from
+    call to call) against one call, to rounding.
 """
 import os
 
@@ -210,3 +212,34 @@ def test_tuning_knobs_same_trajectory(workload):
             worst = max(worst, float(np.abs(got[r].atom_x - base[r].atom_x).max()))
         print("knob %s: plan %s, max |dx| vs default %.2e A" % (change, plan, worst))
         assert worst < 1e-9
+
+
+def test_short_calls_same_trajectory(workload):
+    """300 FP64 steps as fifteen calls of 20 steps (the counts read back at
+    the end of each call, the launch classes reused from call to call) agree
+    with one 300-step call to rounding."""
+    from md_nucleation_amd import autotune
+    reps = workload
+
+    def run(calls):
+        eng = W.build_engine(reps)
+        autotune.apply(eng, dict(autotune.DEFAULT))
+        done = 0
+        for n in calls:
+            eng.run_batched(n, eflag_every=EVERY, step0=done, precision=M.FP64)
+            done += n
+        eng.sync()
+        plan, _ = eng.batch_plan()
+        out = [eng.get_state(r, cap=20000) for r in range(N_REP)]
+        eng.close()
+        return plan, out
+
+    plan1, one = run([300])
+    plan15, many = run([20] * 15)
+    worst = 0.0
+    for r in range(N_REP):
+        assert many[r].c.rebuild_count == one[r].c.rebuild_count, r
+        np.testing.assert_array_equal(many[r].region, one[r].region)
+        worst = max(worst, float(np.abs(many[r].atom_x - one[r].atom_x).max()))
+    print("15 x 20 steps vs 300: plans %s / %s, max |dx| %.2e A" % (plan15, plan1, worst))
+    assert worst < 1e-9
