@@ -45,10 +45,10 @@ PEAK_FP64_TFLOPS = 78.6
 # HBM bytes per launch of the own-CU class (156 replicas with a CU each),
 # from the FETCH_SIZE / WRITE_SIZE passes in
 # profiles/k_batched_mixed_512replicas.md (copy-calibrated: FETCH_SIZE x 2.0,
-# WRITE_SIZE x 1.0): 31.5 MB for a 200-step launch and the same for a 1-step
+# WRITE_SIZE x 1.0): 24.2 MB for a 200-step launch and the same for a 1-step
 # one, i.e. the replicas' state read at the launch's start and written at its
 # end; the steps themselves run out of LDS
-HEAVY_HBM_BYTES_PER_LAUNCH = 31.5e6
+HEAVY_HBM_BYTES_PER_LAUNCH = 24.2e6
 # algorithmic FLOPs per unique pair / bonded term, as written in the reference
 FLOP = {"ljcoul_w": 31, "ljcoul": 30, "lj": 24, "bond": 22, "angle": 70, "dihedral": 150}
 
