@@ -1460,25 +1460,24 @@ __global__ void __launch_bounds__(512) k_batched(const DevReplica *__restrict__ 
               shake_grp(Vs, L.vbase + (slot >> gsh) * L.na_v, slot & ((1 << gsh) - 1), 1 << gsh);
           __builtin_amdgcn_s_setprio(0);
         } else {
-          // ion COM (verlet.cpp:90-107), every phase-A wave for itself
-          double c[6] = {0, 0, 0, 0, 0, 0};
-          for (int i = tid & 63; i < I.natoms; i += 64) {
-            double m = B.kms[i];
-            for (int q = 0; q < 3; q++) { c[q] += B.xr[i].x[q] * m; c[3 + q] += B.v[3 * i + q] * m; }
-          }
-          for (int k = 0; k < 6; k++) c[k] = wave_sum_full(c[k]) * (1 / I.molmass);
-          if (tid < 6) B.sdbl[tid] = c[tid];
-          // blend weights of the gas atoms (verlet.cpp:109-134)
-          for (int a = L.gbase + tid; a < L.n_in; a += 64 * nA) {
-            double dx = B.xr[a].x[0] - c[0], dy = B.xr[a].x[1] - c[1], dz = B.xr[a].x[2] - c[2];
-            B.xr[a].w = blend_weight_r<Rl>(dx * dx + dy * dy + dz * dz);
+          // ion COM (verlet.cpp:90-107) on wave 0; no phase-A term reads a
+          // weight, so the blend weights wait for the barrier
+          if (wave == 0) {
+            double c[6] = {0, 0, 0, 0, 0, 0};
+            for (int i = tid; i < I.natoms; i += 64) {
+              double m = B.kms[i];
+              for (int q = 0; q < 3; q++) { c[q] += B.xr[i].x[q] * m; c[3 + q] += B.v[3 * i + q] * m; }
+            }
+            for (int k = 0; k < 6; k++) c[k] = wave_sum_full(c[k]) * (1 / I.molmass);
+            if (tid < 6) B.sdbl[tid] = c[tid];
           }
           batch_terms_a<Rl, MB>(R, ss, B, tid, 0, 64 * nA, eflag, e);
         }
         __syncthreads();
         {
+          // blend weights of the vapor and gas atoms (verlet.cpp:109-134)
           const double cx = B.sdbl[0], cy = B.sdbl[1], cz = B.sdbl[2];
-          for (int a = L.n_ion + tid; a < L.gbase; a += nt) {
+          for (int a = L.n_ion + tid; a < L.n_in; a += nt) {
             double dx = B.xr[a].x[0] - cx, dy = B.xr[a].x[1] - cy, dz = B.xr[a].x[2] - cz;
             B.xr[a].w = blend_weight_r<Rl>(dx * dx + dy * dy + dz * dz);
           }
