@@ -469,4 +469,74 @@ __device__ inline int block_scan(int v, int *s_tmp, int *total) {
   return base + x - v;
 }
 
+// Ordered compaction over the workgroup without a scan per chunk: each wave
+// takes a contiguous range [n * w / nw, n * (w + 1) / nw) of the items,
+// counts its hits with ballots, and after one exchange of the per-wave
+// counts (s_tmp: >= nthr/64 ints) writes them in item order.  keep(i)
+// tests item i; put(i, p) stores it at output position base + p.  Returns
+// the workgroup's hit count (the same on every thread).
+template <class Keep, class Put>
+__device__ inline int wave_compact(int n, Keep keep, Put put, int *s_tmp) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+  const int i0 = (int)((long long)n * wave / nw), i1 = (int)((long long)n * (wave + 1) / nw);
+  int cnt = 0;
+  for (int c = i0; c < i1; c += 64) {
+    const int i = c + lane;
+    cnt += __popcll(__ballot(i < i1 && keep(i)));
+  }
+  if (lane == 0) s_tmp[wave] = cnt;
+  __syncthreads();
+  int p = 0, tot = 0;
+  for (int w = 0; w < nw; w++) {
+    const int k = s_tmp[w];
+    if (w < wave) p += k;
+    tot += k;
+  }
+  __syncthreads();
+  const unsigned long long below = (1ull << lane) - 1ull;
+  for (int c = i0; c < i1; c += 64) {
+    const int i = c + lane;
+    const bool b = i < i1 && keep(i);
+    const unsigned long long m = __ballot(b);
+    if (b) put(i, p + __popcll(m & below));
+    p += __popcll(m);
+  }
+  return tot;
+}
+
+// The same over the pairs (k, c) of rows k < nk, columns c in [lo(k),
+// hi(k)), in row-major order: each wave a contiguous range of rows, its 64
+// lanes over a row's columns.  keep(k, c) tests a pair, put(k, c, p) stores
+// it at output position p (from 0).
+template <class Lo, class Hi, class Keep, class Put>
+__device__ inline int wave_compact_pairs(int nk, Lo lo, Hi hi, Keep keep, Put put, int *s_tmp) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+  const int k0 = (int)((long long)nk * wave / nw), k1 = (int)((long long)nk * (wave + 1) / nw);
+  int cnt = 0;
+  for (int k = k0; k < k1; k++) {
+    const int c1 = hi(k);
+    for (int c = lo(k) + lane; c - lane < c1; c += 64) cnt += __popcll(__ballot(c < c1 && keep(k, c)));
+  }
+  if (lane == 0) s_tmp[wave] = cnt;
+  __syncthreads();
+  int p = 0, tot = 0;
+  for (int w = 0; w < nw; w++) {
+    const int k = s_tmp[w];
+    if (w < wave) p += k;
+    tot += k;
+  }
+  __syncthreads();
+  const unsigned long long below = (1ull << lane) - 1ull;
+  for (int k = k0; k < k1; k++) {
+    const int c1 = hi(k);
+    for (int c = lo(k) + lane; c - lane < c1; c += 64) {
+      const bool b = c < c1 && keep(k, c);
+      const unsigned long long m = __ballot(b);
+      if (b) put(k, c, p + __popcll(m & below));
+      p += __popcll(m);
+    }
+  }
+  return tot;
+}
+
 }  // namespace mdn

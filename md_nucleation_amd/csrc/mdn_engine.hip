@@ -333,6 +333,7 @@ __global__ void __launch_bounds__(1024) k_rebuild(const DevReplica *__restrict__
   const DevReplica &R = *Rp;
   __shared__ int s_int[64];
   __shared__ double s_dbl[16];
+  __shared__ double s_big[3 * 1024];
   __shared__ int s_do;
   if (threadIdx.x == 0) {
     int go = force;
@@ -343,7 +344,7 @@ __global__ void __launch_bounds__(1024) k_rebuild(const DevReplica *__restrict__
     s_do = go;
   }
   __syncthreads();
-  if (s_do) rebuild_wg(R, s_int, s_dbl);
+  if (s_do) rebuild_wg(R, s_int, s_dbl, s_big, 3 * 1024);
 }
 
 // wave-reduced energy flush
@@ -1532,7 +1533,9 @@ __global__ void __launch_bounds__(512) k_batched(const DevReplica *__restrict__ 
           if (tid < 3) { R.mol_x[tid] = B.sdbl[tid]; R.mol_v[tid] = B.sdbl[3 + tid]; }
           if (tid == 0) { D->loopPair = lp; D->zeta_ion = zeta; D->step = step_base + s; }
           __syncthreads();
-          rebuild_wg(R, B.sint, B.red);
+          // scratch for the list tests: the positions, velocities and forces
+          // (stored above, loaded again below)
+          rebuild_wg(R, B.sint, B.red, reinterpret_cast<double *>(B.xr), 10 * cap);
           L = view_from_dyn(R);
           lp = D->loopPair;
           lu = D->loop_update;

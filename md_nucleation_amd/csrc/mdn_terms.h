@@ -1041,7 +1041,9 @@ __device__ inline void enter_molecule(const DevReplica &R, int i, int sp, long l
   }
 }
 
-__device__ inline void rebuild_wg(const DevReplica &R, int *s_int, double *s_dbl) {
+// s_int: >= 16 ints, s_dbl: >= 16 doubles, s_big: big_cap doubles of
+// workgroup scratch (LDS) for the list tests
+__device__ inline void rebuild_wg(const DevReplica &R, int *s_int, double *s_dbl, double *s_big, int big_cap) {
 #pragma clang fp contract(off)
   const int tid = threadIdx.x, nt = blockDim.x;
   Dyn *D = R.dyn;
@@ -1115,79 +1117,60 @@ __device__ inline void rebuild_wg(const DevReplica &R, int *s_int, double *s_dbl
   }
   __syncthreads();
   // ordered compaction of the in-domain gas and vapor ids
-  int ng = 0, nv = 0;
-  for (int c0 = 0; c0 < R.n_gas; c0 += nt) {
-    int k = c0 + tid, flag = k < R.n_gas ? R.region[1 + k] : 0, tot;
-    int pre = block_scan(flag, s_int, &tot);
-    if (flag) R.in_gas[ng + pre] = 1 + k;
-    ng += tot;
-  }
-  for (int c0 = 0; c0 < R.n_vap; c0 += nt) {
-    int k = c0 + tid, flag = k < R.n_vap ? R.region[1 + R.n_gas + k] : 0, tot;
-    int pre = block_scan(flag, s_int, &tot);
-    if (flag) R.in_vap[nv + pre] = 1 + R.n_gas + k;
-    nv += tot;
-  }
+  const int ng = wave_compact(
+      R.n_gas, [&](int k) { return R.region[1 + k] != 0; }, [&](int k, int p) { R.in_gas[p] = 1 + k; }, s_int);
+  const int nv = wave_compact(
+      R.n_vap, [&](int k) { return R.region[1 + R.n_gas + k] != 0; },
+      [&](int k, int p) { R.in_vap[p] = 1 + R.n_gas + k; }, s_int);
   __syncthreads();
   // Potential::makePair lists (GasIon.cpp:41-60, VaporGas.cpp:47-64, gas-gas),
-  // ascending gas molecule then partner, stored as in-list positions
+  // ascending gas molecule then partner, stored as in-list positions.  The
+  // tests read the ion's atoms and the in-domain COMs from LDS when they fit
+  // (global loads would put a memory latency on every test); gi and gv are
+  // compacted over their row-major index, so that rows shorter than a wave
+  // (a one-atom ion, a few vapor molecules) still fill its lanes.
   const int n_ion = R.sp[0].natoms;
   int n_gi = 0, n_gv = 0, n_gg = 0;
-  for (int c0 = 0; c0 < ng; c0 += nt) {
-    int k = c0 + tid;
-    bool act = k < ng;
-    const double *g = act ? mol + 3 * (size_t)R.in_gas[k] : mol;
-    int cgi = 0, cgv = 0, cgg = 0;
-    if (act && (R.flags & B_GI))
-      for (int j = 0; j < n_ion; j++) {
-        const double *xj = R.atom_x + 3 * (size_t)j;
-        double dx = g[0] - xj[0], dy = g[1] - xj[1], dz = g[2] - xj[2];
-        if ((dx * dx + dy * dy + dz * dz) < R.ML2) cgi++;
-      }
-    if (act && (R.flags & B_GV))
-      for (int l = 0; l < nv; l++) {
-        const double *h = mol + 3 * (size_t)R.in_vap[l];
-        double dx = g[0] - h[0], dy = g[1] - h[1], dz = g[2] - h[2];
-        if ((dx * dx + dy * dy + dz * dz) < R.ML2) cgv++;
-      }
-    if (act && (R.flags & B_GG))
-      for (int l = k + 1; l < ng; l++) {
-        const double *h = mol + 3 * (size_t)R.in_gas[l];
-        double dx = g[0] - h[0], dy = g[1] - h[1], dz = g[2] - h[2];
-        if ((dx * dx + dy * dy + dz * dz) < R.ML2) cgg++;
-      }
-    int t1, t2, t3;
-    int p1 = block_scan(cgi, s_int, &t1) + n_gi;
-    int p2 = block_scan(cgv, s_int, &t2) + n_gv;
-    int p3 = block_scan(cgg, s_int, &t3) + n_gg;
-    if (act && cgi)
-      for (int j = 0; j < n_ion; j++) {
-        const double *xj = R.atom_x + 3 * (size_t)j;
-        double dx = g[0] - xj[0], dy = g[1] - xj[1], dz = g[2] - xj[2];
-        if ((dx * dx + dy * dy + dz * dz) < R.ML2) {
-          if (p1 < R.cap_gi) { R.gi[2 * p1] = k; R.gi[2 * p1 + 1] = j; }
-          p1++;
-        }
-      }
-    if (act && cgv)
-      for (int l = 0; l < nv; l++) {
-        const double *h = mol + 3 * (size_t)R.in_vap[l];
-        double dx = g[0] - h[0], dy = g[1] - h[1], dz = g[2] - h[2];
-        if ((dx * dx + dy * dy + dz * dz) < R.ML2) {
-          if (p2 < R.cap_gv) { R.gv[2 * p2] = k; R.gv[2 * p2 + 1] = l; }
-          p2++;
-        }
-      }
-    if (act && cgg)
-      for (int l = k + 1; l < ng; l++) {
-        const double *h = mol + 3 * (size_t)R.in_gas[l];
-        double dx = g[0] - h[0], dy = g[1] - h[1], dz = g[2] - h[2];
-        if ((dx * dx + dy * dy + dz * dz) < R.ML2) {
-          if (p3 < R.cap_gg) { R.gg[2 * p3] = k; R.gg[2 * p3 + 1] = l; }
-          p3++;
-        }
-      }
-    n_gi += t1; n_gv += t2; n_gg += t3;
+  auto lists = [&](auto ionp, auto gasp, auto vapp) {
+    auto near = [&](const double *g, const double *h) {
+#pragma clang fp contract(off)
+      double dx = g[0] - h[0], dy = g[1] - h[1], dz = g[2] - h[2];
+      return (dx * dx + dy * dy + dz * dz) < R.ML2;
+    };
+    if ((R.flags & B_GI) && n_ion > 0)
+      n_gi = wave_compact(
+          ng * n_ion, [&](int f) { const int k = f / n_ion; return near(gasp(k), ionp(f - k * n_ion)); },
+          [&](int f, int p) {
+            const int k = f / n_ion;
+            if (p < R.cap_gi) { R.gi[2 * p] = k; R.gi[2 * p + 1] = f - k * n_ion; }
+          },
+          s_int);
+    if ((R.flags & B_GV) && nv > 0)
+      n_gv = wave_compact(
+          ng * nv, [&](int f) { const int k = f / nv; return near(gasp(k), vapp(f - k * nv)); },
+          [&](int f, int p) {
+            const int k = f / nv;
+            if (p < R.cap_gv) { R.gv[2 * p] = k; R.gv[2 * p + 1] = f - k * nv; }
+          },
+          s_int);
+    if (R.flags & B_GG)
+      n_gg = wave_compact_pairs(
+          ng, [](int k) { return k + 1; }, [&](int) { return ng; }, [&](int k, int l) { return near(gasp(k), gasp(l)); },
+          [&](int k, int l, int p) {
+            if (p < R.cap_gg) { R.gg[2 * p] = k; R.gg[2 * p + 1] = l; }
+          },
+          s_int);
+  };
+  if (3 * (n_ion + ng + nv) <= big_cap) {
+    double *si = s_big, *sg = s_big + 3 * n_ion, *sv = sg + 3 * ng;
+    for (int t = tid; t < 3 * n_ion; t += nt) si[t] = R.atom_x[t];
+    for (int t = tid; t < 3 * ng; t += nt) sg[t] = mol[3 * (size_t)R.in_gas[t / 3] + t % 3];
+    for (int t = tid; t < 3 * nv; t += nt) sv[t] = mol[3 * (size_t)R.in_vap[t / 3] + t % 3];
+    __syncthreads();
+    lists([&](int j) { return si + 3 * j; }, [&](int k) { return sg + 3 * k; }, [&](int l) { return sv + 3 * l; });
+  } else {
+    lists([&](int j) { return R.atom_x + 3 * (size_t)j; }, [&](int k) { return mol + 3 * (size_t)R.in_gas[k]; },
+          [&](int l) { return mol + 3 * (size_t)R.in_vap[l]; });
   }
   // v_max over every molecule (pairlist.cpp:38-46); max is order-free
   double vm = 0.0;
