@@ -1304,294 +1304,300 @@ template <typename Rl, bool MB>
 __global__ void __launch_bounds__(512) k_batched(const DevReplica *__restrict__ reps,
                                                  const int *__restrict__ order, int n_rep, int nsteps,
                                                  int eflag_every, long long step0, int cap, int cap_h, int kcap,
-                                                 int acap, int scap) {
+                                                 int acap, int scap, int4 *__restrict__ cnt) {
   extern __shared__ __align__(16) char smem[];
   BatchLds<Rl> B = carve<Rl>(smem, cap, cap_h, kcap, acap, scap);
   const int tid = threadIdx.x, nt = blockDim.x;
   for (int rr = blockIdx.x; rr < n_rep; rr += gridDim.x) {
     const DevReplica &R = reps[order[rr]];
     Dyn *D = R.dyn;
-    if (D->status & ST_LDS_CAP) continue;
-    LView L = view_from_dyn(R);
-    if (L.n_in > cap || L.gbase + 64 > cap_h || L.n_ion + L.na_v + L.na_g > kcap ||
-        ((R.flags & B_AMBER) && L.n_ion * R.amber_words > acap)) {
-      if (tid == 0) atomicOr(&D->status, ST_LDS_CAP);
-      continue;
-    }
-    batch_load(R, L, B);
-    int lp = D->loopPair, lu = D->loop_update;
-    long long acc_steps = 0;
-    const DevSpecies &I = R.sp[0];
-    const bool shake = R.shake != 0;
-    // ion thermostat (nvt.cpp): the mode is uniform over the launch, zeta
-    // lives in a register and reaches the replica with the state
-    const int thermo = D->thermo;
-    const double Tnh = D->Tnh_ion;
-    double zeta = D->zeta_ion;
-    const long long step_base = D->step;
-    // SHAKE diagnostics accumulate in LDS (sint[32]: max iterations,
-    // sint[33]: failures) and reach the replica's counters once per launch
-    if (tid == 0) { B.sint[32] = 0; B.sint[33] = 0; }
-    __syncthreads();
-    bool dead = false;
-    // under NVE the second half-kick of a step is carried into the first of
-    // the next (the same per-atom operations in the same order: nothing
-    // between them reads the velocities); the last step of a launch applies
-    // its own
-    bool pend = false;
-    for (int s = 0; s < nsteps && !dead; s++) {
-      const bool eflag = eflag_every > 0 && ((step0 + s + 1) % eflag_every) == 0;
-      // MD::velocity_scaling (verlet.cpp:22): the ion is kicked first, its
-      // temperature reduced and its velocities scaled to 300 K; the fused
-      // loop below then only drifts it (kick coefficient 0)
-      const bool vscale = thermo == THERMO_SCALE;
-      if (vscale) {
-        double k2 = 0;
-        for (int a = tid; a < L.n_ion; a += nt) {
-          double kc = B.kct[B.kidx[a]], m = B.kms[a];
+    {
+      if (D->status & ST_LDS_CAP) goto report;
+      LView L = view_from_dyn(R);
+      if (L.n_in > cap || L.gbase + 64 > cap_h || L.n_ion + L.na_v + L.na_g > kcap ||
+          ((R.flags & B_AMBER) && L.n_ion * R.amber_words > acap)) {
+        if (tid == 0) atomicOr(&D->status, ST_LDS_CAP);
+        goto report;
+      }
+      batch_load(R, L, B);
+      int lp = D->loopPair, lu = D->loop_update;
+      long long acc_steps = 0;
+      const DevSpecies &I = R.sp[0];
+      const bool shake = R.shake != 0;
+      // ion thermostat (nvt.cpp): the mode is uniform over the launch, zeta
+      // lives in a register and reaches the replica with the state
+      const int thermo = D->thermo;
+      const double Tnh = D->Tnh_ion;
+      double zeta = D->zeta_ion;
+      const long long step_base = D->step;
+      // SHAKE diagnostics accumulate in LDS (sint[32]: max iterations,
+      // sint[33]: failures) and reach the replica's counters once per launch
+      if (tid == 0) { B.sint[32] = 0; B.sint[33] = 0; }
+      __syncthreads();
+      bool dead = false;
+      // under NVE the second half-kick of a step is carried into the first of
+      // the next (the same per-atom operations in the same order: nothing
+      // between them reads the velocities); the last step of a launch applies
+      // its own
+      bool pend = false;
+      for (int s = 0; s < nsteps && !dead; s++) {
+        const bool eflag = eflag_every > 0 && ((step0 + s + 1) % eflag_every) == 0;
+        // MD::velocity_scaling (verlet.cpp:22): the ion is kicked first, its
+        // temperature reduced and its velocities scaled to 300 K; the fused
+        // loop below then only drifts it (kick coefficient 0)
+        const bool vscale = thermo == THERMO_SCALE;
+        if (vscale) {
+          double k2 = 0;
+          for (int a = tid; a < L.n_ion; a += nt) {
+            double kc = B.kct[B.kidx[a]], m = B.kms[a];
+            for (int q = 0; q < 3; q++) {
+              double v = B.v[3 * a + q] + B.f[3 * a + q] * kc;
+              B.v[3 * a + q] = v;
+              k2 += v * v * m;
+            }
+          }
+          k2 = wave_sum(k2);
+          if ((tid & 63) == 0) B.red[tid >> 6] = k2;
+          __syncthreads();
+          double sv = 0;
+          for (int w = 0; w < nt / 64; w++) sv += B.red[w];
+          const double ratio = sqrt(300.0 / ion_temperature(sv, I.natoms));
+          for (int a = tid; a < L.n_ion; a += nt)
+            for (int q = 0; q < 3; q++) B.v[3 * a + q] *= ratio;
+        }
+        // first half-kick + drift + force clear (verlet.cpp:21-25), after the
+        // previous step's second half-kick when it was carried over
+        for (int a = tid; a < L.n_in; a += nt) {
+          const double kf = B.kct[B.kidx[a]];
+          double kc = vscale && a < L.n_ion ? 0.0 : kf;
+          bool keep = shake && a < L.gbase;
           for (int q = 0; q < 3; q++) {
-            double v = B.v[3 * a + q] + B.f[3 * a + q] * kc;
+            double v = B.v[3 * a + q];
+            if (pend) v = v + B.f[3 * a + q] * kf;
+            v = v + B.f[3 * a + q] * kc;
             B.v[3 * a + q] = v;
-            k2 += v * v * m;
+            if (keep) B.xo[3 * a + q] = B.xr[a].x[q];
+            B.xr[a].x[q] += v * R.dt;
+            B.f[3 * a + q] = 0.0;
           }
         }
-        k2 = wave_sum(k2);
-        if ((tid & 63) == 0) B.red[tid >> 6] = k2;
         __syncthreads();
-        double sv = 0;
-        for (int w = 0; w < nt / 64; w++) sv += B.red[w];
-        const double ratio = sqrt(300.0 / ion_temperature(sv, I.natoms));
-        for (int a = tid; a < L.n_ion; a += nt)
-          for (int q = 0; q < 3; q++) B.v[3 * a + q] *= ratio;
-      }
-      // first half-kick + drift + force clear (verlet.cpp:21-25), after the
-      // previous step's second half-kick when it was carried over
-      for (int a = tid; a < L.n_in; a += nt) {
-        const double kf = B.kct[B.kidx[a]];
-        double kc = vscale && a < L.n_ion ? 0.0 : kf;
-        bool keep = shake && a < L.gbase;
-        for (int q = 0; q < 3; q++) {
-          double v = B.v[3 * a + q];
-          if (pend) v = v + B.f[3 * a + q] * kf;
-          v = v + B.f[3 * a + q] * kc;
-          B.v[3 * a + q] = v;
-          if (keep) B.xo[3 * a + q] = B.xr[a].x[q];
-          B.xr[a].x[q] += v * R.dt;
-          B.f[3 * a + q] = 0.0;
-        }
-      }
-      __syncthreads();
-      SStoreT<Rl> ss;
-      ss.L = L; ss.xr = B.xr; ss.pf = B.f; ss.pk = B.pk; ss.kpk = B.kpk; ss.kidx = B.kidx;
-      ss.ptype = R.sp[0].type;
-      double e[U_N] = {0, 0, 0, 0, 0, 0, 0, 0};
-      // SHAKE of one molecule whose first local atom is base
-      auto shake_one = [&](const DevSpecies &S, int base) {
-        XRec<Rl> *x = B.xr + base;
-        double *v = B.v + 3 * base;
-        const double *xo = B.xo + 3 * base;
-        auto X = [&](int i, int q) -> double & { return x[i].x[q]; };
-        auto XO = [&](int i, int q) -> double { return xo[3 * i + q]; };
-        auto V = [&](int i, int q) -> double & { return v[3 * i + q]; };
-        const int sidx = &S == &R.sp[0] ? 0 : 1;
-        int it = shake_molecule_lds(R, B.shk, sidx, X, XO, R.shake_max_iter);
-        if (it < 0) {
-          atomicAdd(&B.sint[33], 1);
-          atomicMax(&B.sint[32], -it);
-          shake_undo(S, X, XO, V, R.dt);
-          return;
-        }
-        atomicMax(&B.sint[32], it);
-        if (R.shake == 1) shake_velocity_lds(B.shk, sidx, S.natoms, X, XO, V, R.dt);
-      };
-      // the same for constraint group g of G of a vapor molecule (one lane
-      // per group): the group's own SHAKE velocities; the first lane counts
-      // and, on a failure, undoes the whole molecule
-      auto shake_grp = [&](const DevSpecies &S, int base, int g, int G) {
-        XRec<Rl> *x = B.xr + base;
-        double *v = B.v + 3 * base;
-        const double *xo = B.xo + 3 * base;
-        auto X = [&](int i, int q) -> double & { return x[i].x[q]; };
-        auto XO = [&](int i, int q) -> double { return xo[3 * i + q]; };
-        auto V = [&](int i, int q) -> double & { return v[3 * i + q]; };
-        int it = shake_group_lds(R, B.shk, 1, g, G, X, XO, R.shake_max_iter);
-        if (it < 0) {
-          if (g == 0) {
+        SStoreT<Rl> ss;
+        ss.L = L; ss.xr = B.xr; ss.pf = B.f; ss.pk = B.pk; ss.kpk = B.kpk; ss.kidx = B.kidx;
+        ss.ptype = R.sp[0].type;
+        double e[U_N] = {0, 0, 0, 0, 0, 0, 0, 0};
+        // SHAKE of one molecule whose first local atom is base
+        auto shake_one = [&](const DevSpecies &S, int base) {
+          XRec<Rl> *x = B.xr + base;
+          double *v = B.v + 3 * base;
+          const double *xo = B.xo + 3 * base;
+          auto X = [&](int i, int q) -> double & { return x[i].x[q]; };
+          auto XO = [&](int i, int q) -> double { return xo[3 * i + q]; };
+          auto V = [&](int i, int q) -> double & { return v[3 * i + q]; };
+          const int sidx = &S == &R.sp[0] ? 0 : 1;
+          int it = shake_molecule_lds(R, B.shk, sidx, X, XO, R.shake_max_iter);
+          if (it < 0) {
             atomicAdd(&B.sint[33], 1);
             atomicMax(&B.sint[32], -it);
             shake_undo(S, X, XO, V, R.dt);
+            return;
           }
-          return;
-        }
-        if (g == 0) atomicMax(&B.sint[32], it);
-        if (R.shake == 1) {
-          const double inv_dt = 1.0 / R.dt;
-          const unsigned long long gm = B.shk.gm[4 + g];
-          for (int a = 0; a < S.natoms; a++)
-            if ((gm >> a) & 1ull)
-              for (int q = 0; q < 3; q++) V(a, q) = (X(a, q) - XO(a, q)) * inv_dt;
-        }
-      };
-      const DevSpecies &Vs = R.sp[2];
-      const int nwv = nt >> 6;
-      // A step with constrained vapor, an unconstrained ion and no rebuild
-      // due: the vapor SHAKE (one thread per molecule, latency-bound) runs
-      // on the last wave(s) while the other waves compute the ion COM, the
-      // gas weights and the forces that involve no vapor position (phase A);
-      // then the vapor weights and the remaining forces (phase B).  The
-      // forces are the same terms as in the sequential order; only their
-      // accumulation order differs.
-      const bool overlap =
-          shake && Vs.nrigid && !I.nrigid && L.nv > 0 && nwv >= 2 && lp + 1 <= lu && thermo == THERMO_NVE;
-      if (overlap) {
-        // a vapor molecule whose constraints fall into G groups that share
-        // no atom takes G adjacent lanes, one group each
-        const int G = __builtin_amdgcn_readfirstlane(B.shk.ng[1]);
-        const int gsh = G >= 4 ? 2 : G >= 2 ? 1 : 0;
-        const int nS = min(nwv - 1, ((L.nv << gsh) + 63) >> 6), nA = nwv - nS;
-        const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-        if (wave >= nA) {
-          // the SHAKE chain is a long run of dependent instructions: let it
-          // win instruction issue on its SIMD over the waves of force work
-          __builtin_amdgcn_s_setprio(2);
-          if (gsh == 0)
-            for (int m = tid - 64 * nA; m < L.nv; m += 64 * nS) shake_one(Vs, L.vbase + m * L.na_v);
-          else
-            for (int slot = tid - 64 * nA; slot < (L.nv << gsh); slot += 64 * nS)
-              shake_grp(Vs, L.vbase + (slot >> gsh) * L.na_v, slot & ((1 << gsh) - 1), 1 << gsh);
-          __builtin_amdgcn_s_setprio(0);
+          atomicMax(&B.sint[32], it);
+          if (R.shake == 1) shake_velocity_lds(B.shk, sidx, S.natoms, X, XO, V, R.dt);
+        };
+        // the same for constraint group g of G of a vapor molecule (one lane
+        // per group): the group's own SHAKE velocities; the first lane counts
+        // and, on a failure, undoes the whole molecule
+        auto shake_grp = [&](const DevSpecies &S, int base, int g, int G) {
+          XRec<Rl> *x = B.xr + base;
+          double *v = B.v + 3 * base;
+          const double *xo = B.xo + 3 * base;
+          auto X = [&](int i, int q) -> double & { return x[i].x[q]; };
+          auto XO = [&](int i, int q) -> double { return xo[3 * i + q]; };
+          auto V = [&](int i, int q) -> double & { return v[3 * i + q]; };
+          int it = shake_group_lds(R, B.shk, 1, g, G, X, XO, R.shake_max_iter);
+          if (it < 0) {
+            if (g == 0) {
+              atomicAdd(&B.sint[33], 1);
+              atomicMax(&B.sint[32], -it);
+              shake_undo(S, X, XO, V, R.dt);
+            }
+            return;
+          }
+          if (g == 0) atomicMax(&B.sint[32], it);
+          if (R.shake == 1) {
+            const double inv_dt = 1.0 / R.dt;
+            const unsigned long long gm = B.shk.gm[4 + g];
+            for (int a = 0; a < S.natoms; a++)
+              if ((gm >> a) & 1ull)
+                for (int q = 0; q < 3; q++) V(a, q) = (X(a, q) - XO(a, q)) * inv_dt;
+          }
+        };
+        const DevSpecies &Vs = R.sp[2];
+        const int nwv = nt >> 6;
+        // A step with constrained vapor, an unconstrained ion and no rebuild
+        // due: the vapor SHAKE (one thread per molecule, latency-bound) runs
+        // on the last wave(s) while the other waves compute the ion COM, the
+        // gas weights and the forces that involve no vapor position (phase A);
+        // then the vapor weights and the remaining forces (phase B).  The
+        // forces are the same terms as in the sequential order; only their
+        // accumulation order differs.
+        const bool overlap =
+            shake && Vs.nrigid && !I.nrigid && L.nv > 0 && nwv >= 2 && lp + 1 <= lu && thermo == THERMO_NVE;
+        if (overlap) {
+          // a vapor molecule whose constraints fall into G groups that share
+          // no atom takes G adjacent lanes, one group each
+          const int G = __builtin_amdgcn_readfirstlane(B.shk.ng[1]);
+          const int gsh = G >= 4 ? 2 : G >= 2 ? 1 : 0;
+          const int nS = min(nwv - 1, ((L.nv << gsh) + 63) >> 6), nA = nwv - nS;
+          const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+          if (wave >= nA) {
+            // the SHAKE chain is a long run of dependent instructions: let it
+            // win instruction issue on its SIMD over the waves of force work
+            __builtin_amdgcn_s_setprio(2);
+            if (gsh == 0)
+              for (int m = tid - 64 * nA; m < L.nv; m += 64 * nS) shake_one(Vs, L.vbase + m * L.na_v);
+            else
+              for (int slot = tid - 64 * nA; slot < (L.nv << gsh); slot += 64 * nS)
+                shake_grp(Vs, L.vbase + (slot >> gsh) * L.na_v, slot & ((1 << gsh) - 1), 1 << gsh);
+            __builtin_amdgcn_s_setprio(0);
+          } else {
+            // ion COM (verlet.cpp:90-107) on wave 0; no phase-A term reads a
+            // weight, so the blend weights wait for the barrier
+            if (wave == 0) {
+              double c[6] = {0, 0, 0, 0, 0, 0};
+              for (int i = tid; i < I.natoms; i += 64) {
+                double m = B.kms[i];
+                for (int q = 0; q < 3; q++) { c[q] += B.xr[i].x[q] * m; c[3 + q] += B.v[3 * i + q] * m; }
+              }
+              for (int k = 0; k < 6; k++) c[k] = wave_sum_full(c[k]) * (1 / I.molmass);
+              if (tid < 6) B.sdbl[tid] = c[tid];
+            }
+            batch_terms_a<Rl, MB>(R, ss, B, tid, 0, 64 * nA, eflag, e);
+          }
+          __syncthreads();
+          {
+            // blend weights of the vapor and gas atoms (verlet.cpp:109-134)
+            const double cx = B.sdbl[0], cy = B.sdbl[1], cz = B.sdbl[2];
+            for (int a = L.n_ion + tid; a < L.n_in; a += nt) {
+              double dx = B.xr[a].x[0] - cx, dy = B.xr[a].x[1] - cy, dz = B.xr[a].x[2] - cz;
+              B.xr[a].w = blend_weight_r<Rl>(dx * dx + dy * dy + dz * dz);
+            }
+          }
+          lp++;  // check_pairlist: no rebuild due (pairlist.cpp:58-63)
+          __syncthreads();
+          batch_terms_b<Rl>(R, ss, B, tid, nt, eflag, e);
         } else {
-          // ion COM (verlet.cpp:90-107) on wave 0; no phase-A term reads a
-          // weight, so the blend weights wait for the barrier
-          if (wave == 0) {
-            double c[6] = {0, 0, 0, 0, 0, 0};
-            for (int i = tid; i < I.natoms; i += 64) {
+          if (shake) {
+            // constrained molecules: the vapor and the ion (gas has no rigid
+            // pairs)
+            if (Vs.nrigid)
+              for (int m = tid; m < L.nv; m += nt) shake_one(Vs, L.vbase + m * L.na_v);
+            if (I.nrigid && tid == nt - 1) shake_one(I, 0);
+            __syncthreads();
+          }
+          // ion COM (verlet.cpp:90-107); under Nose-Hoover also sum m v^2 of
+          // the ion for MD::nosehoover_zeta (verlet.cpp:27)
+          {
+            const int nk = thermo == THERMO_NH ? 7 : 6;
+            double c[7] = {0, 0, 0, 0, 0, 0, 0};
+            for (int i = tid; i < I.natoms; i += nt) {
               double m = B.kms[i];
               for (int q = 0; q < 3; q++) { c[q] += B.xr[i].x[q] * m; c[3 + q] += B.v[3 * i + q] * m; }
+              c[6] += B.v[3 * i] * B.v[3 * i] * m + B.v[3 * i + 1] * B.v[3 * i + 1] * m +
+                      B.v[3 * i + 2] * B.v[3 * i + 2] * m;
             }
-            for (int k = 0; k < 6; k++) c[k] = wave_sum_full(c[k]) * (1 / I.molmass);
-            if (tid < 6) B.sdbl[tid] = c[tid];
+            for (int k = 0; k < nk; k++) {
+              double sv = wave_sum(c[k]);
+              if ((tid & 63) == 0) B.red[k * 16 + (tid >> 6)] = sv;
+            }
+            __syncthreads();
+            if (tid < nk) {
+              double sv = 0;
+              for (int w = 0; w < nt / 64; w++) sv += B.red[tid * 16 + w];
+              B.sdbl[tid] = tid < 6 ? sv * (1 / I.molmass) : sv;
+            }
+            __syncthreads();
+            if (thermo == THERMO_NH)
+              zeta = nh_zeta_step(zeta, ion_temperature(B.sdbl[6], I.natoms), Tnh, I.natoms, R.dt);
           }
-          batch_terms_a<Rl, MB>(R, ss, B, tid, 0, 64 * nA, eflag, e);
-        }
-        __syncthreads();
-        {
-          // blend weights of the vapor and gas atoms (verlet.cpp:109-134)
           const double cx = B.sdbl[0], cy = B.sdbl[1], cz = B.sdbl[2];
+          // blend weights (verlet.cpp:109-134)
           for (int a = L.n_ion + tid; a < L.n_in; a += nt) {
             double dx = B.xr[a].x[0] - cx, dy = B.xr[a].x[1] - cy, dz = B.xr[a].x[2] - cz;
             B.xr[a].w = blend_weight_r<Rl>(dx * dx + dy * dy + dz * dz);
           }
+          // check_pairlist (pairlist.cpp:58-63)
+          lp++;
+          if (lp > lu) {
+            __syncthreads();
+            batch_store(R, L, B);
+            if (tid < 3) { R.mol_x[tid] = B.sdbl[tid]; R.mol_v[tid] = B.sdbl[3 + tid]; }
+            if (tid == 0) { D->loopPair = lp; D->zeta_ion = zeta; D->step = step_base + s; }
+            __syncthreads();
+            // scratch for the list tests: the positions, velocities and forces
+            // (stored above, loaded again below)
+            rebuild_wg(R, B.sint, B.red, reinterpret_cast<double *>(B.xr), 10 * cap);
+            L = view_from_dyn(R);
+            lp = D->loopPair;
+            lu = D->loop_update;
+            if (L.n_in > cap || L.gbase + 64 > cap_h) {
+              // the stored state is step s in progress (after the rebuild):
+              // the per-call path finishes it (mdn_verlet)
+              if (tid == 0) { atomicOr(&D->status, ST_LDS_CAP); D->pending = 1; }
+              dead = true;
+              break;
+            }
+            batch_load(R, L, B);
+            ss.L = L;
+          }
+          __syncthreads();
+          // forces of every registered potential (verlet.cpp:31-32)
+          batch_terms<Rl, MB>(R, ss, B, tid, nt, eflag, e);
         }
-        lp++;  // check_pairlist: no rebuild due (pairlist.cpp:58-63)
         __syncthreads();
-        batch_terms_b<Rl>(R, ss, B, tid, nt, eflag, e);
-      } else {
-        if (shake) {
-          // constrained molecules: the vapor and the ion (gas has no rigid
-          // pairs)
-          if (Vs.nrigid)
-            for (int m = tid; m < L.nv; m += nt) shake_one(Vs, L.vbase + m * L.na_v);
-          if (I.nrigid && tid == nt - 1) shake_one(I, 0);
-          __syncthreads();
+        // second half-kick (verlet.cpp:34), then MD::nosehoover_ion on the ion
+        // (verlet.cpp:36)
+        pend = thermo == THERMO_NVE && s + 1 < nsteps;
+        if (!pend) {
+          const double fac = thermo == THERMO_NH ? nh_factor(zeta, R.dt) : 1.0;
+          for (int a = tid; a < L.n_in; a += nt) {
+            double kc = B.kct[B.kidx[a]];
+            if (thermo == THERMO_NH && a < L.n_ion)
+              for (int q = 0; q < 3; q++) B.v[3 * a + q] = (B.v[3 * a + q] + B.f[3 * a + q] * kc) * fac;
+            else
+              for (int q = 0; q < 3; q++) B.v[3 * a + q] += B.f[3 * a + q] * kc;
+          }
         }
-        // ion COM (verlet.cpp:90-107); under Nose-Hoover also sum m v^2 of
-        // the ion for MD::nosehoover_zeta (verlet.cpp:27)
-        {
-          const int nk = thermo == THERMO_NH ? 7 : 6;
-          double c[7] = {0, 0, 0, 0, 0, 0, 0};
-          for (int i = tid; i < I.natoms; i += nt) {
-            double m = B.kms[i];
-            for (int q = 0; q < 3; q++) { c[q] += B.xr[i].x[q] * m; c[3 + q] += B.v[3 * i + q] * m; }
-            c[6] += B.v[3 * i] * B.v[3 * i] * m + B.v[3 * i + 1] * B.v[3 * i + 1] * m +
-                    B.v[3 * i + 2] * B.v[3 * i + 2] * m;
-          }
-          for (int k = 0; k < nk; k++) {
-            double sv = wave_sum(c[k]);
-            if ((tid & 63) == 0) B.red[k * 16 + (tid >> 6)] = sv;
+        acc_steps += L.n_in;
+        if (eflag) {
+          for (int k = 0; k < U_N; k++) {
+            double sv = wave_sum(e[k]);
+            if ((tid & 63) == 0) B.red[(6 + k) * 16 + (tid >> 6)] = sv;
           }
           __syncthreads();
-          if (tid < nk) {
+          if (tid < U_N) {
             double sv = 0;
-            for (int w = 0; w < nt / 64; w++) sv += B.red[tid * 16 + w];
-            B.sdbl[tid] = tid < 6 ? sv * (1 / I.molmass) : sv;
+            for (int w = 0; w < nt / 64; w++) sv += B.red[(6 + tid) * 16 + w];
+            D->U[tid] = sv;
           }
-          __syncthreads();
-          if (thermo == THERMO_NH) zeta = nh_zeta_step(zeta, ion_temperature(B.sdbl[6], I.natoms), Tnh, I.natoms, R.dt);
-        }
-        const double cx = B.sdbl[0], cy = B.sdbl[1], cz = B.sdbl[2];
-        // blend weights (verlet.cpp:109-134)
-        for (int a = L.n_ion + tid; a < L.n_in; a += nt) {
-          double dx = B.xr[a].x[0] - cx, dy = B.xr[a].x[1] - cy, dz = B.xr[a].x[2] - cz;
-          B.xr[a].w = blend_weight_r<Rl>(dx * dx + dy * dy + dz * dz);
-        }
-        // check_pairlist (pairlist.cpp:58-63)
-        lp++;
-        if (lp > lu) {
-          __syncthreads();
-          batch_store(R, L, B);
-          if (tid < 3) { R.mol_x[tid] = B.sdbl[tid]; R.mol_v[tid] = B.sdbl[3 + tid]; }
-          if (tid == 0) { D->loopPair = lp; D->zeta_ion = zeta; D->step = step_base + s; }
-          __syncthreads();
-          // scratch for the list tests: the positions, velocities and forces
-          // (stored above, loaded again below)
-          rebuild_wg(R, B.sint, B.red, reinterpret_cast<double *>(B.xr), 10 * cap);
-          L = view_from_dyn(R);
-          lp = D->loopPair;
-          lu = D->loop_update;
-          if (L.n_in > cap || L.gbase + 64 > cap_h) {
-            // the stored state is step s in progress (after the rebuild):
-            // the per-call path finishes it (mdn_verlet)
-            if (tid == 0) { atomicOr(&D->status, ST_LDS_CAP); D->pending = 1; }
-            dead = true;
-            break;
-          }
-          batch_load(R, L, B);
-          ss.L = L;
         }
         __syncthreads();
-        // forces of every registered potential (verlet.cpp:31-32)
-        batch_terms<Rl, MB>(R, ss, B, tid, nt, eflag, e);
       }
-      __syncthreads();
-      // second half-kick (verlet.cpp:34), then MD::nosehoover_ion on the ion
-      // (verlet.cpp:36)
-      pend = thermo == THERMO_NVE && s + 1 < nsteps;
-      if (!pend) {
-        const double fac = thermo == THERMO_NH ? nh_factor(zeta, R.dt) : 1.0;
-        for (int a = tid; a < L.n_in; a += nt) {
-          double kc = B.kct[B.kidx[a]];
-          if (thermo == THERMO_NH && a < L.n_ion)
-            for (int q = 0; q < 3; q++) B.v[3 * a + q] = (B.v[3 * a + q] + B.f[3 * a + q] * kc) * fac;
-          else
-            for (int q = 0; q < 3; q++) B.v[3 * a + q] += B.f[3 * a + q] * kc;
-        }
+      if (!dead) {
+        batch_store(R, L, B);
+        if (tid < 3) { R.mol_x[tid] = B.sdbl[tid]; R.mol_v[tid] = B.sdbl[3 + tid]; }
+        if (tid == 0) { D->loopPair = lp; D->zeta_ion = zeta; D->step = step_base + nsteps; }
       }
-      acc_steps += L.n_in;
-      if (eflag) {
-        for (int k = 0; k < U_N; k++) {
-          double sv = wave_sum(e[k]);
-          if ((tid & 63) == 0) B.red[(6 + k) * 16 + (tid >> 6)] = sv;
-        }
-        __syncthreads();
-        if (tid < U_N) {
-          double sv = 0;
-          for (int w = 0; w < nt / 64; w++) sv += B.red[(6 + tid) * 16 + w];
-          D->U[tid] = sv;
-        }
+      if (tid == 0) {
+        D->atom_steps += acc_steps;
+        atomicMax(&D->shake_iters_max, B.sint[32]);
+        if (B.sint[33]) atomicAdd(&D->shake_fail, (unsigned long long)B.sint[33]);
       }
-      __syncthreads();
     }
-    if (!dead) {
-      batch_store(R, L, B);
-      if (tid < 3) { R.mol_x[tid] = B.sdbl[tid]; R.mol_v[tid] = B.sdbl[3 + tid]; }
-      if (tid == 0) { D->loopPair = lp; D->zeta_ion = zeta; D->step = step_base + nsteps; }
-    }
-    if (tid == 0) {
-      D->atom_steps += acc_steps;
-      atomicMax(&D->shake_iters_max, B.sint[32]);
-      if (B.sint[33]) atomicAdd(&D->shake_fail, (unsigned long long)B.sint[33]);
-    }
+  report:
+    // the counts the host plans the next launch with (thread 0 wrote them)
+    if (tid == 0) cnt[order[rr]] = make_int4(D->n_in_gas, D->n_in_vap, D->pending, 0);
     __syncthreads();
   }
 }
@@ -1689,6 +1695,9 @@ struct mdn_ctx {
   // (read back behind each chunk's launches) and the launch order
   mdn::Dyn *h_dyn = nullptr;
   int *h_order = nullptr;
+  // per replica {in-domain gas, in-domain vapor, pending}: written by the
+  // batched kernel into pinned host memory as each replica's launch ends
+  int4 *h_cnt = nullptr, *d_cnt = nullptr;
   int h_cap = 0;
 };
 
@@ -2210,8 +2219,8 @@ int mdn_create(int device, mdn_ctx **out) {
   HIPCHK(hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking));
   HIPCHK(hipStreamCreateWithFlags(&c->stream3, hipStreamNonBlocking));
   HIPCHK(hipEventCreateWithFlags(&c->evj, hipEventDisableTiming));
-  HIPCHK(hipEventCreateWithFlags(&c->evk, hipEventDisableTiming));
-  HIPCHK(hipEventCreateWithFlags(&c->evl, hipEventDisableTiming));
+  HIPCHK(hipEventCreate(&c->evk));
+  HIPCHK(hipEventCreate(&c->evl));
   hipDeviceProp_t prop;
   HIPCHK(hipGetDeviceProperties(&prop, device));
   c->n_cu = prop.multiProcessorCount;
@@ -2228,6 +2237,7 @@ void mdn_destroy(mdn_ctx *c) {
   if (c->d_jvp) hipFree(c->d_jvp);
   if (c->h_dyn) hipHostFree(c->h_dyn);
   if (c->h_order) hipHostFree(c->h_order);
+  if (c->h_cnt) hipHostFree(c->h_cnt);
   if (c->ev0) hipEventDestroy(c->ev0);
   if (c->ev1) hipEventDestroy(c->ev1);
   if (c->evj) hipEventDestroy(c->evj);
@@ -2742,7 +2752,7 @@ static BatchCaps caps_of(const mdn_ctx *c, const CapNeed &q, size_t rl, size_t b
   b.lds = batch_lds_bytes(b.cap, b.cap_h, kcap, q.acap, q.scap, rl);
   return b;
 }
-static BatchCaps batch_caps(const mdn_ctx *c, const std::vector<Dyn> &dyn, const std::vector<int> &set, size_t rl,
+static BatchCaps batch_caps(const mdn_ctx *c, const Dyn *dyn, const std::vector<int> &set, size_t rl,
                             size_t budget) {
   CapNeed q;
   for (int r : set) q.add(cap_need(c, dyn[r], r));
@@ -2766,7 +2776,7 @@ static int launch_batched(mdn_ctx *c, hipStream_t st, const int *order, int n, i
   }
   int grid = c->batch_grid > 0 ? std::min(c->batch_grid, n) : n;
   hipLaunchKernelGGL(kern, dim3(grid), dim3(threads), b.lds, st, c->d_reps, order, n, ns, eflag_every, step0,
-                     b.cap, b.cap_h, b.kcap, b.acap, b.scap);
+                     b.cap, b.cap_h, b.kcap, b.acap, b.scap, c->d_cnt);
   HIPCHK(hipGetLastError());
   return MDN_OK;
 }
@@ -2817,14 +2827,19 @@ int mdn_run_batched(mdn_ctx *c, int32_t nsteps, int32_t eflag_every, int64_t ste
   if (c->h_cap < n) {
     if (c->h_dyn) HIPCHK(hipHostFree(c->h_dyn));
     if (c->h_order) HIPCHK(hipHostFree(c->h_order));
-    c->h_dyn = nullptr; c->h_order = nullptr; c->h_cap = 0;
+    if (c->h_cnt) HIPCHK(hipHostFree(c->h_cnt));
+    c->h_dyn = nullptr; c->h_order = nullptr; c->h_cnt = nullptr; c->d_cnt = nullptr; c->h_cap = 0;
     HIPCHK(hipHostMalloc((void **)&c->h_dyn, sizeof(Dyn) * n, hipHostMallocDefault));
     HIPCHK(hipHostMalloc((void **)&c->h_order, sizeof(int) * n, hipHostMallocDefault));
+    HIPCHK(hipHostMalloc((void **)&c->h_cnt, sizeof(int4) * n, hipHostMallocDefault));
+    HIPCHK(hipHostGetDevicePointer((void **)&c->d_cnt, c->h_cnt, 0));
     c->h_cap = n;
   }
   // the counts of every replica at the start of each chunk: read back
   // behind the previous chunk's launches on the same stream
-  std::vector<Dyn> dyn(c->reps.size());
+  // h_dyn: the device's Dyn records after a read-back; the batched launches
+  // then keep their counts and pending flags current (from h_cnt)
+  const Dyn *dyn = c->h_dyn;
   std::vector<int> order(n);
   c->last_ms = 0;
   if (c->dyn_mut != c->mut) {
@@ -2834,7 +2849,6 @@ int mdn_run_batched(mdn_ctx *c, int32_t nsteps, int32_t eflag_every, int64_t ste
   c->dyn_mut = ~0ull;
   for (int done = 0; done < nsteps; done += chunk) {
     const int ns = std::min(chunk, nsteps - done);
-    memcpy(dyn.data(), c->h_dyn, sizeof(Dyn) * n);
     // Work per step grows with the square of the ion + vapor atom count.  The
     // heaviest replicas get a whole CU each (512-thread workgroups: at 256
     // VGPRs one fills a CU); the rest share CUs in 128-thread workgroups, in
@@ -2966,7 +2980,8 @@ int mdn_run_batched(mdn_ctx *c, int32_t nsteps, int32_t eflag_every, int64_t ste
     if (split) c->plan.age += ns;
     const int H = (int)heavy.size(), NB_ = (int)clsB.size(), NA = (int)clsA.size();
     // the launch order on the device (still there when the plan is reused)
-    if (!(reuse && c->plan_on_device)) {
+    const bool upload = !(reuse && c->plan_on_device);
+    if (upload) {
       memcpy(c->h_order, order.data(), sizeof(int) * n);
       HIPCHK(hipMemcpyAsync(c->d_order, c->h_order, sizeof(int) * n, hipMemcpyHostToDevice, c->stream));
     }
@@ -2976,10 +2991,15 @@ int mdn_run_batched(mdn_ctx *c, int32_t nsteps, int32_t eflag_every, int64_t ste
     int rc;
     if (H + NB_ + NA > 0) {
       // heavy on stream, class B on stream2, class A on stream3, all after
-      // the order upload and joined back into stream
-      HIPCHK(hipEventRecord(c->evj, c->stream));
-      HIPCHK(hipStreamWaitEvent(c->stream2, c->evj, 0));
-      HIPCHK(hipStreamWaitEvent(c->stream3, c->evj, 0));
+      // an order upload of this chunk.  Otherwise the streams hold nothing
+      // else: the previous call ended with all three synchronized, and a
+      // state change since then made this call read back (and synchronize)
+      // first
+      if (upload) {
+        HIPCHK(hipEventRecord(c->evj, c->stream));
+        HIPCHK(hipStreamWaitEvent(c->stream2, c->evj, 0));
+        HIPCHK(hipStreamWaitEvent(c->stream3, c->evj, 0));
+      }
       size_t lds = 0;
       int cap = 0;
       struct Part { hipStream_t st; int off, cnt, threads; size_t budget; const std::vector<int> *set; };
@@ -2999,9 +3019,7 @@ int mdn_run_batched(mdn_ctx *c, int32_t nsteps, int32_t eflag_every, int64_t ste
         cap = std::max(cap, bc.cap);
       }
       HIPCHK(hipEventRecord(c->evk, c->stream2));
-      HIPCHK(hipStreamWaitEvent(c->stream, c->evk, 0));
       HIPCHK(hipEventRecord(c->evl, c->stream3));
-      HIPCHK(hipStreamWaitEvent(c->stream, c->evl, 0));
       c->last_cap = cap;
       c->last_lds = (int)lds;
     } else {
@@ -3022,11 +3040,26 @@ int mdn_run_batched(mdn_ctx *c, int32_t nsteps, int32_t eflag_every, int64_t ste
     c->last_classes[3] = H + NB_ + NA > 0 ? 0 : n;
     c->last_order = order;
     HIPCHK(hipEventRecord(c->ev1, c->stream));
-    HIPCHK(hipMemcpyAsync(c->h_dyn, c->d_dyn, sizeof(Dyn) * n, hipMemcpyDeviceToHost, c->stream));
-    float ms = 0;
+    // the chunk's end: each stream synchronized (no joins queued behind the
+    // kernels); the kernel time runs from ev0 to the last of the three ends
     HIPCHK(hipStreamSynchronize(c->stream));
+    float ms = 0, m2 = 0;
     HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    if (H + NB_ + NA > 0) {
+      HIPCHK(hipStreamSynchronize(c->stream2));
+      HIPCHK(hipStreamSynchronize(c->stream3));
+      HIPCHK(hipEventElapsedTime(&m2, c->ev0, c->evk));
+      ms = std::max(ms, m2);
+      HIPCHK(hipEventElapsedTime(&m2, c->ev0, c->evl));
+      ms = std::max(ms, m2);
+    }
     c->last_ms += ms;
+    for (int r = 0; r < n; r++) {
+      const int4 q = c->h_cnt[r];
+      c->h_dyn[r].n_in_gas = q.x;
+      c->h_dyn[r].n_in_vap = q.y;
+      c->h_dyn[r].pending = q.z;
+    }
   }
   // replicas stopped inside a step (LDS capacity outgrown at a rebuild): the
   // per-call path finishes that step first
