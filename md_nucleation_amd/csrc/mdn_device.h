@@ -98,8 +98,6 @@ struct DevReplica {
   uint32_t key0, key1;
   const double *pair_coeff, *btype, *ctype, *dcoeff;
   const int *dmulti;
-  int n_long;
-  const int *long_pairs;
   int amber_words;              // words per row of amber_allow
   const uint32_t *amber_allow;  // [n_ion][amber_words]: bit j of row i = pair (i,j) not excluded
   double *mol_x, *mol_v;

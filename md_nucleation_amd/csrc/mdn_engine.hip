@@ -1852,7 +1852,7 @@ static int upload_all(mdn_ctx *c) {
   Layout lay;
   struct Offs {
     size_t sp[3][12];
-    size_t pc, bt, ct, dc, dm, lp, am;
+    size_t pc, bt, ct, dc, dm, am;
     size_t mol_x, mol_v, region, ax, av, af, aw, axo, ing, inv, gi, gv, gg, dyn, vlog;
   };
   std::vector<Offs> offs(c->reps.size());
@@ -1891,7 +1891,6 @@ static int upload_all(mdn_ctx *c) {
     o.dc = lay.vec(d.dcoeff, true);
     o.dm = lay.vec(d.dmulti, true);
     h.long_pairs = (d.flags & MDN_ION_AMBER) ? amber_long_pairs(d.sp[0]) : std::vector<int>();
-    o.lp = lay.vec(h.long_pairs, true);
     {
       int n = d.sp[0].natoms, W = (n + 31) / 32;
       std::vector<uint32_t> allow((size_t)std::max(1, n * W), 0u);
@@ -1995,7 +1994,6 @@ static int upload_all(mdn_ctx *c) {
     R.key1 = (uint32_t)(h.seed >> 32) ^ h.replica_id;
     R.pair_coeff = at<double>(B, o.pc); R.btype = at<double>(B, o.bt); R.ctype = at<double>(B, o.ct);
     R.dcoeff = at<double>(B, o.dc); R.dmulti = at<int>(B, o.dm);
-    R.n_long = (int)h.long_pairs.size() / 2; R.long_pairs = at<int>(B, o.lp);
     R.amber_words = (d.sp[0].natoms + 31) / 32; R.amber_allow = at<uint32_t>(B, o.am);
     R.mol_x = at<double>(B, o.mol_x); R.mol_v = at<double>(B, o.mol_v); R.region = at<uint8_t>(B, o.region);
     R.atom_x = at<double>(B, o.ax); R.atom_v = at<double>(B, o.av); R.atom_f = at<double>(B, o.af);
@@ -2090,7 +2088,10 @@ __global__ void __launch_bounds__(kBlk) k_force_jvp(const DevReplica *__restrict
   if (bits & B_GG) term_gg<Dual>(R, s, t, nt, false, e);
   if (bits & B_GV) term_gv<Dual>(R, s, t, nt, false, e);
   if (bits & B_VV) term_vv_jvp_rows(R, s, t >> 6, nt >> 6, t & 63);
-  if (bits & B_AMBER) term_amber<Dual>(R, s, t, nt, false, e);
+  if (bits & B_AMBER) {
+    term_amber_long_rows<Dual>(R, s, t >> 6, nt >> 6, t & 63, false, e);
+    term_amber_bonded(R, s, t, nt, false, e);
+  }
   if (bits & B_VINTRA) term_vintra(R, s, t, nt, false, e);
   if (bits & B_GINTRA) term_gintra(R, s, t, nt, false, e);
 }

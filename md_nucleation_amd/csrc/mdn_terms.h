@@ -104,6 +104,27 @@ struct JRowStore : JStore {
   }
 };
 
+// the same for the forces themselves
+struct GRowStore : GStore {
+  int row;
+  double *acc;  // [3]
+  __device__ GRowStore(const GStore &b, int r, double *a) : GStore(b), row(r), acc(a) {}
+  __device__ void fadd(const H &q, double fx, double fy, double fz) const {
+    if (q.g == row) { acc[0] += fx; acc[1] += fy; acc[2] += fz; }
+  }
+};
+__device__ inline GRowStore row_store(const GStore &s, int r, double *a) { return GRowStore(s, r, a); }
+__device__ inline JRowStore row_store(const JStore &s, int r, double *a) { return JRowStore(s, r, a); }
+// one atom's summed row added to the store's output (one lane)
+__device__ inline void row_flush(const GStore &s, const GStore::H &h, const double *a) {
+  double *f = s.R->atom_f + 3 * (size_t)h.g;
+  atomicAdd(f, a[0]); atomicAdd(f + 1, a[1]); atomicAdd(f + 2, a[2]);
+}
+__device__ inline void row_flush(const JStore &s, const GStore::H &h, const double *a) {
+  double *f = s.hu + 3 * (size_t)h.g;
+  atomicAdd(f, a[0]); atomicAdd(f + 1, a[1]); atomicAdd(f + 2, a[2]);
+}
+
 __device__ inline void pc(const DevReplica &R, int t1, int t2, double &c0, double &c1) {
   const double *p = R.pair_coeff + 2 * ((size_t)t1 * R.n_types + t2);
   c0 = p[0]; c1 = p[1];
@@ -293,11 +314,29 @@ __device__ void term_amber_bonded(const DevReplica &R, const S &s, int t, int nt
     else dihedral_term(R, s, 0, I.dih + 5 * (p - nba), eflag, e[U_ION]);
   }
 }
+// The AMBER non-bonded pairs (compute_AMBER.cpp:14-50 over the list of
+// initialAMBER) over global memory, i-centric: wave w of nw takes ion atoms
+// i = w, w + nw, ..., its lanes the partners j that row i of the exclusion
+// bitmask allows; the force on i is summed in registers and added with one
+// atomic per component (a pair list walked pair by pair would add six
+// atomics per pair onto the ion's few hundred force slots).  Every pair is
+// met from both sides, so its energy is halved.
 template <typename Rl, class S>
-__device__ void term_amber(const DevReplica &R, const S &s, int t, int nt, bool eflag, double *e) {
-  for (int p = t; p < R.n_long; p += nt)
-    pair_ab<Rl, true>(R, s, s.h(R.long_pairs[2 * p]), s.h(R.long_pairs[2 * p + 1]), false, eflag, e[U_ION]);
-  term_amber_bonded(R, s, t, nt, eflag, e);
+__device__ void term_amber_long_rows(const DevReplica &R, const S &s, int w, int nw, int lane, bool eflag,
+                                     double *e) {
+  const int n = R.sp[0].natoms, W = R.amber_words;
+  double eh = 0;
+  for (int i = w; i < n; i += nw) {
+    double acc[3] = {0, 0, 0};
+    const typename S::H hi = s.h(i);
+    const auto rs = row_store(s, hi.g, acc);
+    const uint32_t *row = R.amber_allow + (size_t)i * W;
+    for (int j = lane; j - lane < n; j += 64)
+      if (j < n && ((row[j >> 5] >> (j & 31)) & 1u)) pair_ab<Rl, true>(R, rs, hi, rs.h(j), false, eflag, eh);
+    for (int q = 0; q < 3; q++) acc[q] = wave_sum_full(acc[q]);
+    if (lane == 0) row_flush(s, hi, acc);
+  }
+  if (eflag) e[U_ION] += 0.5 * eh;
 }
 
 // ---- vapor intra: PotentialVaporIntra::compute, compute_intra.cpp:45-245
@@ -592,7 +631,11 @@ __device__ void all_terms(const DevReplica &R, const S &s, uint32_t bits, int t,
   if (bits & B_EFIELD) term_efield(R, s, at(L.n_ion), nt);
   if (bits & B_TERSOFF) term_tersoff(R, s, at(0), nt, eflag, e);
   if (bits & B_SW) term_sw(R, s, at(0), nt, eflag, e);
-  if (bits & B_AMBER) term_amber<Rl>(R, s, at(R.n_long + I.nbond + I.nangle + I.ndih), nt, eflag, e);
+  if (bits & B_AMBER) {
+    const int ta = at(64 * I.natoms);
+    term_amber_long_rows<Rl>(R, s, ta >> 6, nt >> 6, ta & 63, eflag, e);
+    term_amber_bonded(R, s, at(I.nbond + I.nangle + I.ndih), nt, eflag, e);
+  }
   if (bits & B_BMH) term_born(R, s, at(0), nt, eflag, e);
   if (bits & B_VINTRA) term_vintra(R, s, at(L.nv * (V.nbond + V.nangle + V.ndih)), nt, eflag, e);
   if (bits & B_GINTRA) term_gintra(R, s, at(L.ng * G.nbond), nt, eflag, e);
