@@ -162,7 +162,14 @@ def test_gradient_vs_gpu_finite_differences(seg):
 
 def test_gradient_vs_oracle_trajectory(seg):
     """the reference-derived gradient: central differences of the oracle's
-    own trajectory loss (its MD::verlet, its pair table)"""
+    own trajectory loss (its MD::verlet, its pair table) with h = 1e-6.  The
+    two perturbed 100-step trajectories round differently, and the state
+    they start from (100 FP64 warm-up steps whose forces sum by atomics in
+    no fixed order) differs between runs in the last bits, so this
+    difference quotient moves by up to about 1.2e-9 from run to run (σ of
+    atom 42: 0.00310814838, ...81278, ...79786, ...92241 against a reverse
+    gradient stable to 1e-15); the bound, 3e-7 of the largest component
+    (3.6e-9 for σ), sits above that noise"""
     sd, st1, s, x0, v0, theta, tgt = seg
     _, g = SG.loss_and_grad(s, theta, x0, v0, T, tgt, OBS, every=EVERY)
     N = s.idx.size
@@ -175,7 +182,7 @@ def test_gradient_vs_oracle_trajectory(seg):
             vals.append(sum(float(((xo[s.idx] - tgt[(k + 1) * EVERY]) ** 2).sum()) / N for k, xo in enumerate(ox)))
         fd = (vals[0] - vals[1]) / (2 * H)
         print("param", p, "atom", i, "reverse", g[p][i], "oracle fd", fd)
-        assert abs(g[p][i] - fd) <= 1e-7 * np.abs(g[p]).max()
+        assert abs(g[p][i] - fd) <= 3e-7 * np.abs(g[p]).max()
 
 
 def test_checkpoint_spacing(seg):
