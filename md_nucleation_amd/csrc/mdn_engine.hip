@@ -3205,11 +3205,53 @@ __global__ void k_seg_kick(int n, const double *__restrict__ v, const double *__
   out[t] = v[t] + c * F[t] / m[t / 3];
 }
 
-// drift: out = x + dt vh
-__global__ void k_seg_drift(int n3, const double *__restrict__ x, const double *__restrict__ vh, double dt,
-                            double *__restrict__ out) {
+// a step's first half: vh = v + (c F) / m and xn = x + dt vh, xn scattered
+// into the replica's positions ax (global atom order), and the replica's
+// forces af (n_af values) cleared for the force kernels that follow (one
+// launch instead of four)
+__global__ void k_seg_kick_drift(int n, const double *__restrict__ v, const double *__restrict__ F,
+                                 const double *__restrict__ m, double c, const double *__restrict__ x, double dt,
+                                 double *__restrict__ vh, double *__restrict__ xn, const int *__restrict__ idx,
+                                 double *__restrict__ ax, double *__restrict__ af, int n_af) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < n3) out[t] = x[t] + dt * vh[t];
+  if (t < 3 * n) {
+    const int a = t / 3, q = t - 3 * a;
+    const double u = v[t] + c * F[t] / m[a];
+    const double y = x[t] + dt * u;
+    vh[t] = u;
+    xn[t] = y;
+    ax[3 * (size_t)idx[a] + q] = y;
+  }
+  if (t < n_af) af[t] = 0.0;
+}
+
+// a step's second half: F += the replica's forces af (gathered from global
+// atom order), then vn = vh + (c F) / m
+__global__ void k_seg_gather_kick(int n, const int *__restrict__ idx, const double *__restrict__ af,
+                                  double *__restrict__ F, const double *__restrict__ vh,
+                                  const double *__restrict__ m, double c, double *__restrict__ vn) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 3 * n) return;
+  const int a = t / 3, q = t - 3 * a;
+  const double f = F[t] + af[3 * (size_t)idx[a] + q];
+  F[t] = f;
+  vn[t] = vh[t] + c * f / m[a];
+}
+
+// an HVP's set-up: x into the replica's positions, u into the direction
+// array gu (both global atom order), the result array ghu (n_h values)
+// cleared
+__global__ void k_seg_hvp_setup(const int *__restrict__ idx, int n, const double *__restrict__ x,
+                                double *__restrict__ ax, const double *__restrict__ u, double *__restrict__ gu,
+                                double *__restrict__ ghu, int n_h) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n_h) ghu[t] = 0.0;
+  if (t < 3 * n) {
+    const int a = t / 3, q = t - 3 * a;
+    const size_t g = 3 * (size_t)idx[a] + q;
+    ax[g] = x[t];
+    gu[g] = u[t];
+  }
 }
 
 // out[0] = sum of (x - y)^2 over n3 values, in a fixed order (one workgroup)
@@ -3300,8 +3342,14 @@ struct Segment {
   int forces(const double *x, double *F) {
     const int g = seg_grid(3 * (size_t)N);
     hipLaunchKernelGGL(k_seg_scatter, dim3(g), dim3(kSegBlk), 0, c->stream, idx, N, x, R.atom_x);
+    if (rest) HIPCHK(hipMemsetAsync(R.atom_f, 0, sizeof(double) * 3 * (size_t)n_atoms, c->stream));
+    return forces_staged(x, F, true);
+  }
+  // the same once x is in the replica's positions and its forces are clear;
+  // without gather, the replica's forces are left for the caller to add
+  int forces_staged(const double *x, double *F, bool gather) {
+    const int g = seg_grid(3 * (size_t)N);
     if (rest) {
-      HIPCHK(hipMemsetAsync(R.atom_f, 0, sizeof(double) * 3 * (size_t)n_atoms, c->stream));
       int rc = launch_terms<double>(c, *h, Rd, rest, false);
       if (rc) return rc;
     }
@@ -3311,7 +3359,7 @@ struct Segment {
     } else {
       HIPCHK(hipMemsetAsync(F, 0, sizeof(double) * 3 * (size_t)N, c->stream));
     }
-    if (rest) hipLaunchKernelGGL(k_seg_gather, dim3(g), dim3(kSegBlk), 0, c->stream, idx, N, R.atom_f, F, 1);
+    if (rest && gather) hipLaunchKernelGGL(k_seg_gather, dim3(g), dim3(kSegBlk), 0, c->stream, idx, N, R.atom_f, F, 1);
     HIPCHK(hipGetLastError());
     return MDN_OK;
   }
@@ -3326,9 +3374,9 @@ struct Segment {
     }
     const uint32_t bits = rest & ~(uint32_t)MDN_EFIELD;  // the Efield force does not depend on x
     if (bits) {
-      hipLaunchKernelGGL(k_seg_scatter, dim3(g), dim3(kSegBlk), 0, c->stream, idx, N, x, R.atom_x);
-      hipLaunchKernelGGL(k_seg_scatter, dim3(g), dim3(kSegBlk), 0, c->stream, idx, N, u, gU);
-      HIPCHK(hipMemsetAsync(gHU, 0, sizeof(double) * 3 * (size_t)n_atoms, c->stream));
+      const int n_h = 3 * n_atoms, gs = seg_grid(std::max<size_t>(3 * (size_t)N, (size_t)n_h));
+      hipLaunchKernelGGL(k_seg_hvp_setup, dim3(gs), dim3(kSegBlk), 0, c->stream, idx, N, x, R.atom_x, u, gU, gHU,
+                         n_h);
       hipLaunchKernelGGL(k_force_jvp, dim3(512), dim3(kBlk), 0, c->stream, Rd, bits, gU, gHU);
       hipLaunchKernelGGL(k_seg_gather, dim3(g), dim3(kSegBlk), 0, c->stream, idx, N, gHU, hu, 1);
     }
@@ -3338,11 +3386,16 @@ struct Segment {
   // one MD::verlet step: (x, v, F) -> (xn, vn, F); vh is scratch
   int step(const double *x, const double *v, double *F, double *vh, double *xn, double *vn, double cc, double dt) {
     const int g = seg_grid(3 * (size_t)N);
-    hipLaunchKernelGGL(k_seg_kick, dim3(g), dim3(kSegBlk), 0, c->stream, N, v, F, mass, cc, vh);
-    hipLaunchKernelGGL(k_seg_drift, dim3(g), dim3(kSegBlk), 0, c->stream, 3 * N, x, vh, dt, xn);
-    int rc = forces(xn, F);
+    const int n_af = rest ? 3 * n_atoms : 0, gk = seg_grid(std::max<size_t>(3 * (size_t)N, (size_t)n_af));
+    hipLaunchKernelGGL(k_seg_kick_drift, dim3(gk), dim3(kSegBlk), 0, c->stream, N, v, F, mass, cc, x, dt, vh, xn, idx,
+                       R.atom_x, R.atom_f, n_af);
+    int rc = forces_staged(xn, F, false);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_seg_kick, dim3(g), dim3(kSegBlk), 0, c->stream, N, vh, F, mass, cc, vn);
+    if (rest)
+      hipLaunchKernelGGL(k_seg_gather_kick, dim3(g), dim3(kSegBlk), 0, c->stream, N, idx, R.atom_f, F, vh, mass, cc,
+                         vn);
+    else
+      hipLaunchKernelGGL(k_seg_kick, dim3(g), dim3(kSegBlk), 0, c->stream, N, vh, F, mass, cc, vn);
     HIPCHK(hipGetLastError());
     return MDN_OK;
   }
