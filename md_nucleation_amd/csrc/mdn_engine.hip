@@ -2059,8 +2059,12 @@ static int launch_terms(mdn_ctx *c, const HostReplica &h, const DevReplica *R, u
       if (eflag) hipLaunchKernelGGL((k_vv_n3<Rl, true>), dim3(gx), dim3(kVVSuper), 0, c->stream, R, tv0, ntv);
       else hipLaunchKernelGGL((k_vv_n3<Rl, false>), dim3(gx), dim3(kVVSuper), 0, c->stream, R, tv0, ntv);
     } else {
+      // the partner range split gy ways for about four workgroups per CU,
+      // at most 32: each split adds one FP64 atomic per atom and component
+      // onto the same force slots (for a 600-atom vapor set, 32 splits
+      // instead of 64 shortened a 100-step segment gradient by 3 %)
       const int gx = std::max(1, (nva + kVVTile - 1) / kVVTile);
-      const int gy = std::max(1, std::min(64, (4 * c->n_cu + gx - 1) / gx));
+      const int gy = std::max(1, std::min(32, (4 * c->n_cu + gx - 1) / gx));
       if (eflag) hipLaunchKernelGGL((k_vv_tiled<Rl, true>), dim3(gx, gy), dim3(kVVTile), 0, c->stream, R, tv0, ntv);
       else hipLaunchKernelGGL((k_vv_tiled<Rl, false>), dim3(gx, gy), dim3(kVVTile), 0, c->stream, R, tv0, ntv);
     }
