@@ -2087,17 +2087,33 @@ __global__ void __launch_bounds__(kBlk) k_force_jvp(const DevReplica *__restrict
   s.hu = hu;
   const int t = blockIdx.x * blockDim.x + threadIdx.x, nt = gridDim.x * blockDim.x;
   double e[U_N];
-  if (bits & B_GI) term_gi<Dual>(R, s, t, nt, false, e);
-  if (bits & B_VI) term_vi<Dual>(R, s, t, nt, false, e);
-  if (bits & B_GG) term_gg<Dual>(R, s, t, nt, false, e);
-  if (bits & B_GV) term_gv<Dual>(R, s, t, nt, false, e);
-  if (bits & B_VV) term_vv_jvp_rows(R, s, t >> 6, nt >> 6, t & 63);
-  if (bits & B_AMBER) {
-    term_amber_long_rows<Dual>(R, s, t >> 6, nt >> 6, t & 63, false, e);
-    term_amber_bonded(R, s, t, nt, false, e);
+  // each term's items start where the previous term's ended (modulo the
+  // grid, in whole waves), so that the terms run side by side rather than
+  // one after another on the first waves
+  const LView &L = s.L;
+  int off = 0;
+  auto at = [&](int items) {
+    const int tt = t - off;
+    off = (int)((off + ((long long)items + 63) / 64 * 64) % nt);
+    return tt < 0 ? tt + nt : tt;
+  };
+  const DevSpecies &I = R.sp[0], &G = R.sp[1], &V = R.sp[2];
+  const int nj = L.na_g < L.na_v ? L.na_g : L.na_v;
+  if (bits & B_GI) term_gi<Dual>(R, s, at(L.n_gi * L.na_g), nt, false, e);
+  if (bits & B_VI) term_vi<Dual>(R, s, at(L.nv * L.na_v * L.n_ion), nt, false, e);
+  if (bits & B_GG) term_gg<Dual>(R, s, at(L.n_gg * L.na_g * L.na_g), nt, false, e);
+  if (bits & B_GV) term_gv<Dual>(R, s, at(L.n_gv * L.na_g * nj), nt, false, e);
+  if (bits & B_VV) {
+    const int tv = at(64 * L.nv * L.na_v);
+    term_vv_jvp_rows(R, s, tv >> 6, nt >> 6, tv & 63);
   }
-  if (bits & B_VINTRA) term_vintra(R, s, t, nt, false, e);
-  if (bits & B_GINTRA) term_gintra(R, s, t, nt, false, e);
+  if (bits & B_AMBER) {
+    const int ta = at(64 * I.natoms);
+    term_amber_long_rows<Dual>(R, s, ta >> 6, nt >> 6, ta & 63, false, e);
+    term_amber_bonded(R, s, at(I.nbond + I.nangle + I.ndih), nt, false, e);
+  }
+  if (bits & B_VINTRA) term_vintra(R, s, at(L.nv * (V.nbond + V.nangle + V.ndih)), nt, false, e);
+  if (bits & B_GINTRA) term_gintra(R, s, at(L.ng * G.nbond), nt, false, e);
 }
 
 // start of the rest of a step the batched kernel left unfinished: clears the
