@@ -78,9 +78,10 @@ __device__ inline double team_sum(double v, int T) {
   return v;
 }
 
-// the team's (snapshot, atom) index and lane; false for a team past n
-__device__ inline bool team_of(int n, int T, int &k, int &lane) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+// the team's (snapshot, atom) index and lane in workgroup bx of a side's
+// grid; false for a team past n
+__device__ inline bool team_of(int n, int T, int bx, int &k, int &lane) {
+  const int t = bx * blockDim.x + threadIdx.x;
   k = t / T;
   lane = t - k * T;
   return k < n;
@@ -90,9 +91,9 @@ __device__ inline bool team_of(int n, int T, int &k, int &lane) {
 // Hessian of -E applied to u): with d = x_j - x_i and du = u_j - u_i,
 //   dF_j = f du + 2 (d . du) (df/dr^2) d,   dF_i = -dF_j
 // ion side: one team per (snapshot, ion atom)
-__global__ void k_vi_jvp_ion(ViBatch b, const double *U, double *HU, int T) {
+__device__ void vi_jvp_ion(const ViBatch &b, const double *U, double *HU, int T, int bx) {
   int k, lane;
-  const bool on = team_of(b.S * b.n_ion, T, k, lane);
+  const bool on = team_of(b.S * b.n_ion, T, bx, k, lane);
   const int s = on ? k / b.n_ion : 0, i = on ? k - s * b.n_ion : 0, n = b.n_ion + b.n_vap;
   const double *X = b.x + (size_t)s * n * 3, *W = b.w + (size_t)s * n, *Us = U + (size_t)s * n * 3;
   const double xi = X[3 * i], yi = X[3 * i + 1], zi = X[3 * i + 2];
@@ -117,9 +118,9 @@ __global__ void k_vi_jvp_ion(ViBatch b, const double *U, double *HU, int T) {
 }
 
 // vapor side: one team per (snapshot, vapor atom)
-__global__ void k_vi_jvp_vap(ViBatch b, const double *U, double *HU, int T) {
+__device__ void vi_jvp_vap(const ViBatch &b, const double *U, double *HU, int T, int bx) {
   int k, lane;
-  const bool on = team_of(b.S * b.n_vap, T, k, lane);
+  const bool on = team_of(b.S * b.n_vap, T, bx, k, lane);
   const int s = on ? k / b.n_vap : 0, j = on ? k - s * b.n_vap : 0, n = b.n_ion + b.n_vap, a = b.n_ion + j;
   const double *X = b.x + (size_t)s * n * 3, *W = b.w + (size_t)s * n, *Us = U + (size_t)s * n * 3;
   const double xj = X[3 * a], yj = X[3 * a + 1], zj = X[3 * a + 2];
@@ -144,9 +145,9 @@ __global__ void k_vi_jvp_vap(ViBatch b, const double *U, double *HU, int T) {
 
 // forward, ion side: one team per (snapshot, ion atom): F_i and the
 // snapshot's energy partial of atom i
-__global__ void k_vi_fwd_ion(ViBatch b, double *F, double *Epart, int T) {
+__device__ void vi_fwd_ion(const ViBatch &b, double *F, double *Epart, int T, int bx) {
   int k, lane;
-  const bool on = team_of(b.S * b.n_ion, T, k, lane);
+  const bool on = team_of(b.S * b.n_ion, T, bx, k, lane);
   const int s = on ? k / b.n_ion : 0, i = on ? k - s * b.n_ion : 0, n = b.n_ion + b.n_vap;
   const double *X = b.x + (size_t)s * n * 3, *W = b.w + (size_t)s * n;
   const double xi = X[3 * i], yi = X[3 * i + 1], zi = X[3 * i + 2];
@@ -169,9 +170,9 @@ __global__ void k_vi_fwd_ion(ViBatch b, double *F, double *Epart, int T) {
 }
 
 // forward, vapor side: one team per (snapshot, vapor atom): F_j
-__global__ void k_vi_fwd_vap(ViBatch b, double *F, int T) {
+__device__ void vi_fwd_vap(const ViBatch &b, double *F, int T, int bx) {
   int k, lane;
-  const bool on = team_of(b.S * b.n_vap, T, k, lane);
+  const bool on = team_of(b.S * b.n_vap, T, bx, k, lane);
   const int s = on ? k / b.n_vap : 0, j = on ? k - s * b.n_vap : 0, n = b.n_ion + b.n_vap, a = b.n_ion + j;
   const double *X = b.x + (size_t)s * n * 3, *W = b.w + (size_t)s * n;
   const double xj = X[3 * a], yj = X[3 * a + 1], zj = X[3 * a + 2];
@@ -190,6 +191,18 @@ __global__ void k_vi_fwd_vap(ViBatch b, double *F, int T) {
   }
 }
 
+// The ion and vapor sides in one launch (they write the forces of disjoint
+// atoms): workgroups [0, gi) take the ion side, the rest the vapor side, so
+// that the two run side by side rather than one after the other.
+__global__ void k_vi_fwd(ViBatch b, double *F, double *Epart, int Ti, int Tv, int gi) {
+  if ((int)blockIdx.x < gi) vi_fwd_ion(b, F, Epart, Ti, blockIdx.x);
+  else vi_fwd_vap(b, F, Tv, blockIdx.x - gi);
+}
+__global__ void k_vi_jvp(ViBatch b, const double *U, double *HU, int Ti, int Tv, int gi) {
+  if ((int)blockIdx.x < gi) vi_jvp_ion(b, U, HU, Ti, blockIdx.x);
+  else vi_jvp_vap(b, U, HU, Tv, blockIdx.x - gi);
+}
+
 // E[s] = sum over ion atoms of the partials (fixed order)
 __global__ void k_vi_sum_e(const double *Epart, int S, int n_ion, double *E) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -203,7 +216,7 @@ __global__ void k_vi_sum_e(const double *Epart, int S, int n_ion, double *E) {
 // to dL/d(sigma_i, eps_i, q_i)
 __global__ void k_vi_adj(ViBatch b, const double *gE, const double *gF, double *part, int T) {
   int k, lane;
-  const bool on = team_of(b.S * b.n_ion, T, k, lane);
+  const bool on = team_of(b.S * b.n_ion, T, blockIdx.x, k, lane);
   const int s = on ? k / b.n_ion : 0, i = on ? k - s * b.n_ion : 0, n = b.n_ion + b.n_vap;
   const double *X = b.x + (size_t)s * n * 3, *W = b.w + (size_t)s * n;
   const double *G = gF ? gF + (size_t)s * n * 3 : nullptr;
@@ -258,8 +271,8 @@ static dim3 team_grid(int teams, int T) { return dim3((unsigned)(((size_t)teams 
 int vi_forward_dev(const ViBatch &b, double *F, double *Epart, double *E, void *stream) {
   hipStream_t st = static_cast<hipStream_t>(stream);
   const int ni = b.S * b.n_ion, nv = b.S * b.n_vap, Ti = team_size(b.n_vap), Tv = team_size(b.n_ion);
-  hipLaunchKernelGGL(k_vi_fwd_ion, team_grid(ni, Ti), dim3(256), 0, st, b, F, Epart, Ti);
-  if (nv) hipLaunchKernelGGL(k_vi_fwd_vap, team_grid(nv, Tv), dim3(256), 0, st, b, F, Tv);
+  const int gi = (int)team_grid(ni, Ti).x, gv = nv ? (int)team_grid(nv, Tv).x : 0;
+  hipLaunchKernelGGL(k_vi_fwd, dim3(gi + gv), dim3(256), 0, st, b, F, Epart, Ti, Tv, gi);
   if (E) hipLaunchKernelGGL(k_vi_sum_e, dim3((b.S + 255) / 256), dim3(256), 0, st, Epart, b.S, b.n_ion, E);
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? MDN_OK : set_error(MDN_E_HIP, std::string("vi forward: ") + hipGetErrorString(e));
@@ -268,8 +281,8 @@ int vi_forward_dev(const ViBatch &b, double *F, double *Epart, double *E, void *
 int vi_jvp_dev(const ViBatch &b, const double *U, double *HU, void *stream) {
   hipStream_t st = static_cast<hipStream_t>(stream);
   const int ni = b.S * b.n_ion, nv = b.S * b.n_vap, Ti = team_size(b.n_vap), Tv = team_size(b.n_ion);
-  hipLaunchKernelGGL(k_vi_jvp_ion, team_grid(ni, Ti), dim3(256), 0, st, b, U, HU, Ti);
-  if (nv) hipLaunchKernelGGL(k_vi_jvp_vap, team_grid(nv, Tv), dim3(256), 0, st, b, U, HU, Tv);
+  const int gi = (int)team_grid(ni, Ti).x, gv = nv ? (int)team_grid(nv, Tv).x : 0;
+  hipLaunchKernelGGL(k_vi_jvp, dim3(gi + gv), dim3(256), 0, st, b, U, HU, Ti, Tv, gi);
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? MDN_OK : set_error(MDN_E_HIP, std::string("vi jvp: ") + hipGetErrorString(e));
 }
@@ -360,8 +373,8 @@ int mdn_vi_forward(const double *x, const double *w, int32_t S, int32_t n_ion, i
   if ((rc = db.alloc(S, &dE)) || (rc = db.alloc((size_t)S * n * 3, &dF)) || (rc = db.alloc((size_t)S * n_ion, &work)))
     return rc;
   int ni = S * n_ion, nv = S * n_vap, Ti = team_size(n_vap), Tv = team_size(n_ion);
-  hipLaunchKernelGGL(k_vi_fwd_ion, team_grid(ni, Ti), dim3(256), 0, 0, b, dF, work, Ti);
-  if (nv) hipLaunchKernelGGL(k_vi_fwd_vap, team_grid(nv, Tv), dim3(256), 0, 0, b, dF, Tv);
+  const int gi = (int)team_grid(ni, Ti).x, gv = nv ? (int)team_grid(nv, Tv).x : 0;
+  hipLaunchKernelGGL(k_vi_fwd, dim3(gi + gv), dim3(256), 0, 0, b, dF, work, Ti, Tv, gi);
   hipLaunchKernelGGL(k_vi_sum_e, dim3((S + 255) / 256), dim3(256), 0, 0, work, S, n_ion, dE);
   if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess)
     return set_error(MDN_E_HIP, "mdn_vi_forward: kernel failed");
@@ -387,8 +400,8 @@ int mdn_vi_jvp(const double *x, const double *w, int32_t S, int32_t n_ion, int32
   double *dH = nullptr;
   if ((rc = db.up(u, (size_t)S * n * 3, &dU)) || (rc = db.alloc((size_t)S * n * 3, &dH))) return rc;
   int ni = S * n_ion, nv = S * n_vap, Ti = team_size(n_vap), Tv = team_size(n_ion);
-  hipLaunchKernelGGL(k_vi_jvp_ion, team_grid(ni, Ti), dim3(256), 0, 0, b, dU, dH, Ti);
-  if (nv) hipLaunchKernelGGL(k_vi_jvp_vap, team_grid(nv, Tv), dim3(256), 0, 0, b, dU, dH, Tv);
+  const int gi = (int)team_grid(ni, Ti).x, gv = nv ? (int)team_grid(nv, Tv).x : 0;
+  hipLaunchKernelGGL(k_vi_jvp, dim3(gi + gv), dim3(256), 0, 0, b, dU, dH, Ti, Tv, gi);
   if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess)
     return set_error(MDN_E_HIP, "mdn_vi_jvp: kernel failed");
   if (hipMemcpy(hu, dH, sizeof(double) * S * n * 3, hipMemcpyDeviceToHost) != hipSuccess)
