@@ -367,6 +367,7 @@ __global__ void __launch_bounds__(kBlk) k_terms(const DevReplica *__restrict__ R
   if (eflag) flush_energy(R, e);
 }
 
+
 // every lane takes the value of lane (l + 1) mod 64 (DPP wave_rol:1; all 64
 // lanes must be active)
 __device__ inline double wave_rol1(double v) {
@@ -394,8 +395,9 @@ template <> struct Pack4<double> { typedef double4 T; };
 // values as dividing per pair).
 
 constexpr int kVVTile = 256;
+// workgroup (bx, by) of a gx x gy grid
 template <typename Rl, bool EF>
-__global__ void __launch_bounds__(kVVTile) k_vv_tiled(const DevReplica *__restrict__ Rp, int tv0, int ntv) {
+__device__ void vv_tiled(const DevReplica *__restrict__ Rp, int tv0, int ntv, int bx, int by, int gy) {
   const DevReplica &R = *Rp;
   __shared__ double sx[kVVTile * 3];
   __shared__ Rl sq[kVVTile], sw[kVVTile];
@@ -410,7 +412,7 @@ __global__ void __launch_bounds__(kVVTile) k_vv_tiled(const DevReplica *__restri
     const Rl c0 = (Rl)p[0], c1 = (Rl)p[1];
     sc[k] = {c0, c1, c0 / Rl(12), c1 / Rl(6)};
   }
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = bx * blockDim.x + threadIdx.x;
   const bool act = i < nva;
   double xi[3] = {0, 0, 0}, qi = 0, wi = 0;
   int ti = 0, mi = -1, gi = 0;
@@ -427,8 +429,8 @@ __global__ void __launch_bounds__(kVVTile) k_vv_tiled(const DevReplica *__restri
   const int trow = ti * ntv;
   // charge (times qqrd2e) and weight products in pair precision
   const Rl qr = (Rl)qi, wr = (Rl)wi;
-  int per = (nva + gridDim.y - 1) / gridDim.y;
-  int j0 = blockIdx.y * per, j1 = min(nva, j0 + per);
+  int per = (nva + gy - 1) / gy;
+  int j0 = by * per, j1 = min(nva, j0 + per);
   double fx = 0, fy = 0, fz = 0, e = 0;
   // one partner b; valid: b is a real entry of the tile (uniform)
   auto pair = [&](int b, bool valid) {
@@ -487,6 +489,33 @@ __global__ void __launch_bounds__(kVVTile) k_vv_tiled(const DevReplica *__restri
   if (EF) {
     double s = wave_sum(0.5 * e);
     if ((threadIdx.x & 63) == 0 && s != 0.0) atomicAdd(&R.dyn->U[U_VV], s);
+  }
+}
+
+template <typename Rl, bool EF>
+__global__ void __launch_bounds__(kVVTile) k_vv_tiled(const DevReplica *__restrict__ Rp, int tv0, int ntv) {
+  vv_tiled<Rl, EF>(Rp, tv0, ntv, blockIdx.x, blockIdx.y, gridDim.y);
+}
+
+// k_terms on workgroups [0, kTermsGrid) and the tiled vv (a gx x gy grid,
+// row-major) on the rest, in one launch: both only add into the force
+// array, so they run side by side rather than one after the other
+constexpr int kTermsGrid = 512;
+template <typename Rl, bool EF>
+__global__ void __launch_bounds__(kBlk) k_terms_vv(const DevReplica *__restrict__ Rp, uint32_t bits, int tv0, int ntv,
+                                                   int gx, int gy) {
+  if ((int)blockIdx.x < kTermsGrid) {
+    const DevReplica &R = *Rp;
+    GStore s;
+    s.R = Rp;
+    s.L = view_from_dyn(R);
+    const int t = blockIdx.x * blockDim.x + threadIdx.x, nt = kTermsGrid * kBlk;
+    double e[U_N] = {0, 0, 0, 0, 0, 0, 0, 0};
+    all_terms<Rl>(R, s, bits, t, nt, EF, e);
+    if (EF) flush_energy(R, e);
+  } else {
+    const int b = blockIdx.x - kTermsGrid;
+    vv_tiled<Rl, EF>(Rp, tv0, ntv, b % gx, b / gx, gy);
   }
 }
 
@@ -2048,7 +2077,20 @@ static int launch_terms(mdn_ctx *c, const HostReplica &h, const DevReplica *R, u
   int ntv = d.sp[2].natoms ? *std::max_element(d.sp[2].type.begin(), d.sp[2].type.end()) - tv0 + 1 : 1;
   bool tiled = ntv <= 8;
   uint32_t rest = tiled ? bits & ~(uint32_t)MDN_VV : bits;
-  if (rest) hipLaunchKernelGGL(k_terms<Rl>, dim3(512), dim3(kBlk), 0, c->stream, R, rest, eflag ? 1 : 0);
+  const int nva_b = d.n_vap * d.sp[2].natoms;  // bound on the in-domain vapor atoms
+  if (rest && (bits & MDN_VV) && tiled && nva_b < kVVNewtonMin) {
+    const int gx = std::max(1, (nva_b + kVVTile - 1) / kVVTile);
+    const int gy = std::max(1, std::min(32, (4 * c->n_cu + gx - 1) / gx));
+    if (eflag)
+      hipLaunchKernelGGL((k_terms_vv<Rl, true>), dim3(kTermsGrid + gx * gy), dim3(kBlk), 0, c->stream, R, rest, tv0, ntv,
+                         gx, gy);
+    else
+      hipLaunchKernelGGL((k_terms_vv<Rl, false>), dim3(kTermsGrid + gx * gy), dim3(kBlk), 0, c->stream, R, rest, tv0,
+                         ntv, gx, gy);
+    HIPCHK(hipGetLastError());
+    return MDN_OK;
+  }
+  if (rest) hipLaunchKernelGGL(k_terms<Rl>, dim3(kTermsGrid), dim3(kBlk), 0, c->stream, R, rest, eflag ? 1 : 0);
   if ((bits & MDN_VV) && tiled) {
     const int nva = d.n_vap * d.sp[2].natoms;  // bound on the in-domain vapor atoms
     if (nva >= kVVNewtonMin) {
